@@ -1,0 +1,248 @@
+/*
+ * otg.h -- C ABI of the MI355X-native RC-tree timing and levelization engine.
+ *
+ * Two data-parallel paths of OpenTimer are served here:
+ *   P1  per-net RC-tree Elmore analysis: load, delay, ldelay, beta, impulse
+ *       and output slew of every node for the 4 corners (MIN/MAX x RISE/FALL);
+ *   P2  sink-first levelization of the timing graph, plus the per-net BFS
+ *       ordering of RC trees that P1 consumes.
+ *
+ * Conventions shared by every entry point:
+ *   - corner index c = el*2 + rf: 0 MIN/RISE, 1 MIN/FALL, 2 MAX/RISE,
+ *     3 MAX/FALL (reference ot/headerdef.hpp:172-177); per-node quantities
+ *     are stored node-major, [node*4 + c] (reference ot/cuda/rct.cu:12-14);
+ *   - "BFS order": nodes of a net are stored so that every node comes after
+ *     its parent; the parent of position i is i - pid[i] (reference
+ *     ot/cuda/rct.cuh:36-37).  Any topological order is accepted by the
+ *     solver; otg_rct_bfs produces the reference's exact BFS order;
+ *   - units are the caller's: R*C must already be in the time unit of the
+ *     driver slew (the TAU/SPEF convention kOhm * fF = ps);
+ *   - every call returns an otg_status; on failure otg_last_error() holds a
+ *     message.  Entry points that take an otg_stream are asynchronous on that
+ *     stream (NULL = the context's own stream) and use device pointers; the
+ *     reference-shaped entry points (otg_rct_bfs_and_compute,
+ *     otg_toposort_compute) take host pointers and are synchronous, like the
+ *     functions they replace.
+ *   - a context is not thread-safe; calls that overlap on one context race.
+ */
+#ifndef OTG_H_
+#define OTG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OTG_NUM_CORNERS 4
+
+typedef enum otg_status {
+  OTG_OK = 0,
+  OTG_ERR_INVALID_ARGUMENT = 1, /* null pointer, negative size, bad enum   */
+  OTG_ERR_NOT_A_TREE = 2,       /* a net's edges do not span it as a tree  */
+  OTG_ERR_BAD_TOPOLOGY = 3,     /* pid points outside the net or forward   */
+  OTG_ERR_DEVICE = 4,           /* HIP runtime error                       */
+  OTG_ERR_OUT_OF_MEMORY = 5,
+  OTG_ERR_NO_DEVICE = 6,
+  OTG_ERR_CYCLE = 7,            /* levelization: a candidate pin on a cycle */
+  OTG_ERR_UNSUPPORTED = 8
+} otg_status;
+
+typedef struct otg_context otg_context;   /* device, stream, scratch, tuning */
+typedef struct otg_rct_plan otg_rct_plan; /* per-topology work list          */
+typedef void* otg_stream;                 /* a hipStream_t                   */
+
+/* Context.  No reference counterpart: the reference uses the implicit CUDA
+ * device 0 and the default stream (ot/cuda/utils.cuh:12-57). */
+otg_status otg_create(int device, otg_context** ctx);
+otg_status otg_destroy(otg_context* ctx);
+otg_status otg_synchronize(otg_context* ctx, otg_stream stream);
+const char* otg_status_string(otg_status s);
+const char* otg_last_error(const otg_context* ctx);
+int otg_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* P1: RC-tree Elmore                                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef enum otg_rct_method {
+  /* One thread per (net, corner), five serial sweeps per net, the exact
+   * arithmetic of compute_net_timing (reference ot/cuda/rct.cu:11-59). */
+  OTG_RCT_PLAIN = 0,
+  /* Size-binned persistent launch, FP32 arithmetic. */
+  OTG_RCT_BATCHED = 1,
+  /* Size-binned persistent launch, FP32 with compensated (two-sum)
+   * accumulation of every subtree and path sum; FP64 only in the final
+   * per-node combination that forms beta, impulse and slew. */
+  OTG_RCT_BATCHED_COMPENSATED = 2
+} otg_rct_method;
+
+/* A forest of RC trees in BFS (topological) order, device pointers.
+ * Mirrors the device-side fields of RctCUDA (reference ot/cuda/rct.cuh:7-13)
+ * plus the driver slew that the reference applies later per query
+ * (FlatRct::slew, ot/timer/net.cpp:280-283). */
+typedef struct otg_rct_tree {
+  int32_t num_nets;
+  int32_t total_num_nodes;
+  const int32_t* nodes_start; /* [num_nets+1]                                */
+  const int32_t* pid;         /* [V] parent of i is i-pid[i]; ignored at net  */
+                              /*     starts (the driver)                      */
+  const float* pres;          /* [V] resistance of the edge into node i       */
+  const float* cap;           /* [4V] node capacitance (ground + sink pin)    */
+  const float* slew_in;       /* [4*num_nets] driver slew; NULL = 0           */
+} otg_rct_tree;
+
+/* Outputs, device pointers, each [4V] in the tree's order.  A NULL field is
+ * not written, except `load`, which the batched methods also use as working
+ * storage when a net is larger than one on-chip window (so it must be given
+ * whenever the tree holds such nets). */
+typedef struct otg_rct_result {
+  float* load;    /* downstream capacitance (net.cpp:127-140)                */
+  float* delay;   /* Elmore delay (net.cpp:144-157)                          */
+  float* ldelay;  /* sum of cap*delay over the subtree (net.cpp:161-175)     */
+  float* beta;    /* second moment (net.cpp:179-191)                         */
+  float* impulse; /* 2*beta - delay^2 (net.cpp:191)                          */
+  float* slew;    /* sign(si)*sqrt(si^2 + impulse) (net.cpp:30-32)           */
+} otg_rct_result;
+
+/* Tuning knobs of the batched launch; zero fields take the defaults (or the
+ * values persisted by the autotuner, see otg_rct_autotune). */
+typedef struct otg_rct_tuning {
+  int32_t ring_slots;     /* on-chip window per wavefront: 256, 512 or 1024  */
+  int32_t waves_per_wg;   /* wavefronts per workgroup: 1, 2 or 4            */
+  int32_t small_target;   /* node budget of one packed group of small nets  */
+  int32_t grid_per_cu;    /* persistent workgroups per CU (0 = occupancy)   */
+} otg_rct_tuning;
+
+/* Build the work list of a topology (net sizes and parent offsets only):
+ * validates pid, bins nets (packed small groups / streamed large nets /
+ * global-window nets) and orders the queue largest-first.  The plan stays
+ * valid while nodes_start and pid do not change; R, C and slew may change
+ * freely between solves.  No reference counterpart (the reference rebuilds
+ * everything on every update, ot/cuda/rct.cu:361-397). */
+otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* topology,
+                               const otg_rct_tuning* tuning, otg_rct_method method,
+                               otg_rct_plan** plan);
+otg_status otg_rct_plan_destroy(otg_rct_plan* plan);
+/* Counts of the plan's bins: [0] packed groups, [1] nets in them,
+ * [2] streamed nets, [3] global-window nets, [4] max net size. */
+otg_status otg_rct_plan_stats(const otg_rct_plan* plan, int64_t stats[8]);
+
+/* Solve P1 for every net.  Replaces the compute stage of
+ * rct_bfs_and_compute_cuda (reference ot/cuda/rct.cu:385-389, kernel
+ * compute_net_timing rct.cu:11-59) on device-resident data; adds beta and
+ * slew.  plan may be NULL (then one is built and released inside the call,
+ * which allocates).  Asynchronous on `stream`. */
+otg_status otg_rct_solve(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tree* tree,
+                         const otg_rct_result* out, otg_rct_method method,
+                         otg_stream stream);
+
+/* Re-solve only the nets listed in net_ids (device array of `count` net
+ * indices, any order, no duplicates); other nets' outputs are untouched.
+ * Used by incremental re-timing.  Capturable in a hipGraph (no allocation,
+ * no memset, no host synchronisation).  No reference counterpart (the
+ * reference CUDA path re-flattens all nets, ot/timer/timer.cpp:931). */
+otg_status otg_rct_solve_subset(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tree* tree,
+                                const otg_rct_result* out, otg_rct_method method,
+                                const int32_t* net_ids, int32_t count, otg_stream stream);
+
+/* Undirected RC edges of a forest in original node numbering, device
+ * pointers (reference RctEdgeArrayCUDA inputs, ot/cuda/rct.cuh:20-28). */
+typedef struct otg_rct_edges {
+  int32_t num_nets;
+  int32_t total_num_nodes;
+  const int32_t* nodes_start; /* [N+1]                                        */
+  const int32_t* edges;       /* [2E] (s,t) net-local ids; edges of net k     */
+                              /*      start at nodes_start[k]-k, E = V-N      */
+  const int32_t* roots;       /* [N] net-local driver id                      */
+  const float* edge_res;      /* [E]                                          */
+  const float* node_cap;      /* [4V], original order                         */
+} otg_rct_edges;
+
+/* Per-net BFS ordering plus reordering of R and C into BFS order.
+ * Replaces kernels compute_net_bfs (reference ot/cuda/rct.cu:120-245, with
+ * block_couting_sort ot/cuda/sort.cuh:63-142) and prepare_res_cap
+ * (rct.cu:314-359).  Outputs (device): node2bfs[V] net-local BFS position of
+ * each node; pid[V] (0 at the driver); edges_directed[2E] (s = parent),
+ * may alias in->edges; pres[V] (0 at the driver); cap_bfs[4V].  Position of
+ * node u = #(nodes nearer the driver) + #(nodes at equal distance with a
+ * larger id).  Returns OTG_ERR_NOT_A_TREE if a net is not a spanning tree
+ * (checked on the device; the call synchronises to report it). */
+otg_status otg_rct_bfs(otg_context* ctx, const otg_rct_edges* in, int32_t* node2bfs,
+                       int32_t* pid, int32_t* edges_directed, float* pres, float* cap_bfs,
+                       otg_stream stream);
+
+/* Host-pointer drop-in for rct_bfs_and_compute_cuda(RctEdgeArrayCUDA)
+ * (reference ot/cuda/rct.cuh:45, rct.cu:361-397).  Same field meaning as
+ * RctEdgeArrayCUDA (rct.cuh:20-42); rct_edges is oriented in place;
+ * outputs are in BFS order.  beta, slew and slew_in are extensions (NULL
+ * skips them).  Synchronous; allocates and frees device memory per call. */
+typedef struct otg_rct_edge_array {
+  int32_t num_nets;
+  int32_t total_num_nodes;
+  int32_t total_num_edges;
+  int32_t* rct_nodes_start;    /* in  [N+1]  */
+  int32_t* rct_edges;          /* in/out [2E] */
+  int32_t* rct_roots;          /* in  [N]    */
+  float* rct_edges_res;        /* in  [E]    */
+  float* rct_nodes_cap;        /* in  [4V]   */
+  int32_t* rct_node2bfs_order; /* out [V]    */
+  int32_t* rct_pid;            /* out [V]    */
+  float* load;                 /* out [4V]   */
+  float* delay;                /* out [4V]   */
+  float* ldelay;               /* out [4V]   */
+  float* impulse;              /* out [4V]   */
+  float* beta;                 /* out [4V], optional */
+  float* slew;                 /* out [4V], optional */
+  const float* slew_in;        /* in  [4N], optional */
+} otg_rct_edge_array;
+
+otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_array* data,
+                                   otg_rct_method method);
+
+/* Autotuner: times candidate tunings of the batched launch on `tree`,
+ * keeps the fastest in the context (used by later plans) and, when
+ * json_path is non-NULL, writes it there.  No reference counterpart (the
+ * reference hard-codes chunk=64 and 128 threads, rct.cu:9,371). */
+otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tree,
+                            const otg_rct_result* scratch_out, otg_rct_method method,
+                            const char* json_path, otg_rct_tuning* best);
+otg_status otg_rct_tuning_load(otg_context* ctx, const char* json_path, otg_rct_tuning* t);
+
+/* ------------------------------------------------------------------------ */
+/* P2: levelization                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Host-pointer drop-in for toposort_compute_cuda (reference
+ * ot/cuda/toposort.cuh:3-6, toposort.cu:25-72), with C types instead of
+ * the std::vector& out-parameter.  edgelist_start[n+1]/edgelist[m] is the
+ * fanin CSR, out[n] the fanout count (not modified), frontiers[0:first_size)
+ * the initial frontier (pins without fanout); on return frontiers[n] holds
+ * the pins grouped by level (order inside a level is unspecified, as in the
+ * reference) and frontiers_ends[0..*num_ends) the running level ends,
+ * starting 0, first_size (timer.cpp:1061-1062).  frontiers_ends needs room
+ * for n+2 entries.  level_of_pin (optional, [n]) gets each pin's level or -1.
+ * Synchronous. */
+otg_status otg_toposort_compute(otg_context* ctx, int32_t n, int32_t num_edges,
+                                int32_t first_size, const int32_t* edgelist_start,
+                                const int32_t* edgelist, const int32_t* out,
+                                int32_t* frontiers, int32_t* frontiers_ends,
+                                int32_t* num_ends, int32_t* level_of_pin);
+
+/* Device-resident levelization: same semantics on device arrays, one
+ * persistent cooperative launch (a grid barrier per level, no host round
+ * trip).  out_degree is not modified (a device copy is consumed).
+ * frontiers_ends needs n+2 entries; *num_ends is written to host memory
+ * (the call synchronises `stream` to read it). */
+otg_status otg_levelize(otg_context* ctx, int32_t n, int32_t num_edges,
+                        const int32_t* fanin_start, const int32_t* fanin,
+                        const int32_t* out_degree, const int32_t* first_frontier,
+                        int32_t first_size, int32_t* frontiers, int32_t* frontiers_ends,
+                        int32_t* level_of_pin, int32_t* num_ends, otg_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OTG_H_ */

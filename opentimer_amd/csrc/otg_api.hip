@@ -1,0 +1,469 @@
+// C ABI of libotg: context management, P1 / P2 entry points and the
+// host-pointer drop-ins for the reference's GPU functions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "otg_internal.hpp"
+
+otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_size,
+                                   const int32_t* fanin_start, const int32_t* fanin,
+                                   int32_t* out_copy, int32_t* frontiers, int32_t* ends,
+                                   int32_t* level, int32_t* counters, hipStream_t s);
+
+namespace {
+thread_local std::string g_last_error;  // errors raised before a context exists
+}
+
+void otg_set_error(const otg_context* ctx, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) const_cast<otg_context*>(ctx)->last_error = buf;
+  else g_last_error = buf;
+}
+
+extern "C" int otg_version(void) { return 1; }
+
+extern "C" const char* otg_status_string(otg_status s) {
+  switch (s) {
+    case OTG_OK: return "ok";
+    case OTG_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case OTG_ERR_NOT_A_TREE: return "net is not a spanning tree";
+    case OTG_ERR_BAD_TOPOLOGY: return "bad parent offsets";
+    case OTG_ERR_DEVICE: return "HIP runtime error";
+    case OTG_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case OTG_ERR_NO_DEVICE: return "no HIP device";
+    case OTG_ERR_CYCLE: return "cycle in the timing graph";
+    case OTG_ERR_UNSUPPORTED: return "unsupported";
+  }
+  return "unknown status";
+}
+
+extern "C" const char* otg_last_error(const otg_context* ctx) {
+  return ctx ? ctx->last_error.c_str() : g_last_error.c_str();
+}
+
+extern "C" otg_status otg_create(int device, otg_context** out) {
+  if (!out) return OTG_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    otg_set_error(nullptr, "no HIP device visible");
+    return OTG_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= count) return OTG_ERR_INVALID_ARGUMENT;
+  otg_context* ctx = new otg_context();
+  ctx->device = device;
+  auto fail = [&](const char* what, hipError_t e) {
+    otg_set_error(nullptr, "%s: %s", what, hipGetErrorString(e));
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return OTG_ERR_DEVICE;
+  };
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return fail("hipSetDevice", e);
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return fail("hipGetDeviceProperties", e);
+  ctx->num_cus = prop.multiProcessorCount;
+  ctx->lds_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail("hipStreamCreate", e);
+  *out = ctx;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_destroy(otg_context* ctx) {
+  if (!ctx) return OTG_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) {
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_synchronize(otg_context* ctx, otg_stream stream) {
+  if (!ctx) return OTG_ERR_INVALID_ARGUMENT;
+  OTG_HIP(ctx, hipStreamSynchronize(otg_pick_stream(ctx, stream)));
+  return OTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// P1
+// ---------------------------------------------------------------------------
+static otg_status check_tree(otg_context* ctx, const otg_rct_tree* t) {
+  if (!t || t->num_nets < 0 || t->total_num_nodes < 0) return OTG_ERR_INVALID_ARGUMENT;
+  if (t->num_nets > 0 && (!t->nodes_start || !t->pid || !t->pres || !t->cap)) {
+    OTG_SET_ERR(ctx, "tree arrays must be non-null");
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_rct_solve(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tree* tree,
+                                    const otg_rct_result* out, otg_rct_method method,
+                                    otg_stream stream) {
+  if (!ctx || !out) return OTG_ERR_INVALID_ARGUMENT;
+  otg_status st = check_tree(ctx, tree);
+  if (st != OTG_OK) return st;
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  if (method == OTG_RCT_PLAIN) return otg_rct_launch_plain(ctx, tree, out, s);
+  if (method != OTG_RCT_BATCHED && method != OTG_RCT_BATCHED_COMPENSATED)
+    return OTG_ERR_INVALID_ARGUMENT;
+  if (plan) {
+    if (plan->num_nets != tree->num_nets || plan->total_nodes != tree->total_num_nodes) {
+      OTG_SET_ERR(ctx, "plan was built for a different topology");
+      return OTG_ERR_INVALID_ARGUMENT;
+    }
+    if (plan->method != method) {
+      OTG_SET_ERR(ctx, "plan was built for a different method");
+      return OTG_ERR_INVALID_ARGUMENT;
+    }
+    return otg_rct_launch_batched(ctx, plan, tree, out, nullptr, 0, s);
+  }
+  otg_rct_plan* tmp = nullptr;
+  st = otg_rct_plan_create(ctx, tree, nullptr, method, &tmp);
+  if (st != OTG_OK) return st;
+  st = otg_rct_launch_batched(ctx, tmp, tree, out, nullptr, 0, s);
+  hipError_t e = hipStreamSynchronize(s);
+  otg_rct_plan_destroy(tmp);
+  if (st == OTG_OK && e != hipSuccess) {
+    OTG_SET_ERR(ctx, "solve failed: %s", hipGetErrorString(e));
+    st = OTG_ERR_DEVICE;
+  }
+  return st;
+}
+
+extern "C" otg_status otg_rct_solve_subset(otg_context* ctx, otg_rct_plan* plan,
+                                           const otg_rct_tree* tree, const otg_rct_result* out,
+                                           otg_rct_method method, const int32_t* net_ids,
+                                           int32_t count, otg_stream stream) {
+  if (!ctx || !plan || !out || count < 0 || (count > 0 && !net_ids))
+    return OTG_ERR_INVALID_ARGUMENT;
+  otg_status st = check_tree(ctx, tree);
+  if (st != OTG_OK) return st;
+  if (method == OTG_RCT_PLAIN || plan->method != method) {
+    OTG_SET_ERR(ctx, "subset solves need a batched plan of the same method");
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  if (plan->num_nets != tree->num_nets) return OTG_ERR_INVALID_ARGUMENT;
+  return otg_rct_launch_batched(ctx, plan, tree, out, net_ids, count, otg_pick_stream(ctx, stream));
+}
+
+extern "C" otg_status otg_rct_bfs(otg_context* ctx, const otg_rct_edges* in, int32_t* node2bfs,
+                                  int32_t* pid, int32_t* edges_directed, float* pres,
+                                  float* cap_bfs, otg_stream stream) {
+  if (!ctx || !in || in->num_nets < 0 || in->total_num_nodes < 0) return OTG_ERR_INVALID_ARGUMENT;
+  if (in->num_nets == 0) return OTG_OK;
+  if (!in->nodes_start || !in->edges || !in->roots || !in->edge_res || !in->node_cap || !node2bfs ||
+      !pid || !edges_directed || !pres || !cap_bfs)
+    return OTG_ERR_INVALID_ARGUMENT;
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  int32_t* d_bad = nullptr;
+  OTG_HIP(ctx, hipMallocAsync((void**)&d_bad, sizeof(int32_t), s));
+  const int32_t big = INT32_MAX;
+  OTG_HIP(ctx, hipMemcpyAsync(d_bad, &big, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  otg_status st = otg_rct_bfs_launch(ctx, in, node2bfs, pid, edges_directed, pres, cap_bfs, d_bad, s);
+  int32_t bad = INT32_MAX;
+  if (st == OTG_OK) {
+    OTG_HIP(ctx, hipMemcpyAsync(&bad, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  }
+  OTG_HIP(ctx, hipFreeAsync(d_bad, s));
+  OTG_HIP(ctx, hipStreamSynchronize(s));
+  if (st != OTG_OK) return st;
+  if (bad != INT32_MAX) {
+    OTG_SET_ERR(ctx, "net %d: edges do not form a tree spanning its nodes from its root", bad);
+    return OTG_ERR_NOT_A_TREE;
+  }
+  return OTG_OK;
+}
+
+namespace {
+// RAII list of device buffers for the host-pointer entry points
+struct DevBufs {
+  std::vector<void*> p;
+  ~DevBufs() {
+    for (void* q : p) hipFree(q);
+  }
+  template <class T>
+  hipError_t alloc(T** out, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
+    if (e == hipSuccess) p.push_back(q);
+    *out = reinterpret_cast<T*>(q);
+    return e;
+  }
+};
+}  // namespace
+
+extern "C" otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_array* d,
+                                              otg_rct_method method) {
+  if (!ctx || !d || d->num_nets < 0 || d->total_num_nodes < 0) return OTG_ERR_INVALID_ARGUMENT;
+  if (d->num_nets == 0) return OTG_OK;
+  if (d->total_num_edges != d->total_num_nodes - d->num_nets) {
+    OTG_SET_ERR(ctx, "total_num_edges must equal total_num_nodes - num_nets (a forest)");
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  if (!d->rct_nodes_start || !d->rct_edges || !d->rct_roots || !d->rct_edges_res ||
+      !d->rct_nodes_cap || !d->rct_node2bfs_order || !d->rct_pid || !d->load || !d->delay ||
+      !d->ldelay || !d->impulse)
+    return OTG_ERR_INVALID_ARGUMENT;
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t N = d->num_nets, V = d->total_num_nodes, E = d->total_num_edges;
+  DevBufs B;
+  int32_t *ns, *ed, *rt, *n2b, *pid;
+  float *res, *ncap, *pres, *cap, *si = nullptr;
+  float *load, *delay, *ldelay, *beta, *imp, *slew = nullptr;
+  OTG_HIP(ctx, B.alloc(&ns, N + 1));
+  OTG_HIP(ctx, B.alloc(&ed, 2 * E));
+  OTG_HIP(ctx, B.alloc(&rt, N));
+  OTG_HIP(ctx, B.alloc(&res, E));
+  OTG_HIP(ctx, B.alloc(&ncap, 4 * V));
+  OTG_HIP(ctx, B.alloc(&n2b, V));
+  OTG_HIP(ctx, B.alloc(&pid, V));
+  OTG_HIP(ctx, B.alloc(&pres, V));
+  OTG_HIP(ctx, B.alloc(&cap, 4 * V));
+  OTG_HIP(ctx, B.alloc(&load, 4 * V));
+  OTG_HIP(ctx, B.alloc(&delay, 4 * V));
+  OTG_HIP(ctx, B.alloc(&ldelay, 4 * V));
+  OTG_HIP(ctx, B.alloc(&beta, 4 * V));
+  OTG_HIP(ctx, B.alloc(&imp, 4 * V));
+  if (d->slew) OTG_HIP(ctx, B.alloc(&slew, 4 * V));
+  if (d->slew_in) OTG_HIP(ctx, B.alloc(&si, 4 * N));
+  OTG_HIP(ctx, hipMemcpyAsync(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, s));
+  OTG_HIP(ctx, hipMemcpyAsync(ed, d->rct_edges, sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice, s));
+  OTG_HIP(ctx, hipMemcpyAsync(rt, d->rct_roots, sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+  OTG_HIP(ctx, hipMemcpyAsync(res, d->rct_edges_res, sizeof(float) * E, hipMemcpyHostToDevice, s));
+  OTG_HIP(ctx, hipMemcpyAsync(ncap, d->rct_nodes_cap, sizeof(float) * 4 * V, hipMemcpyHostToDevice, s));
+  if (si) OTG_HIP(ctx, hipMemcpyAsync(si, d->slew_in, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s));
+  otg_rct_edges in{d->num_nets, d->total_num_nodes, ns, ed, rt, res, ncap};
+  otg_status st = otg_rct_bfs(ctx, &in, n2b, pid, ed, pres, cap, s);
+  if (st != OTG_OK) return st;
+  otg_rct_tree tree{d->num_nets, d->total_num_nodes, ns, pid, pres, cap, si};
+  otg_rct_result out{load, delay, ldelay, beta, imp, slew};
+  st = otg_rct_solve(ctx, nullptr, &tree, &out, method, s);
+  if (st != OTG_OK) return st;
+  OTG_HIP(ctx, hipMemcpyAsync(d->rct_edges, ed, sizeof(int32_t) * 2 * E, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(d->rct_node2bfs_order, n2b, sizeof(int32_t) * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(d->rct_pid, pid, sizeof(int32_t) * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(d->load, load, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(d->delay, delay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(d->ldelay, ldelay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(d->impulse, imp, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  if (d->beta) OTG_HIP(ctx, hipMemcpyAsync(d->beta, beta, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  if (d->slew) OTG_HIP(ctx, hipMemcpyAsync(d->slew, slew, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipStreamSynchronize(s));
+  return OTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// P2
+// ---------------------------------------------------------------------------
+extern "C" otg_status otg_levelize(otg_context* ctx, int32_t n, int32_t num_edges,
+                                   const int32_t* fanin_start, const int32_t* fanin,
+                                   const int32_t* out_degree, const int32_t* first_frontier,
+                                   int32_t first_size, int32_t* frontiers, int32_t* frontiers_ends,
+                                   int32_t* level_of_pin, int32_t* num_ends, otg_stream stream) {
+  if (!ctx || n < 0 || num_edges < 0 || first_size < 0 || first_size > n || !num_ends)
+    return OTG_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (!fanin_start || !out_degree || !frontiers || !frontiers_ends ||
+                (num_edges > 0 && !fanin) || (first_size > 0 && !first_frontier)))
+    return OTG_ERR_INVALID_ARGUMENT;
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  if (n == 0) {
+    *num_ends = 2;
+    const int32_t z[2] = {0, 0};
+    if (frontiers_ends) OTG_HIP(ctx, hipMemcpyAsync(frontiers_ends, z, sizeof(z), hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, hipStreamSynchronize(s));
+    return OTG_OK;
+  }
+  int32_t *out_copy = nullptr, *counters = nullptr;
+  OTG_HIP(ctx, hipMallocAsync((void**)&out_copy, sizeof(int32_t) * n, s));
+  OTG_HIP(ctx, hipMallocAsync((void**)&counters, sizeof(int32_t) * 4, s));
+  OTG_HIP(ctx, hipMemcpyAsync(out_copy, out_degree, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  if (first_size > 0 && first_frontier != frontiers)
+    OTG_HIP(ctx, hipMemcpyAsync(frontiers, first_frontier, sizeof(int32_t) * first_size,
+                                hipMemcpyDeviceToDevice, s));
+  otg_status st = otg_levelize_launch_raw(ctx, n, first_size, fanin_start, fanin, out_copy, frontiers,
+                                          frontiers_ends, level_of_pin, counters, s);
+  int32_t ne = 0;
+  if (st == OTG_OK)
+    OTG_HIP(ctx, hipMemcpyAsync(&ne, counters + 3, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipFreeAsync(out_copy, s));
+  OTG_HIP(ctx, hipFreeAsync(counters, s));
+  OTG_HIP(ctx, hipStreamSynchronize(s));
+  if (st != OTG_OK) return st;
+  *num_ends = ne;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_toposort_compute(otg_context* ctx, int32_t n, int32_t num_edges,
+                                           int32_t first_size, const int32_t* edgelist_start,
+                                           const int32_t* edgelist, const int32_t* out,
+                                           int32_t* frontiers, int32_t* frontiers_ends,
+                                           int32_t* num_ends, int32_t* level_of_pin) {
+  if (!ctx || n < 0 || num_edges < 0 || first_size < 0 || first_size > n || !num_ends ||
+      !frontiers_ends)
+    return OTG_ERR_INVALID_ARGUMENT;
+  if (n > 0 && (!edgelist_start || !out || !frontiers || (num_edges > 0 && !edgelist)))
+    return OTG_ERR_INVALID_ARGUMENT;
+  if (n == 0) {
+    frontiers_ends[0] = 0;
+    frontiers_ends[1] = 0;
+    *num_ends = 2;
+    return OTG_OK;
+  }
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  DevBufs B;
+  int32_t *st_d, *el_d, *out_d, *fr_d, *ends_d, *lv_d = nullptr;
+  OTG_HIP(ctx, B.alloc(&st_d, (size_t)n + 1));
+  OTG_HIP(ctx, B.alloc(&el_d, (size_t)num_edges));
+  OTG_HIP(ctx, B.alloc(&out_d, (size_t)n));
+  OTG_HIP(ctx, B.alloc(&fr_d, (size_t)n));
+  OTG_HIP(ctx, B.alloc(&ends_d, (size_t)n + 2));
+  if (level_of_pin) OTG_HIP(ctx, B.alloc(&lv_d, (size_t)n));
+  OTG_HIP(ctx, hipMemcpyAsync(st_d, edgelist_start, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+  if (num_edges)
+    OTG_HIP(ctx, hipMemcpyAsync(el_d, edgelist, sizeof(int32_t) * num_edges, hipMemcpyHostToDevice, s));
+  OTG_HIP(ctx, hipMemcpyAsync(out_d, out, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  OTG_HIP(ctx, hipMemcpyAsync(fr_d, frontiers, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  int32_t ne = 0;
+  otg_status st = otg_levelize(ctx, n, num_edges, st_d, el_d, out_d, fr_d, first_size, fr_d, ends_d,
+                               lv_d, &ne, s);
+  if (st != OTG_OK) return st;
+  OTG_HIP(ctx, hipMemcpyAsync(frontiers, fr_d, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(frontiers_ends, ends_d, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, s));
+  if (lv_d) OTG_HIP(ctx, hipMemcpyAsync(level_of_pin, lv_d, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipStreamSynchronize(s));
+  *num_ends = ne;
+  return OTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner
+// ---------------------------------------------------------------------------
+static bool write_tuning(const char* path, const otg_rct_tuning& t, otg_rct_method m, double ms) {
+  FILE* f = fopen(path, "w");
+  if (!f) return false;
+  fprintf(f,
+          "{\"ring_slots\": %d, \"waves_per_wg\": %d, \"small_target\": %d, \"grid_per_cu\": %d, "
+          "\"method\": %d, \"ms\": %.6f}\n",
+          t.ring_slots, t.waves_per_wg, t.small_target, t.grid_per_cu, (int)m, ms);
+  fclose(f);
+  return true;
+}
+
+static int json_int(const std::string& s, const char* key, int dflt) {
+  std::string k = std::string("\"") + key + "\"";
+  size_t p = s.find(k);
+  if (p == std::string::npos) return dflt;
+  p = s.find(':', p);
+  if (p == std::string::npos) return dflt;
+  return atoi(s.c_str() + p + 1);
+}
+
+extern "C" otg_status otg_rct_tuning_load(otg_context* ctx, const char* path, otg_rct_tuning* t) {
+  if (!ctx || !path) return OTG_ERR_INVALID_ARGUMENT;
+  FILE* f = fopen(path, "r");
+  if (!f) {
+    OTG_SET_ERR(ctx, "cannot open %s", path);
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  std::string s;
+  char buf[512];
+  size_t k;
+  while ((k = fread(buf, 1, sizeof(buf), f)) > 0) s.append(buf, k);
+  fclose(f);
+  otg_rct_tuning r{};
+  r.ring_slots = json_int(s, "ring_slots", 0);
+  r.waves_per_wg = json_int(s, "waves_per_wg", 1);
+  r.small_target = json_int(s, "small_target", 0);
+  r.grid_per_cu = json_int(s, "grid_per_cu", 0);
+  ctx->tuning = r;
+  if (t) *t = r;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tree,
+                                       const otg_rct_result* scratch_out, otg_rct_method method,
+                                       const char* json_path, otg_rct_tuning* best) {
+  if (!ctx || !tree || !scratch_out) return OTG_ERR_INVALID_ARGUMENT;
+  if (method != OTG_RCT_BATCHED && method != OTG_RCT_BATCHED_COMPENSATED)
+    return OTG_ERR_INVALID_ARGUMENT;
+  otg_status st = check_tree(ctx, tree);
+  if (st != OTG_OK) return st;
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int rings[] = {256, 512, 1024};
+  const int targets_frac[] = {2, 1};  // small_target = ring / frac
+  std::vector<otg_rct_tuning> cands;
+  for (int W : rings)
+    for (int fr : targets_frac)
+      for (int g : {0, 2, 4}) {
+        otg_rct_tuning t{};
+        t.ring_slots = W;
+        t.waves_per_wg = 1;
+        t.small_target = W / fr;
+        t.grid_per_cu = g;
+        cands.push_back(t);
+      }
+  hipEvent_t e0, e1;
+  OTG_HIP(ctx, hipEventCreate(&e0));
+  OTG_HIP(ctx, hipEventCreate(&e1));
+  double best_ms = 1e30;
+  otg_rct_tuning bt{};
+  for (const auto& t : cands) {
+    otg_rct_plan* plan = nullptr;
+    if (otg_rct_plan_create(ctx, tree, &t, method, &plan) != OTG_OK) continue;
+    // skip candidates that push nets to the global-memory window when a
+    // larger ring keeps them on chip
+    st = otg_rct_launch_batched(ctx, plan, tree, scratch_out, nullptr, 0, s);  // warm-up
+    if (st != OTG_OK) {
+      otg_rct_plan_destroy(plan);
+      continue;
+    }
+    const int reps = 3;
+    OTG_HIP(ctx, hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) otg_rct_launch_batched(ctx, plan, tree, scratch_out, nullptr, 0, s);
+    OTG_HIP(ctx, hipEventRecord(e1, s));
+    OTG_HIP(ctx, hipEventSynchronize(e1));
+    float ms = 0;
+    OTG_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    otg_rct_tuning used = plan->tuning;
+    otg_rct_plan_destroy(plan);
+    if (ms < best_ms) {
+      best_ms = ms;
+      bt = used;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (best_ms >= 1e29) {
+    OTG_SET_ERR(ctx, "autotune: no candidate ran");
+    return OTG_ERR_DEVICE;
+  }
+  ctx->tuning = bt;
+  if (best) *best = bt;
+  if (json_path && !write_tuning(json_path, bt, method, best_ms)) {
+    OTG_SET_ERR(ctx, "cannot write %s", json_path);
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  return OTG_OK;
+}

@@ -1,0 +1,86 @@
+// Internal declarations shared by the HIP translation units of libotg.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/otg.h"
+
+struct otg_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  int lds_per_cu = 0;
+  std::string last_error;
+  otg_rct_tuning tuning{};  // defaults for new plans (autotuner may update)
+};
+
+// One unit of work of the batched P1 launch: a contiguous range of whole
+// nets [net_begin, net_end) occupying positions [pos_begin, pos_end).
+enum RctItemKind : int32_t {
+  ITEM_PACKED = 0,  // group of nets that fits the on-chip window entirely
+  ITEM_STREAM = 1,  // one net larger than the window, parent offsets < window-64
+  ITEM_GLOBAL = 2   // one net that needs a per-node window in global memory
+};
+
+struct RctItem {
+  int32_t pos_begin, pos_end;
+  int32_t net_begin, net_end;
+  int32_t kind;
+  int32_t pad;
+  int64_t scratch;  // float offset into the plan scratch (stream/global items)
+};
+
+struct otg_rct_plan {
+  otg_context* ctx = nullptr;
+  otg_rct_method method = OTG_RCT_BATCHED;
+  otg_rct_tuning tuning{};
+  int32_t num_nets = 0;
+  int32_t total_nodes = 0;
+  int32_t num_items = 0;
+  RctItem* d_items = nullptr;
+  int32_t* d_queue = nullptr;      // [0] next item, [1] finished workgroups
+  float* d_scratch = nullptr;      // window storage of stream/global items
+  int64_t scratch_floats = 0;
+  int32_t* d_net_maxpid = nullptr;  // per net, for subset solves
+  int64_t* d_net_scratch = nullptr; // per net scratch offset (-1: none)
+  int grid = 0;                     // persistent workgroups
+  int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+#define OTG_SET_ERR(ctx, ...) otg_set_error((ctx), __VA_ARGS__)
+void otg_set_error(const otg_context* ctx, const char* fmt, ...);
+
+#define OTG_HIP(ctx, expr)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      otg_set_error((ctx), "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                          \
+      return e_ == hipErrorOutOfMemory ? OTG_ERR_OUT_OF_MEMORY : OTG_ERR_DEVICE;  \
+    }                                                                             \
+  } while (0)
+
+inline hipStream_t otg_pick_stream(const otg_context* ctx, otg_stream s) {
+  return s ? (hipStream_t)s : ctx->stream;
+}
+
+// Defaults of the batched launch, resolved against the method.
+otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuning* t,
+                                      otg_rct_method m);
+
+// Launchers implemented in otg_rct.hip
+otg_status otg_rct_launch_plain(otg_context* ctx, const otg_rct_tree* tree,
+                                const otg_rct_result* out, hipStream_t s);
+otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tree* tree,
+                                  const otg_rct_result* out, const int32_t* subset,
+                                  int32_t count, hipStream_t s);
+int otg_rct_blocks_per_cu(otg_context* ctx, const otg_rct_tuning& t, bool comp);
+
+// otg_bfs.hip
+otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t* node2bfs,
+                              int32_t* pid, int32_t* edges_directed, float* pres,
+                              float* cap_bfs, int32_t* d_bad, hipStream_t s);

@@ -1,0 +1,185 @@
+"""ctypes binding of libotg.so (include/otg.h, include/otg_gen.h).
+
+The library is the product: every compute call goes to its HIP kernels.  If
+the library is missing this module raises at import time -- there is no CPU
+fallback.  Device buffers are passed as raw pointers (PyTorch tensors are used
+for allocation and streams, nothing more).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libotg.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `python -m opentimer_amd.build` "
+        "(or __graft_entry__.build()) first")
+
+lib = C.CDLL(LIB_PATH)
+
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+f32p = C.POINTER(C.c_float)
+vp = C.c_void_p
+
+OK = 0
+STATUS = {
+    0: "OTG_OK", 1: "OTG_ERR_INVALID_ARGUMENT", 2: "OTG_ERR_NOT_A_TREE",
+    3: "OTG_ERR_BAD_TOPOLOGY", 4: "OTG_ERR_DEVICE", 5: "OTG_ERR_OUT_OF_MEMORY",
+    6: "OTG_ERR_NO_DEVICE", 7: "OTG_ERR_CYCLE", 8: "OTG_ERR_UNSUPPORTED",
+}
+
+PLAIN, BATCHED, BATCHED_COMPENSATED = 0, 1, 2
+SHAPE_RANDOM, SHAPE_CHAIN, SHAPE_BALANCED = 0, 1, 2
+
+
+class OtgError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class RctTree(C.Structure):
+    _fields_ = [("num_nets", C.c_int32), ("total_num_nodes", C.c_int32),
+                ("nodes_start", vp), ("pid", vp), ("pres", vp), ("cap", vp),
+                ("slew_in", vp)]
+
+
+class RctResult(C.Structure):
+    _fields_ = [("load", vp), ("delay", vp), ("ldelay", vp), ("beta", vp),
+                ("impulse", vp), ("slew", vp)]
+
+
+class RctTuning(C.Structure):
+    _fields_ = [("ring_slots", C.c_int32), ("waves_per_wg", C.c_int32),
+                ("small_target", C.c_int32), ("grid_per_cu", C.c_int32)]
+
+
+class RctEdges(C.Structure):
+    _fields_ = [("num_nets", C.c_int32), ("total_num_nodes", C.c_int32),
+                ("nodes_start", vp), ("edges", vp), ("roots", vp), ("edge_res", vp),
+                ("node_cap", vp)]
+
+
+class RctEdgeArray(C.Structure):
+    _fields_ = [("num_nets", C.c_int32), ("total_num_nodes", C.c_int32),
+                ("total_num_edges", C.c_int32),
+                ("rct_nodes_start", vp), ("rct_edges", vp), ("rct_roots", vp),
+                ("rct_edges_res", vp), ("rct_nodes_cap", vp),
+                ("rct_node2bfs_order", vp), ("rct_pid", vp),
+                ("load", vp), ("delay", vp), ("ldelay", vp), ("impulse", vp),
+                ("beta", vp), ("slew", vp), ("slew_in", vp)]
+
+
+class GenSizeClass(C.Structure):
+    _fields_ = [("weight", C.c_double), ("min_nodes", C.c_int32), ("max_nodes", C.c_int32),
+                ("shape", C.c_int32), ("pad", C.c_int32), ("branch_prob", C.c_double)]
+
+
+class GenRcParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("num_nets", C.c_int32), ("num_classes", C.c_int32),
+                ("classes", GenSizeClass * 4),
+                ("r_min", C.c_double), ("r_max", C.c_double),
+                ("c_min", C.c_double), ("c_max", C.c_double),
+                ("pin_min", C.c_double), ("pin_max", C.c_double),
+                ("corner_scale", C.c_double * 4),
+                ("slew_min", C.c_double), ("slew_max", C.c_double),
+                ("r_unit", C.c_double), ("relabel", C.c_int32), ("random_orient", C.c_int32)]
+
+
+class GenDagParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("num_pins", C.c_int32), ("num_layers", C.c_int32),
+                ("num_arcs", C.c_int64), ("fanout_alpha", C.c_double),
+                ("fanout_cap", C.c_int32), ("skip_min", C.c_int32), ("skip_max", C.c_int32),
+                ("pad", C.c_int32), ("skip_prob", C.c_double), ("shuffle_ids", C.c_int32)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+ctx_p = C.c_void_p
+plan_p = C.c_void_p
+_sig("otg_create", C.c_int, C.c_int, C.POINTER(ctx_p))
+_sig("otg_destroy", C.c_int, ctx_p)
+_sig("otg_synchronize", C.c_int, ctx_p, vp)
+_sig("otg_status_string", C.c_char_p, C.c_int)
+_sig("otg_last_error", C.c_char_p, ctx_p)
+_sig("otg_version", C.c_int)
+_sig("otg_rct_plan_create", C.c_int, ctx_p, C.POINTER(RctTree), C.POINTER(RctTuning), C.c_int,
+     C.POINTER(plan_p))
+_sig("otg_rct_plan_destroy", C.c_int, plan_p)
+_sig("otg_rct_plan_stats", C.c_int, plan_p, i64p)
+_sig("otg_rct_solve", C.c_int, ctx_p, plan_p, C.POINTER(RctTree), C.POINTER(RctResult), C.c_int, vp)
+_sig("otg_rct_solve_subset", C.c_int, ctx_p, plan_p, C.POINTER(RctTree), C.POINTER(RctResult),
+     C.c_int, vp, C.c_int32, vp)
+_sig("otg_rct_bfs", C.c_int, ctx_p, C.POINTER(RctEdges), vp, vp, vp, vp, vp, vp)
+_sig("otg_rct_bfs_and_compute", C.c_int, ctx_p, C.POINTER(RctEdgeArray), C.c_int)
+_sig("otg_rct_autotune", C.c_int, ctx_p, C.POINTER(RctTree), C.POINTER(RctResult), C.c_int,
+     C.c_char_p, C.POINTER(RctTuning))
+_sig("otg_rct_tuning_load", C.c_int, ctx_p, C.c_char_p, C.POINTER(RctTuning))
+_sig("otg_toposort_compute", C.c_int, ctx_p, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp, vp,
+     i32p, vp)
+_sig("otg_levelize", C.c_int, ctx_p, C.c_int32, C.c_int32, vp, vp, vp, vp, C.c_int32, vp, vp, vp,
+     i32p, vp)
+_sig("otg_gen_rc_sizes", C.c_int, C.POINTER(GenRcParams), vp, i64p)
+_sig("otg_gen_rc_fill", C.c_int, C.POINTER(GenRcParams), vp, vp, vp, vp, vp, vp, vp, C.c_int)
+_sig("otg_gen_dag", C.c_int, C.POINTER(GenDagParams), vp, vp, C.c_int)
+_sig("otg_build_fanin_csr", C.c_int, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, i32p,
+     i64p)
+
+# Every symbol the C headers declare; tests check the library exports each.
+EXPORTED = [
+    "otg_create", "otg_destroy", "otg_synchronize", "otg_status_string", "otg_last_error",
+    "otg_version", "otg_rct_plan_create", "otg_rct_plan_destroy", "otg_rct_plan_stats",
+    "otg_rct_solve", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
+    "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize",
+    "otg_gen_rc_sizes", "otg_gen_rc_fill", "otg_gen_dag", "otg_build_fanin_csr",
+]
+
+
+def check(status: int, ctx=None):
+    if status != OK:
+        msg = lib.otg_last_error(ctx)
+        raise OtgError(status, (msg or b"").decode(errors="replace"))
+
+
+def addr(x):
+    """Raw address of a numpy array or torch tensor (None -> NULL)."""
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    return x.ctypes.data
+
+
+class Context:
+    """Owns an otg_context (device, stream)."""
+
+    def __init__(self, device: int = 0):
+        self.ptr = ctx_p()
+        check(lib.otg_create(device, C.byref(self.ptr)))
+        self.device = device
+
+    def close(self):
+        if self.ptr:
+            lib.otg_destroy(self.ptr)
+            self.ptr = ctx_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, status):
+        check(status, self.ptr)
+
+    def sync(self, stream=None):
+        self.check(lib.otg_synchronize(self.ptr, stream))

@@ -1,0 +1,159 @@
+"""numpy front-end of the CPU oracle (oracle/otg_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, and only as the checker.  The product
+(opentimer_amd, libotg.so) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "_build", "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return SO
+
+
+def _load():
+    if not os.path.exists(SO):
+        build()
+    return C.CDLL(SO)
+
+
+_lib = _load()
+vp = C.c_void_p
+for name, args, res in [
+    ("ora_rct_bfs", [C.c_int, vp, vp, vp, vp, vp], C.c_int),
+    ("ora_rct_prepare", [C.c_int, vp, vp, vp, vp, vp, vp, vp], None),
+    ("ora_rct_flat_f32", [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
+    ("ora_rct_dfs_f32", [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
+    ("ora_rct_dfs_f64", [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
+    ("ora_slew_f32", [C.c_int, vp, vp, vp, vp], None),
+    ("ora_slew_f64", [C.c_int, vp, vp, vp, vp], None),
+    ("ora_levelize", [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp], C.c_int),
+    ("ora_levels_longest_path", [C.c_int, vp, vp, vp], None),
+    ("ora_counting_argsort", [C.c_int, vp, C.c_int, C.c_int, vp, vp, vp], None),
+]:
+    f = getattr(_lib, name)
+    f.argtypes = args
+    f.restype = res
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def rct_bfs(nodes_start, edges, roots):
+    """BFS order of every net (rct.cu:120-245 + sort.cuh:63-142).
+
+    Returns (edges_directed, node2bfs, pid)."""
+    ns = _i32(nodes_start)
+    e = _i32(edges).copy()
+    r = _i32(roots)
+    V = int(ns[-1])
+    n2b = np.zeros(V, np.int32)
+    pid = np.zeros(V, np.int32)
+    rc = _lib.ora_rct_bfs(len(ns) - 1, _p(ns), _p(e), _p(r), _p(n2b), _p(pid))
+    if rc != 0:
+        raise ValueError("not a forest of trees")
+    return e, n2b, pid
+
+
+def rct_prepare(nodes_start, edges_directed, node2bfs, edge_res, node_cap):
+    """pres / cap in BFS order (rct.cu:314-359)."""
+    ns = _i32(nodes_start)
+    V = int(ns[-1])
+    pres = np.zeros(V, np.float32)
+    cap = np.zeros(4 * V, np.float32)
+    _lib.ora_rct_prepare(len(ns) - 1, _p(ns), _p(_i32(edges_directed)), _p(_i32(node2bfs)),
+                         _p(_f32(edge_res)), _p(_f32(node_cap)), _p(pres), _p(cap))
+    return pres, cap
+
+
+def _moments(fn, dtype, nodes_start, pid, pres, cap):
+    ns = _i32(nodes_start)
+    V = int(ns[-1])
+    outs = [np.zeros(4 * V, dtype) for _ in range(5)]
+    fn(len(ns) - 1, _p(ns), _p(_i32(pid)), _p(_f32(pres)), _p(_f32(cap)), *[_p(o) for o in outs])
+    return dict(zip(["load", "delay", "ldelay", "beta", "impulse"], outs))
+
+
+def rct_flat_f32(nodes_start, pid, pres, cap):
+    """Reference GPU kernel arithmetic (rct.cu:11-59), FP32."""
+    return _moments(_lib.ora_rct_flat_f32, np.float32, nodes_start, pid, pres, cap)
+
+
+def rct_dfs_f32(nodes_start, pid, pres, cap):
+    """CPU ot::Timer arithmetic (net.cpp:101-193), FP32 with double pow."""
+    return _moments(_lib.ora_rct_dfs_f32, np.float32, nodes_start, pid, pres, cap)
+
+
+def rct_dfs_f64(nodes_start, pid, pres, cap):
+    """FP64 oracle of the same sweeps."""
+    return _moments(_lib.ora_rct_dfs_f64, np.float64, nodes_start, pid, pres, cap)
+
+
+def slew(nodes_start, slew_in, impulse):
+    ns = _i32(nodes_start)
+    if impulse.dtype == np.float64:
+        out = np.zeros_like(impulse)
+        _lib.ora_slew_f64(len(ns) - 1, _p(ns), _p(_f32(slew_in)), _p(impulse), _p(out))
+    else:
+        out = np.zeros(impulse.shape, np.float32)
+        _lib.ora_slew_f32(len(ns) - 1, _p(ns), _p(_f32(slew_in)), _p(_f32(impulse)), _p(out))
+    return out
+
+
+def rct_all(nodes_start, pid, pres, cap, slew_in, kind="dfs_f32"):
+    fn = {"dfs_f32": rct_dfs_f32, "flat_f32": rct_flat_f32, "f64": rct_dfs_f64}[kind]
+    r = fn(nodes_start, pid, pres, cap)
+    r["slew"] = slew(nodes_start, slew_in, r["impulse"])
+    return r
+
+
+def levelize(n, fanin_start, fanin, out_degree, frontier0):
+    """Sink-first Kahn (toposort.cu:4-72).  Returns (frontiers, ends, level)."""
+    fs = _i32(fanin_start)
+    fi = _i32(fanin) if len(fanin) else np.zeros(1, np.int32)
+    out = _i32(out_degree).copy()
+    fr = np.zeros(max(n, 1), np.int32)
+    f0 = _i32(frontier0)
+    fr[: len(f0)] = f0
+    ends = np.zeros(n + 2, np.int32)
+    lv = np.zeros(max(n, 1), np.int32)
+    ne = _lib.ora_levelize(n, _p(fs), _p(fi), _p(out), len(f0), _p(fr), _p(ends), _p(lv))
+    return fr[:n], ends[:ne], lv[:n]
+
+
+def counting_argsort(inputs, amin, amax):
+    """block_couting_sort (sort.cuh:63-142).  Returns (orders, outputs, counts)."""
+    x = _i32(inputs)
+    n = len(x)
+    counts = np.zeros(amax - amin + 1, np.int32)
+    orders = np.zeros(max(n, 1), np.int32)
+    outputs = np.zeros(max(n, 1), np.int32)
+    _lib.ora_counting_argsort(n, _p(x), amin, amax, _p(counts), _p(orders), _p(outputs))
+    return orders[:n], outputs[:n], counts
+
+
+def levels_longest_path(n, fanout_start, fanout):
+    fs = _i32(fanout_start)
+    fo = _i32(fanout) if len(fanout) else np.zeros(1, np.int32)
+    lv = np.zeros(max(n, 1), np.int32)
+    _lib.ora_levels_longest_path(n, _p(fs), _p(fo), _p(lv))
+    return lv[:n]

@@ -1,0 +1,386 @@
+/*
+ * otg_oracle.c -- CPU restatement of OpenTimer's RC-tree Elmore path and
+ * timing-graph levelization, used ONLY as the checker.
+ *
+ * TEST INFRASTRUCTURE.  Nothing in the product (the opentimer_amd package and
+ * its libotg.so) links, loads or calls this file.  Only tests/, the smoke
+ * check in __graft_entry__.py and the cpu_baseline leg of bench.py load the
+ * shared library built from it (oracle/_build/liboracle.so).
+ *
+ * Every function cites the reference file:line whose semantics it restates.
+ * The reference's own implementation cannot be compiled here (its CPU timer
+ * headers do not build against libstdc++ 11 without source patches and its
+ * CUDA files need nvcc), so this restatement is pinned instead by the closed
+ * forms, brute-force sums and invariants in tests/test_oracle.py.
+ *
+ * Layout conventions (reference ot/cuda/rct.cuh:20-42):
+ *   nodes_start[N+1]    node offsets per net
+ *   edges[2*E]          undirected (s,t) pairs, net-local ids; the edges of
+ *                       net k start at nodes_start[k]-k (rct.cu:144)
+ *   roots[N]            net-local id of the driver
+ *   edge_res[E], node_cap[4*V]   (original node order)
+ *   corner index c = el*2 + rf  (ot/headerdef.hpp:172-177)
+ * Build with -ffp-contract=off so each float operation rounds once.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NC 4 /* MAX_SPLIT_TRAN */
+
+/* ------------------------------------------------------------------------ */
+/* Counting argsort, block_couting_sort (ot/cuda/sort.cuh:63-142) with       */
+/* init_counts=true: counts[v-amin] = occurrences (:69-73), inclusive prefix  */
+/* sum (:85, block_prefix_sum sort.cuh:14-51), then for i = 0..n-1 in order   */
+/* orders[i] = --counts[inputs[i]-amin] and outputs[that] = inputs[i]         */
+/* (:102-121).  Equal values therefore take positions in descending index    */
+/* order.  counts needs amax-amin+1 entries; orders/outputs may be NULL.      */
+/* ------------------------------------------------------------------------ */
+void ora_counting_argsort(int n, const int32_t* inputs, int amin, int amax, int32_t* counts,
+                          int32_t* orders, int32_t* outputs) {
+  const int nb = amax - amin + 1;
+  for (int b = 0; b < nb; ++b) counts[b] = 0;
+  for (int i = 0; i < n; ++i) counts[inputs[i] - amin]++;
+  for (int b = 1; b < nb; ++b) counts[b] += counts[b - 1];
+  for (int i = 0; i < n; ++i) {
+    const int c = --counts[inputs[i] - amin];
+    if (orders) orders[i] = c;
+    if (outputs) outputs[c] = inputs[i];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Per-net BFS ordering.                                                     */
+/* Restates compute_net_bfs (ot/cuda/rct.cu:120-245) and the argsort done by  */
+/* block_couting_sort (ot/cuda/sort.cuh:63-142):                              */
+/*  - distances are BFS hop counts from the root over undirected edges;       */
+/*  - every edge is oriented parent->child in place (rct.cu:185-199);         */
+/*  - the sort walks node ids in ascending order and hands out positions from */
+/*    the END of each distance bucket (--counts[val], sort.cuh:102-121), so   */
+/*    position(u) = #(nodes with smaller distance)                            */
+/*                + #(nodes with equal distance and larger id);               */
+/*  - pid[pos(t)] = pos(t) - pos(s) for every oriented edge (rct.cu:233-243). */
+/* The reference leaves pid at the root position unwritten (cudaMalloc'd);    */
+/* this restatement writes 0 there.                                           */
+/* Returns 0, or -1 if some net is not a connected tree (the reference would  */
+/* index counts[INT_MAX] in that case).                                       */
+/* ------------------------------------------------------------------------ */
+int ora_rct_bfs(int num_nets, const int32_t* nodes_start, int32_t* edges,
+                const int32_t* roots, int32_t* node2bfs, int32_t* pid) {
+  int maxn = 0;
+  for (int k = 0; k < num_nets; ++k) {
+    int n = nodes_start[k + 1] - nodes_start[k];
+    if (n > maxn) maxn = n;
+  }
+  if (maxn == 0) return 0;
+  int32_t* dist = (int32_t*)malloc(sizeof(int32_t) * maxn);
+  int32_t* deg = (int32_t*)malloc(sizeof(int32_t) * (maxn + 1));
+  int32_t* adj = (int32_t*)malloc(sizeof(int32_t) * 2 * maxn);
+  int32_t* queue = (int32_t*)malloc(sizeof(int32_t) * maxn);
+  int32_t* bucket = (int32_t*)malloc(sizeof(int32_t) * (maxn + 1));
+  int rc = 0;
+  for (int k = 0; k < num_nets && rc == 0; ++k) {
+    const int off = nodes_start[k];
+    const int n = nodes_start[k + 1] - off;
+    const int ne = n - 1;
+    int32_t* e = edges + 2 * (int64_t)(off - k);
+    if (n <= 0) continue;
+    if (roots[k] < 0 || roots[k] >= n) { rc = -1; break; }
+    /* adjacency (CSR) */
+    memset(deg, 0, sizeof(int32_t) * (n + 1));
+    for (int i = 0; i < ne; ++i) {
+      if (e[2 * i] < 0 || e[2 * i] >= n || e[2 * i + 1] < 0 || e[2 * i + 1] >= n) { rc = -1; break; }
+      deg[e[2 * i] + 1]++;
+      deg[e[2 * i + 1] + 1]++;
+    }
+    if (rc) break;
+    for (int i = 0; i < n; ++i) deg[i + 1] += deg[i];
+    for (int i = 0; i < ne; ++i) {
+      adj[deg[e[2 * i]]++] = e[2 * i + 1];
+      adj[deg[e[2 * i + 1]]++] = e[2 * i];
+    }
+    for (int i = n; i > 0; --i) deg[i] = deg[i - 1];
+    deg[0] = 0;
+    /* BFS hop distances */
+    for (int i = 0; i < n; ++i) dist[i] = -1;
+    int head = 0, tail = 0;
+    dist[roots[k]] = 0;
+    queue[tail++] = roots[k];
+    while (head < tail) {
+      int u = queue[head++];
+      for (int j = deg[u]; j < deg[u + 1]; ++j) {
+        int v = adj[j];
+        if (dist[v] < 0) { dist[v] = dist[u] + 1; queue[tail++] = v; }
+      }
+    }
+    if (tail != n) { rc = -1; break; }
+    /* counting sort by distance, positions handed out from the bucket ends */
+    ora_counting_argsort(n, dist, 0, n - 1, bucket, node2bfs + off, NULL);
+    /* orient edges and fill pid */
+    pid[off + node2bfs[off + roots[k]]] = 0;
+    for (int i = 0; i < ne; ++i) {
+      int s = e[2 * i], t = e[2 * i + 1];
+      if (dist[s] > dist[t]) { e[2 * i] = t; e[2 * i + 1] = s; int x = s; s = t; t = x; }
+      if (dist[t] != dist[s] + 1) { rc = -1; break; }
+      pid[off + node2bfs[off + t]] = node2bfs[off + t] - node2bfs[off + s];
+    }
+  }
+  free(dist); free(deg); free(adj); free(queue); free(bucket);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reorder R and C into BFS order: prepare_res_cap (ot/cuda/rct.cu:314-359),  */
+/* preceded by the two cudaMemset calls (rct.cu:378-379) that leave the root  */
+/* position's resistance at 0.  edges must already be oriented.               */
+/* ------------------------------------------------------------------------ */
+void ora_rct_prepare(int num_nets, const int32_t* nodes_start, const int32_t* edges,
+                     const int32_t* node2bfs, const float* edge_res, const float* node_cap,
+                     float* pres, float* cap) {
+  for (int k = 0; k < num_nets; ++k) {
+    const int off = nodes_start[k];
+    const int n = nodes_start[k + 1] - off;
+    const int32_t* e = edges + 2 * (int64_t)(off - k);
+    const float* r = edge_res + (off - k);
+    for (int i = 0; i < n; ++i) pres[off + i] = 0.0f;
+    for (int i = 0; i + 1 < n; ++i) pres[off + node2bfs[off + e[2 * i + 1]]] = r[i];
+    for (int u = 0; u < n; ++u)
+      for (int c = 0; c < NC; ++c)
+        cap[(int64_t)(off + node2bfs[off + u]) * NC + c] = node_cap[(int64_t)(off + u) * NC + c];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference GPU kernel arithmetic: compute_net_timing (ot/cuda/rct.cu:11-59) */
+/* on BFS-ordered arrays, one net and corner at a time, FP32, same operation  */
+/* order.  beta is also returned (the reference overwrites it with impulse,   */
+/* rct.cu:49-58).  Any output pointer may be NULL except the work arrays      */
+/* load/delay/ldelay/beta, which must be present.                             */
+/* ------------------------------------------------------------------------ */
+void ora_rct_flat_f32(int num_nets, const int32_t* nodes_start, const int32_t* pid,
+                      const float* pres, const float* cap, float* load, float* delay,
+                      float* ldelay, float* beta, float* impulse) {
+  for (int k = 0; k < num_nets; ++k) {
+    const int64_t st = nodes_start[k], ed = nodes_start[k + 1];
+    for (int c = 0; c < NC; ++c) {
+      for (int64_t i = st; i < ed; ++i) {
+        load[i * NC + c] = cap[i * NC + c];
+        delay[i * NC + c] = 0.0f; ldelay[i * NC + c] = 0.0f; beta[i * NC + c] = 0.0f;
+      }
+      for (int64_t i = ed - 1; i > st; --i) load[(i - pid[i]) * NC + c] += load[i * NC + c];
+      for (int64_t i = st + 1; i < ed; ++i)
+        delay[i * NC + c] += delay[(i - pid[i]) * NC + c] + load[i * NC + c] * pres[i];
+      for (int64_t i = ed - 1; i > st; --i) {
+        ldelay[i * NC + c] += cap[i * NC + c] * delay[i * NC + c];
+        ldelay[(i - pid[i]) * NC + c] += ldelay[i * NC + c];
+      }
+      if (ed > st) ldelay[st * NC + c] += cap[st * NC + c] * delay[st * NC + c];
+      for (int64_t i = st + 1; i < ed; ++i)
+        beta[i * NC + c] += beta[(i - pid[i]) * NC + c] + ldelay[i * NC + c] * pres[i];
+      if (impulse)
+        for (int64_t i = st; i < ed; ++i) {
+          float t = delay[i * NC + c];
+          impulse[i * NC + c] = 2.0f * beta[i * NC + c] - t * t;
+        }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CPU ot::Timer arithmetic: Rct::update_rc_timing and its four recursive     */
+/* sweeps (ot/timer/net.cpp:101-193), restated without recursion on the       */
+/* BFS-ordered arrays.  A node's fanout is visited in ascending BFS position  */
+/* (the reference visits _fanout in *RES insertion order).  Per node:         */
+/*   load   = (sum over children, in order, of child load) + cap  (:127-140)  */
+/*   delay  = delay(parent) + res * load                          (:144-157)  */
+/*   ldelay = (sum over children of child ldelay) + cap*delay     (:161-175)  */
+/*   beta   = beta(parent) + res * ldelay                         (:179-193)  */
+/*   impulse = 2.0f*beta - std::pow(delay, 2): std::pow(float,int) returns    */
+/*   double, so the subtraction runs in double and rounds once to float.      */
+/* ------------------------------------------------------------------------ */
+static void children_csr(int64_t st, int64_t ed, const int32_t* pid, int32_t* cstart, int32_t* clist) {
+  const int n = (int)(ed - st);
+  for (int i = 0; i <= n; ++i) cstart[i] = 0;
+  for (int i = 1; i < n; ++i) cstart[i - pid[st + i] + 1]++;
+  for (int i = 0; i < n; ++i) cstart[i + 1] += cstart[i];
+  for (int i = 1; i < n; ++i) clist[cstart[i - pid[st + i]]++] = i; /* ascending */
+  for (int i = n; i > 0; --i) cstart[i] = cstart[i - 1];
+  cstart[0] = 0;
+}
+
+static int max_net_size(int num_nets, const int32_t* nodes_start) {
+  int m = 1;
+  for (int k = 0; k < num_nets; ++k) {
+    int n = nodes_start[k + 1] - nodes_start[k];
+    if (n > m) m = n;
+  }
+  return m;
+}
+
+void ora_rct_dfs_f32(int num_nets, const int32_t* nodes_start, const int32_t* pid,
+                     const float* pres, const float* cap, float* load, float* delay,
+                     float* ldelay, float* beta, float* impulse) {
+  const int maxn = max_net_size(num_nets, nodes_start);
+  int32_t* cstart = (int32_t*)malloc(sizeof(int32_t) * (maxn + 1));
+  int32_t* clist = (int32_t*)malloc(sizeof(int32_t) * (maxn + 1));
+  for (int k = 0; k < num_nets; ++k) {
+    const int64_t st = nodes_start[k], ed = nodes_start[k + 1];
+    const int n = (int)(ed - st);
+    if (n <= 0) continue;
+    children_csr(st, ed, pid, cstart, clist);
+    for (int c = 0; c < NC; ++c) {
+#define A(arr, i) arr[(st + (i)) * NC + c]
+      for (int i = n - 1; i >= 0; --i) {
+        float acc = 0.0f;
+        for (int j = cstart[i]; j < cstart[i + 1]; ++j) acc += A(load, clist[j]);
+        A(load, i) = acc + A(cap, i);
+      }
+      A(delay, 0) = 0.0f;
+      for (int i = 1; i < n; ++i) A(delay, i) = A(delay, i - pid[st + i]) + pres[st + i] * A(load, i);
+      for (int i = n - 1; i >= 0; --i) {
+        float acc = 0.0f;
+        for (int j = cstart[i]; j < cstart[i + 1]; ++j) acc += A(ldelay, clist[j]);
+        A(ldelay, i) = acc + A(cap, i) * A(delay, i);
+      }
+      A(beta, 0) = 0.0f;
+      for (int i = 1; i < n; ++i) A(beta, i) = A(beta, i - pid[st + i]) + pres[st + i] * A(ldelay, i);
+      for (int i = 0; i < n; ++i) {
+        double d = (double)A(delay, i);
+        A(impulse, i) = (float)((double)(2.0f * A(beta, i)) - d * d);
+      }
+#undef A
+    }
+  }
+  free(cstart); free(clist);
+}
+
+/* FP64 oracle: the same sweeps as ora_rct_dfs_f32 carried out in double     */
+/* from the FP32 inputs.  This is the reference value for precision reports.  */
+void ora_rct_dfs_f64(int num_nets, const int32_t* nodes_start, const int32_t* pid,
+                     const float* pres, const float* cap, double* load, double* delay,
+                     double* ldelay, double* beta, double* impulse) {
+  const int maxn = max_net_size(num_nets, nodes_start);
+  int32_t* cstart = (int32_t*)malloc(sizeof(int32_t) * (maxn + 1));
+  int32_t* clist = (int32_t*)malloc(sizeof(int32_t) * (maxn + 1));
+  for (int k = 0; k < num_nets; ++k) {
+    const int64_t st = nodes_start[k], ed = nodes_start[k + 1];
+    const int n = (int)(ed - st);
+    if (n <= 0) continue;
+    children_csr(st, ed, pid, cstart, clist);
+    for (int c = 0; c < NC; ++c) {
+#define A(arr, i) arr[(st + (i)) * NC + c]
+      for (int i = n - 1; i >= 0; --i) {
+        double acc = 0.0;
+        for (int j = cstart[i]; j < cstart[i + 1]; ++j) acc += A(load, clist[j]);
+        A(load, i) = acc + (double)A(cap, i);
+      }
+      A(delay, 0) = 0.0;
+      for (int i = 1; i < n; ++i) A(delay, i) = A(delay, i - pid[st + i]) + (double)pres[st + i] * A(load, i);
+      for (int i = n - 1; i >= 0; --i) {
+        double acc = 0.0;
+        for (int j = cstart[i]; j < cstart[i + 1]; ++j) acc += A(ldelay, clist[j]);
+        A(ldelay, i) = acc + (double)A(cap, i) * A(delay, i);
+      }
+      A(beta, 0) = 0.0;
+      for (int i = 1; i < n; ++i) A(beta, i) = A(beta, i - pid[st + i]) + (double)pres[st + i] * A(ldelay, i);
+      for (int i = 0; i < n; ++i) A(impulse, i) = 2.0 * A(beta, i) - A(delay, i) * A(delay, i);
+#undef A
+    }
+  }
+  free(cstart); free(clist);
+}
+
+/* Output slew: RctNode::slew / FlatRct::slew (ot/timer/net.cpp:30-32,      */
+/* :280-283): sign(si) * sqrt(si^2 + impulse), with si the driver-side slew  */
+/* of the net for that corner (si[4*net + c]).                               */
+void ora_slew_f32(int num_nets, const int32_t* nodes_start, const float* si,
+                  const float* impulse, float* slew) {
+  for (int k = 0; k < num_nets; ++k)
+    for (int64_t i = nodes_start[k]; i < nodes_start[k + 1]; ++i)
+      for (int c = 0; c < NC; ++c) {
+        float s = si[(int64_t)k * NC + c];
+        float v = sqrtf(s * s + impulse[i * NC + c]);
+        slew[i * NC + c] = s < 0.0f ? -v : v;
+      }
+}
+
+void ora_slew_f64(int num_nets, const int32_t* nodes_start, const float* si,
+                  const double* impulse, double* slew) {
+  for (int k = 0; k < num_nets; ++k)
+    for (int64_t i = nodes_start[k]; i < nodes_start[k + 1]; ++i)
+      for (int c = 0; c < NC; ++c) {
+        double s = si[(int64_t)k * NC + c];
+        double v = sqrt(s * s + impulse[i * NC + c]);
+        slew[i * NC + c] = s < 0.0 ? -v : v;
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sink-first levelization: toposort_advance + toposort_compute_cuda          */
+/* (ot/cuda/toposort.cu:4-72) driven as in Timer::_build_prop_tasks_cuda      */
+/* (ot/timer/timer.cpp:1053-1071).  frontiers[0:first_size) holds the         */
+/* initial frontier (pins with no fanout); out[] is the fanout count and is   */
+/* consumed.  For u in the current frontier, for v in fanin(u): --out[v] and  */
+/* v joins the next frontier when it reaches 0.  ends[] gets 0, first_size,   */
+/* then the running end of every non-empty frontier; the return value is the */
+/* number of entries written to ends (levels + 1).  level[v] = index of the   */
+/* frontier holding v, -1 for pins never reached.                             */
+/* ------------------------------------------------------------------------ */
+int ora_levelize(int n, const int32_t* fanin_start, const int32_t* fanin, int32_t* out,
+                 int first_size, int32_t* frontiers, int32_t* ends, int32_t* level) {
+  for (int i = 0; i < n; ++i) level[i] = -1;
+  int nends = 0;
+  ends[nends++] = 0;
+  ends[nends++] = first_size;
+  for (int i = 0; i < first_size; ++i) level[frontiers[i]] = 0;
+  int wo_last = 0, last = first_size, lv = 0;
+  while (last > 0) {
+    int new_size = 0;
+    int32_t* next = frontiers + wo_last + last;
+    for (int id = 0; id < last; ++id) {
+      int u = frontiers[wo_last + id];
+      for (int j = fanin_start[u]; j < fanin_start[u + 1]; ++j) {
+        int v = fanin[j];
+        if (out[v]-- == 1) { next[new_size++] = v; level[v] = lv + 1; }
+      }
+    }
+    if (!new_size) break;
+    wo_last += last;
+    last = new_size;
+    ends[nends++] = wo_last + new_size;
+    ++lv;
+  }
+  return nends;
+}
+
+/* Longest-path definition of the same levels, by memoised DFS over fanouts: */
+/* level(u) = 0 if u has no fanout, else 1 + max over fanouts.  Used as an    */
+/* independent check of ora_levelize on acyclic inputs (fanout CSR given).    */
+void ora_levels_longest_path(int n, const int32_t* fanout_start, const int32_t* fanout,
+                             int32_t* level) {
+  int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  int32_t* iter = (int32_t*)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) level[i] = -1;
+  for (int r = 0; r < n; ++r) {
+    if (level[r] >= 0) continue;
+    int sp = 0;
+    stack[sp++] = r;
+    iter[r] = fanout_start[r];
+    level[r] = -2; /* on stack */
+    while (sp) {
+      int u = stack[sp - 1];
+      if (iter[u] < fanout_start[u + 1]) {
+        int v = fanout[iter[u]++];
+        if (level[v] == -1) { level[v] = -2; iter[v] = fanout_start[v]; stack[sp++] = v; }
+      } else {
+        int best = 0;
+        for (int j = fanout_start[u]; j < fanout_start[u + 1]; ++j)
+          if (level[fanout[j]] + 1 > best) best = level[fanout[j]] + 1;
+        level[u] = best;
+        --sp;
+      }
+    }
+  }
+  free(stack); free(iter);
+}

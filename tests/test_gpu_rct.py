@@ -1,0 +1,254 @@
+"""GPU parity of P1 (RC-tree Elmore) and the per-net BFS ordering.
+
+Bars:
+  * BFS ordering (node2bfs, pid, oriented edges), pres and cap in BFS order:
+    bit-exact vs the oracle (integer / copy work).
+  * plain kernel: bit-exact vs the oracle's restatement of the reference
+    kernel arithmetic (rct.cu:11-59, FP32, one rounding per operation).
+  * batched FP32: within 1e-5 relative of the CPU ot::Timer arithmetic
+    (net.cpp:101-193) on delay and slew, and of the FP64 oracle on all
+    outputs (impulse measured against max(|impulse|, 1e-6*beta) since it is
+    a difference that can vanish).
+  * batched compensated: within 1e-6 relative of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+import oracle as O
+from opentimer_amd import native as N
+from opentimer_amd import workloads as W
+from opentimer_amd.rct import RcSolver
+
+OUT = ("load", "delay", "ldelay", "beta", "impulse", "slew")
+
+
+def rel_err(got, ref, floor):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), floor))) if ref.size else 0.0
+
+
+def floors(ref):
+    """Per-output floor for relative errors: exact zeros (driver delay, beta,
+    impulse) are compared absolutely against the output's scale."""
+    f = {}
+    for k in OUT:
+        scale = float(np.max(np.abs(ref[k]))) if ref[k].size else 1.0
+        f[k] = max(scale * 1e-7, 1e-30)
+    f["impulse"] = max(float(np.max(np.abs(ref["beta"]))) * 1e-6, 1e-30)
+    return f
+
+
+def oracle_inputs(forest):
+    e, n2b, pid = O.rct_bfs(forest.nodes_start, forest.edges, forest.roots)
+    pres, cap = O.rct_prepare(forest.nodes_start, e, n2b, forest.edge_res, forest.node_cap)
+    return e, n2b, pid, pres, cap
+
+
+@pytest.fixture(scope="module")
+def cfg0(ctx):
+    f = W.generate_rc(W.config0())
+    s = RcSolver.from_edges(f, ctx)
+    return f, s
+
+
+def test_bfs_bitexact(cfg0):
+    f, s = cfg0
+    e, n2b, pid, pres, cap = oracle_inputs(f)
+    h = s.host_topology()
+    assert np.array_equal(h["node2bfs"], n2b)
+    assert np.array_equal(h["pid"], pid)
+    assert np.array_equal(s.edges_directed.cpu().numpy(), e)
+    assert np.array_equal(h["pres"].view(np.uint32), pres.view(np.uint32))
+    assert np.array_equal(h["cap"].view(np.uint32), cap.view(np.uint32))
+
+
+def test_plain_bitexact_vs_reference_arithmetic(cfg0):
+    f, s = cfg0
+    h = s.host_topology()
+    s.solve("plain")
+    g = s.host()
+    ref = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "flat_f32")
+    for k in ("load", "delay", "ldelay", "beta", "impulse"):
+        assert np.array_equal(g[k].view(np.uint32), ref[k].view(np.uint32)), k
+    assert rel_err(g["slew"], ref["slew"], 1e-30) < 1e-6
+
+
+@pytest.mark.parametrize("method,tol64", [("batched", 1e-5), ("compensated", 1e-6)])
+def test_batched_cfg0(cfg0, method, tol64):
+    f, s = cfg0
+    h = s.host_topology()
+    s.solve(method)
+    g = s.host()
+    r32 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "dfs_f32")
+    r64 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "f64")
+    fl = floors(r64)
+    for k in ("delay", "slew"):
+        assert rel_err(g[k], r32[k], fl[k]) < 1e-5, (k, "vs cpu timer fp32")
+    for k in OUT:
+        assert rel_err(g[k], r64[k], fl[k]) < tol64, (k, "vs fp64")
+
+
+def _forest(cfg, ctx):
+    f = W.generate_rc(cfg)
+    return f, RcSolver.from_edges(f, ctx)
+
+
+@pytest.mark.parametrize("ring", [256, 512, 1024])
+@pytest.mark.parametrize("method,tol64", [("batched", 1e-5), ("compensated", 1e-6)])
+def test_batched_mixed_sizes(ctx, ring, method, tol64):
+    # heavy-tailed mix (cfg1 distribution, scaled down) exercises packed groups,
+    # streamed nets and global-window nets
+    cfg = W.config1(num_nets=3000, seed=21)
+    cfg.classes[2].weight = 0.02
+    cfg.classes[2].max_nodes = 3000
+    f, s = _forest(cfg, ctx)
+    h = s.host_topology()
+    s.solve(method, tuning=(ring, 1, 0, 0))
+    st = s.plan_stats(method, (ring, 1, 0, 0))
+    assert st["packed_nets"] > 0 and st["stream_nets"] + st["global_nets"] > 0
+    g = s.host()
+    r32 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "dfs_f32")
+    r64 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "f64")
+    fl = floors(r64)
+    for k in ("delay", "slew"):
+        assert rel_err(g[k], r32[k], fl[k]) < 1e-5, k
+    for k in OUT:
+        assert rel_err(g[k], r64[k], fl[k]) < tol64, k
+
+
+def test_global_window_nets(ctx):
+    # wide nets (stars / balanced trees) larger than the window go through the
+    # global-memory window
+    cfg = W.RcConfig("wide", 40, [W.SizeClass(0.5, 600, 1500, N.SHAPE_BALANCED),
+                                  W.SizeClass(0.5, 600, 900, N.SHAPE_RANDOM)], seed=5)
+    f, s = _forest(cfg, ctx)
+    h = s.host_topology()
+    for method, tol in (("batched", 1e-5), ("compensated", 1e-6)):
+        s.solve(method, tuning=(256, 1, 0, 0))
+        assert s.plan_stats(method, (256, 1, 0, 0))["global_nets"] > 0
+        g = s.host()
+        r64 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "f64")
+        fl = floors(r64)
+        for k in OUT:
+            assert rel_err(g[k], r64[k], fl[k]) < tol, (method, k)
+
+
+def test_deep_chain_precision(ctx):
+    cfg = W.config3(num_nets=6, seed=4)
+    cfg.classes[0].max_nodes = 6000
+    f, s = _forest(cfg, ctx)
+    h = s.host_topology()
+    r64 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "f64")
+    fl = floors(r64)
+    s.solve("compensated")
+    g = s.host()
+    for k in OUT:
+        assert rel_err(g[k], r64[k], fl[k]) < 1e-6, k
+
+
+def test_edge_cases(ctx):
+    # single-node nets, two-node nets, empty net, net of one chain
+    ns = np.array([0, 1, 1, 3, 4, 9], np.int32)
+    pid = np.array([0, 0, 1, 0, 0, 1, 1, 2, 3], np.int32)
+    rng = np.random.default_rng(0)
+    pres = rng.uniform(0.01, 1, 9).astype(np.float32)
+    pres[[0, 1, 3, 4]] = 0
+    cap = rng.uniform(0.1, 2, 36).astype(np.float32)
+    si = rng.uniform(-20, 20, 20).astype(np.float32)
+    s = RcSolver.from_bfs_arrays(ns, pid, pres, cap, si, ctx)
+    r64 = O.rct_all(ns, pid, pres, cap, si, "f64")
+    fl = floors(r64)
+    for method in ("plain", "batched", "compensated"):
+        s.solve(method)
+        g = s.host()
+        for k in OUT:
+            assert rel_err(g[k], r64[k], fl[k]) < 1e-5, (method, k)
+
+
+def test_repeat_and_determinism(cfg0):
+    f, s = cfg0
+    s.solve("batched")
+    a = {k: v.copy() for k, v in s.host().items()}
+    for _ in range(3):
+        s.solve("batched")
+    b = s.host()
+    for k in OUT:
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+
+
+def test_subset_solve(ctx):
+    cfg = W.config1(num_nets=2000, seed=31)
+    f, s = _forest(cfg, ctx)
+    s.solve("batched")
+    full = {k: v.copy() for k, v in s.host().items()}
+    # perturb caps of some nets on the device, re-solve only those
+    rng = np.random.default_rng(1)
+    ids = np.sort(rng.choice(cfg.num_nets, 100, replace=False)).astype(np.int32)
+    ns = s.nodes_start.cpu().numpy()
+    cap = s.cap.clone()
+    for a in ids:
+        cap[ns[a] * 4: ns[a + 1] * 4] *= 1.1
+    s.cap = cap
+    s.solve_subset(torch.from_numpy(ids).to(s.device), "batched")
+    sub = s.host()
+    h = s.host_topology()
+    r64 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "f64")
+    fl = floors(r64)
+    mask = np.zeros(s.num_nodes * 4, bool)
+    for a in ids:
+        mask[ns[a] * 4: ns[a + 1] * 4] = True
+    for k in OUT:
+        assert rel_err(sub[k][mask], r64[k][mask], fl[k]) < 1e-5, k
+        assert np.array_equal(sub[k][~mask], full[k][~mask]), k
+
+
+def test_bfs_and_compute_host_api(ctx):
+    f = W.generate_rc(W.config0(num_nets=300, seed=9))
+    V, Nn = f.num_nodes, f.num_nets
+    a = N.RctEdgeArray()
+    bufs = dict(edges=f.edges.copy(), n2b=np.zeros(V, np.int32), pid=np.zeros(V, np.int32))
+    outs = {k: np.zeros(4 * V, np.float32) for k in OUT}
+    a.num_nets, a.total_num_nodes, a.total_num_edges = Nn, V, V - Nn
+    a.rct_nodes_start = N.addr(f.nodes_start)
+    a.rct_edges = N.addr(bufs["edges"])
+    a.rct_roots = N.addr(f.roots)
+    a.rct_edges_res = N.addr(f.edge_res)
+    a.rct_nodes_cap = N.addr(f.node_cap)
+    a.rct_node2bfs_order = N.addr(bufs["n2b"])
+    a.rct_pid = N.addr(bufs["pid"])
+    for k in OUT:
+        setattr(a, k, N.addr(outs[k]))
+    a.slew_in = N.addr(f.slew_in)
+    import ctypes as C
+    ctx.check(N.lib.otg_rct_bfs_and_compute(ctx.ptr, C.byref(a), N.BATCHED))
+    e, n2b, pid, pres, cap = oracle_inputs(f)
+    assert np.array_equal(bufs["n2b"], n2b) and np.array_equal(bufs["pid"], pid)
+    assert np.array_equal(bufs["edges"], e)
+    r64 = O.rct_all(f.nodes_start, pid, pres, cap, f.slew_in, "f64")
+    fl = floors(r64)
+    for k in OUT:
+        assert rel_err(outs[k], r64[k], fl[k]) < 1e-5, k
+
+
+def test_bad_topology_rejected(ctx):
+    ns = np.array([0, 3], np.int32)
+    pid = np.array([0, 1, 3], np.int32)  # node 2 points before the net start
+    s = RcSolver.from_bfs_arrays(ns, pid, np.ones(3, np.float32), np.ones(12, np.float32),
+                                 np.ones(4, np.float32), ctx)
+    with pytest.raises(N.OtgError) as ei:
+        s.solve("batched")
+    assert ei.value.status == 3
+
+
+def test_not_a_tree_rejected(ctx):
+    f = W.generate_rc(W.config0(num_nets=10, seed=3))
+    bad = f.edges.copy()
+    bad[0], bad[1] = bad[2], bad[3]  # duplicate an edge -> cycle + disconnected node
+    f.edges = bad
+    with pytest.raises(N.OtgError) as ei:
+        RcSolver.from_edges(f, ctx)
+    assert ei.value.status == 2
