@@ -79,6 +79,38 @@ def cpu_baseline(forest_host, topo, budget_s):
                       f"({int(ns[sample])} nodes), oracle dfs_f32 + slew, {dt:.2f}s"}
 
 
+def run_levelize(ctx, stream, sh, reps=5):
+    import torch
+
+    from opentimer_amd import workloads as W
+    from opentimer_amd.levelize import Levelizer
+    dcfg = W.config2()
+    dag = W.generate_dag(dcfg, nthreads=16)
+    csr = W.fanin_csr(dag)
+    L = Levelizer(csr, ctx)
+    for _ in range(2):
+        L.run(stream=sh)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.run(stream=sh)
+    torch.cuda.synchronize()
+    lms = (time.perf_counter() - t0) * 1e3 / reps  # includes the per-batch host checks
+    levels = L.num_ends - 1
+    m = len(csr.fanin)
+    lev = {"workload": dcfg.name, "pins": dcfg.num_pins, "arcs": m, "levels": levels,
+           "ms": lms, "edges_per_s": m / (lms / 1e3), "us_per_level": lms * 1e3 / levels,
+           "algorithmic_GBps": (20 * dcfg.num_pins + 12 * m + 4) / (lms / 1e3) / 1e9}
+    # exactness on the full-size graph: levels vs the oracle
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    _, ends, lv = L.host()
+    _, oends, olv = O.levelize(csr.n, csr.start, csr.fanin, csr.out_degree, csr.frontier)
+    lev["levels_bitexact_vs_oracle"] = bool(np.array_equal(lv, olv) and np.array_equal(ends, oends))
+    log(f"levelize: {lev}")
+    return lev
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -99,6 +131,10 @@ def main():
     if os.path.exists(args.tuning) and not args.autotune:
         import ctypes as C
         ctx.check(N.lib.otg_rct_tuning_load(ctx.ptr, args.tuning.encode(), None))
+
+    if args.only == "levelize":
+        print(json.dumps(run_levelize(ctx, stream, sh)))
+        return
 
     cfg = W.config1(num_nets=args.nets)
     t0 = time.perf_counter()
@@ -205,32 +241,8 @@ def main():
 
     # ---- levelization ---------------------------------------------------------
     lev = None
-    if not args.no_levelize and args.only in ("", "levelize"):
-        from opentimer_amd.levelize import Levelizer
-        dcfg = W.config2()
-        dag = W.generate_dag(dcfg, nthreads=16)
-        csr = W.fanin_csr(dag)
-        L = Levelizer(csr, ctx)
-        for _ in range(2):
-            L.run(stream=sh)
-        reps = 5
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(reps):
-            L.run(stream=sh)
-        b.record(stream)
-        torch.cuda.synchronize()
-        lms = a.elapsed_time(b) / reps
-        levels = L.num_ends - 1
-        m = len(csr.fanin)
-        lev = {"workload": dcfg.name, "pins": dcfg.num_pins, "arcs": m, "levels": levels,
-               "ms": lms, "edges_per_s": m / (lms / 1e3), "us_per_level": lms * 1e3 / levels,
-               "algorithmic_GBps": (20 * dcfg.num_pins + 12 * m + 4) / (lms / 1e3) / 1e9}
-        log(f"levelize: {lev}")
-        if args.only == "levelize":
-            print(json.dumps(lev))
-            return
+    if not args.no_levelize:
+        lev = run_levelize(ctx, stream, sh)
 
     ms = result["ms"]
     alg_bytes = 120.0 * V + 20.0 * Nn

@@ -19,7 +19,7 @@
  *     driver slew (the TAU/SPEF convention kOhm * fF = ps);
  *   - every call returns an otg_status; on failure otg_last_error() holds a
  *     message.  Entry points that take an otg_stream are asynchronous on that
- *     stream (NULL = the context's own stream) and use device pointers; the
+ *     stream (NULL = the device's default stream) and use device pointers; the
  *     reference-shaped entry points (otg_rct_bfs_and_compute,
  *     otg_toposort_compute) take host pointers and are synchronous, like the
  *     functions they replace.
@@ -51,7 +51,7 @@ typedef enum otg_status {
 
 typedef struct otg_context otg_context;   /* device, stream, scratch, tuning */
 typedef struct otg_rct_plan otg_rct_plan; /* per-topology work list          */
-typedef void* otg_stream;                 /* a hipStream_t                   */
+typedef void* otg_stream;                 /* a hipStream_t; NULL = default   */
 
 /* Context.  No reference counterpart: the reference uses the implicit CUDA
  * device 0 and the default stream (ot/cuda/utils.cuh:12-57). */
@@ -230,11 +230,11 @@ otg_status otg_toposort_compute(otg_context* ctx, int32_t n, int32_t num_edges,
                                 int32_t* frontiers, int32_t* frontiers_ends,
                                 int32_t* num_ends, int32_t* level_of_pin);
 
-/* Device-resident levelization: same semantics on device arrays, one
- * persistent cooperative launch (a grid barrier per level, no host round
- * trip).  out_degree is not modified (a device copy is consumed).
- * frontiers_ends needs n+2 entries; *num_ends is written to host memory
- * (the call synchronises `stream` to read it). */
+/* Device-resident levelization: same semantics on device arrays.  The level
+ * kernels are launched back-to-back on `stream` with no host round trip per
+ * level (the host reads one flag per batch of 32 levels).  out_degree is not
+ * modified (a device copy is consumed).  frontiers_ends needs n+2 entries;
+ * *num_ends is written to host memory.  Synchronous. */
 otg_status otg_levelize(otg_context* ctx, int32_t n, int32_t num_edges,
                         const int32_t* fanin_start, const int32_t* fanin,
                         const int32_t* out_degree, const int32_t* first_frontier,

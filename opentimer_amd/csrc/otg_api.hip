@@ -15,7 +15,8 @@
 otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_size,
                                    const int32_t* fanin_start, const int32_t* fanin,
                                    int32_t* out_copy, int32_t* frontiers, int32_t* ends,
-                                   int32_t* level, int32_t* counters, hipStream_t s);
+                                   int32_t* level, int32_t* scratch, int32_t* num_ends_host,
+                                   hipStream_t s);
 
 namespace {
 thread_local std::string g_last_error;  // errors raised before a context exists
@@ -65,7 +66,6 @@ extern "C" otg_status otg_create(int device, otg_context** out) {
   ctx->device = device;
   auto fail = [&](const char* what, hipError_t e) {
     otg_set_error(nullptr, "%s: %s", what, hipGetErrorString(e));
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return OTG_ERR_DEVICE;
   };
@@ -76,8 +76,10 @@ extern "C" otg_status otg_create(int device, otg_context** out) {
   if (e != hipSuccess) return fail("hipGetDeviceProperties", e);
   ctx->num_cus = prop.multiProcessorCount;
   ctx->lds_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
-  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) return fail("hipStreamCreate", e);
+  // Work the library issues for itself (plan building, host-pointer entry
+  // points) goes to the device's default stream, so it is ordered with the
+  // caller's default-stream copies and kernels.
+  ctx->stream = nullptr;
   *out = ctx;
   return OTG_OK;
 }
@@ -85,10 +87,7 @@ extern "C" otg_status otg_create(int device, otg_context** out) {
 extern "C" otg_status otg_destroy(otg_context* ctx) {
   if (!ctx) return OTG_OK;
   hipSetDevice(ctx->device);
-  if (ctx->stream) {
-    hipStreamSynchronize(ctx->stream);
-    hipStreamDestroy(ctx->stream);
-  }
+  hipStreamSynchronize(ctx->stream);
   delete ctx;
   return OTG_OK;
 }
@@ -292,20 +291,18 @@ extern "C" otg_status otg_levelize(otg_context* ctx, int32_t n, int32_t num_edge
     OTG_HIP(ctx, hipStreamSynchronize(s));
     return OTG_OK;
   }
-  int32_t *out_copy = nullptr, *counters = nullptr;
+  int32_t *out_copy = nullptr, *scratch = nullptr;
   OTG_HIP(ctx, hipMallocAsync((void**)&out_copy, sizeof(int32_t) * n, s));
-  OTG_HIP(ctx, hipMallocAsync((void**)&counters, sizeof(int32_t) * 4, s));
+  OTG_HIP(ctx, hipMallocAsync((void**)&scratch, sizeof(int32_t) * ((int64_t)n + 3), s));
   OTG_HIP(ctx, hipMemcpyAsync(out_copy, out_degree, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
   if (first_size > 0 && first_frontier != frontiers)
     OTG_HIP(ctx, hipMemcpyAsync(frontiers, first_frontier, sizeof(int32_t) * first_size,
                                 hipMemcpyDeviceToDevice, s));
-  otg_status st = otg_levelize_launch_raw(ctx, n, first_size, fanin_start, fanin, out_copy, frontiers,
-                                          frontiers_ends, level_of_pin, counters, s);
   int32_t ne = 0;
-  if (st == OTG_OK)
-    OTG_HIP(ctx, hipMemcpyAsync(&ne, counters + 3, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  otg_status st = otg_levelize_launch_raw(ctx, n, first_size, fanin_start, fanin, out_copy, frontiers,
+                                          frontiers_ends, level_of_pin, scratch, &ne, s);
   OTG_HIP(ctx, hipFreeAsync(out_copy, s));
-  OTG_HIP(ctx, hipFreeAsync(counters, s));
+  OTG_HIP(ctx, hipFreeAsync(scratch, s));
   OTG_HIP(ctx, hipStreamSynchronize(s));
   if (st != OTG_OK) return st;
   *num_ends = ne;

@@ -4,14 +4,22 @@
 // (rct.cu:314-359).
 //
 // One wavefront per net.  The net's undirected edges are turned into an
-// adjacency list in LDS (small nets) or in global scratch (large nets); the
-// BFS runs frontier by frontier with ballot compaction; the ordering is a
-// stable counting sort by distance in which node ids are visited from the
-// largest down, so that the position of node u is
+// adjacency list in LDS (nets up to kLdsNodes) or in global scratch (larger
+// nets); the BFS runs frontier by frontier with ballot compaction; the
+// ordering is a stable counting sort by distance in which node ids are
+// visited from the largest down, so that the position of node u is
 //   #(nodes with smaller distance) + #(nodes with equal distance, larger id)
 // -- the order the reference's bucket-end counting sort produces.  Work is
 // O(nodes) per net (the reference rescans every edge once per BFS level).
+//
+// Ordering inside the wavefront: LDS operations of one wavefront execute in
+// issue order, so the LDS path only needs the compiler to keep program
+// order.  The global-scratch path reads and writes scratch with L2-coherent
+// atomic accesses (no stale L1 lines) and waits for its own outstanding
+// memory operations between phases.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "otg_internal.hpp"
 
@@ -36,10 +44,29 @@ struct BfsParams {
   int32_t* bad;       // smallest failing net id
 };
 
+template <bool GLOBAL>
+struct Mem {
+  __device__ __forceinline__ static int ld(const int32_t* p) {
+    if constexpr (GLOBAL) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *p;
+  }
+  __device__ __forceinline__ static void st(int32_t* p, int v) {
+    if constexpr (GLOBAL) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+  }
+  __device__ __forceinline__ static void sync() {
+    if constexpr (GLOBAL) __builtin_amdgcn_s_waitcnt(0);  // own loads/stores done
+    __builtin_amdgcn_wave_barrier();
+    __asm__ __volatile__("" ::: "memory");
+  }
+};
+
 // Scratch layout per net with n nodes (ints): off[n+1] | adj[2n] | dist[n] |
 // frontier[n] | count[n+1]
 template <bool GLOBAL>
-__device__ void bfs_net(const BfsParams& P, int net, int32_t* base, int lane) {
+__device__ __forceinline__ void bfs_net(const BfsParams& P, int net, int32_t* base,
+                                        int32_t* lds_free, int lds_cap, int lane) {
+  using M = Mem<GLOBAL>;
   const int off = P.nodes_start[net];
   const int n = P.nodes_start[net + 1] - off;
   if (n <= 0) return;
@@ -50,20 +77,16 @@ __device__ void bfs_net(const BfsParams& P, int net, int32_t* base, int lane) {
   int32_t* dist = adj + 2 * n;       // n
   int32_t* front = dist + n;         // n
   int32_t* cnt = front + n;          // n+1
-  auto sync = [&]() {
-    if constexpr (GLOBAL) __threadfence();
-    __syncthreads();
-  };
   int ok = 1;
   const int root = P.roots[net];
   if (root < 0 || root >= n) ok = 0;
   // degrees
-  for (int k = lane; k <= n; k += kWave) adjoff[k] = 0;
-  for (int k = lane; k < n; k += kWave) {
-    dist[k] = -1;
-    cnt[k] = 0;
+  for (int k = lane; k <= n; k += kWave) {
+    M::st(&adjoff[k], 0);
+    M::st(&cnt[k], 0);
   }
-  sync();
+  for (int k = lane; k < n; k += kWave) M::st(&dist[k], -1);
+  M::sync();
   for (int k = lane; k < ne; k += kWave) {
     const int s = e[2 * k], t = e[2 * k + 1];
     if (s < 0 || s >= n || t < 0 || t >= n || s == t) {
@@ -73,7 +96,7 @@ __device__ void bfs_net(const BfsParams& P, int net, int32_t* base, int lane) {
       atomicAdd(&adjoff[t + 1], 1);
     }
   }
-  sync();
+  M::sync();
   ok = __all(ok);
   if (!ok) {
     if (lane == 0) atomicMin(P.bad, net);
@@ -84,107 +107,117 @@ __device__ void bfs_net(const BfsParams& P, int net, int32_t* base, int lane) {
     int carry = 0;
     for (int b = 0; b <= n; b += kWave) {
       const int k = b + lane;
-      int v = k <= n ? adjoff[k] : 0;
+      int v = k <= n ? M::ld(&adjoff[k]) : 0;
 #pragma unroll
       for (int d = 1; d < kWave; d <<= 1) {
         const int t = __shfl_up(v, d);
         if (lane >= d) v += t;
       }
-      if (k <= n) adjoff[k] = v + carry;
+      if (k <= n) M::st(&adjoff[k], v + carry);
       carry += __shfl(v, kWave - 1);
     }
   }
-  sync();
+  M::sync();
   // fill adjacency (cnt[] as per-node fill cursor)
   for (int k = lane; k < ne; k += kWave) {
     const int s = e[2 * k], t = e[2 * k + 1];
-    adj[adjoff[s] + atomicAdd(&cnt[s], 1)] = t;
-    adj[adjoff[t] + atomicAdd(&cnt[t], 1)] = s;
+    M::st(&adj[M::ld(&adjoff[s]) + atomicAdd(&cnt[s], 1)], t);
+    M::st(&adj[M::ld(&adjoff[t]) + atomicAdd(&cnt[t], 1)], s);
   }
-  sync();
-  // BFS frontier by frontier
   if (lane == 0) {
-    dist[root] = 0;
-    front[0] = root;
+    M::st(&dist[root], 0);
+    M::st(&front[0], root);
   }
-  sync();
-  int fb = 0, fe = 1, level = 0, reached = 1;
+  M::sync();
+  // BFS frontier by frontier
+  int fb = 0, fe = 1, level = 0;
   while (fb < fe) {
     int tail = fe;
     for (int b = fb; b < fe; b += kWave) {
       const int k = b + lane;
-      const int u = k < fe ? front[k] : -1;
-      const int jb = u >= 0 ? adjoff[u] : 0, je = u >= 0 ? adjoff[u + 1] : 0;
+      const int u = k < fe ? M::ld(&front[k]) : -1;
+      const int jb = u >= 0 ? M::ld(&adjoff[u]) : 0, je = u >= 0 ? M::ld(&adjoff[u + 1]) : 0;
       for (int j = jb;; ++j) {
         const bool has = j < je;
         if (!__any(has)) break;
         int v = -1;
         bool take = false;
         if (has) {
-          v = adj[j];
+          v = M::ld(&adj[j]);
           // a tree reaches v from exactly one neighbour; the CAS keeps a
           // malformed input (cycle) from queueing a node twice
-          if (dist[v] < 0 && atomicCAS(&dist[v], -1, level + 1) == -1) take = true;
+          if (M::ld(&dist[v]) < 0 && atomicCAS(&dist[v], -1, level + 1) == -1) take = true;
         }
         const uint64_t m = __ballot(take);
-        if (take) front[tail + __popcll(m & ((1ull << lane) - 1ull))] = v;
+        if (take) M::st(&front[tail + __popcll(m & ((1ull << lane) - 1ull))], v);
         tail += __popcll(m);
       }
     }
-    sync();
+    M::sync();
     fb = fe;
     fe = tail;
-    reached = tail;
     ++level;
   }
-  if (reached != n) {
+  if (fe != n) {
     if (lane == 0) atomicMin(P.bad, net);
     return;
   }
   // level sizes -> level start positions (cnt[d])
-  for (int k = lane; k <= n; k += kWave) cnt[k] = 0;
-  sync();
-  for (int k = lane; k < n; k += kWave) atomicAdd(&cnt[dist[k]], 1);
-  sync();
+  for (int k = lane; k <= n; k += kWave) M::st(&cnt[k], 0);
+  M::sync();
+  for (int k = lane; k < n; k += kWave) atomicAdd(&cnt[M::ld(&dist[k])], 1);
+  M::sync();
   {
     int carry = 0;
     for (int b = 0; b < level; b += kWave) {
       const int k = b + lane;
-      int v = k < level ? cnt[k] : 0;
+      int v = k < level ? M::ld(&cnt[k]) : 0;
       const int own = v;
 #pragma unroll
       for (int d = 1; d < kWave; d <<= 1) {
         const int t = __shfl_up(v, d);
         if (lane >= d) v += t;
       }
-      if (k < level) cnt[k] = v - own + carry;  // exclusive start of level k
+      if (k < level) M::st(&cnt[k], v - own + carry);  // exclusive start of level k
       carry += __shfl(v, kWave - 1);
     }
   }
-  sync();
-  // stable ranking: ids from the largest down, same-distance ids in
-  // descending order take consecutive positions
+  M::sync();
+  // a large net ranks with its level cursors in LDS when they fit
+  bool cur_lds = false;
+  if constexpr (GLOBAL) {
+    if (level <= lds_cap) {
+      for (int k = lane; k < level; k += kWave) lds_free[k] = M::ld(&cnt[k]);
+      cur_lds = true;
+      M::sync();
+    }
+  }
+  // stable ranking: ids from the largest down; in each chunk the lanes that
+  // share a distance take consecutive positions, larger ids first, from a
+  // base the group leader claims with one atomic on that level's cursor
   for (int top = n - 1; top >= 0; top -= kWave) {
     const int u = top - lane;  // lane 0 holds the largest id of the chunk
     const bool valid = u >= 0;
-    const int d = valid ? dist[u] : -1;
+    const int d = valid ? M::ld(&dist[u]) : -1;
     uint64_t remaining = __ballot(valid);
+    int pos = 0;
     while (remaining) {
       const int leader = __ffsll((unsigned long long)remaining) - 1;
       const int key = __shfl(d, leader);
       const uint64_t m = __ballot(valid && d == key);
-      if (valid && d == key) {
-        const int pos = cnt[key] + __popcll(m & ((1ull << lane) - 1ull));
-        P.node2bfs[off + u] = pos;
-        dist[u] = pos;  // reuse: dist now holds the position
-      }
-      sync();
-      if (lane == leader) cnt[key] += __popcll(m);
-      sync();
+      int basepos = 0;
+      if (lane == leader)
+        basepos = cur_lds ? atomicAdd(&lds_free[key], __popcll(m)) : atomicAdd(&cnt[key], __popcll(m));
+      basepos = __shfl(basepos, leader);
+      if (valid && d == key) pos = basepos + __popcll(m & ((1ull << lane) - 1ull));
       remaining &= ~m;
     }
+    if (valid) {
+      P.node2bfs[off + u] = pos;
+      M::st(&dist[u], pos);  // dist now holds the position
+    }
   }
-  sync();
+  M::sync();
   // orient edges, write pid / pres / cap in BFS order
   if (lane == 0) {
     P.pid[off] = 0;
@@ -194,7 +227,7 @@ __device__ void bfs_net(const BfsParams& P, int net, int32_t* base, int lane) {
   const float* res = P.edge_res + (off - net);
   for (int k = lane; k < ne; k += kWave) {
     int s = e[2 * k], t = e[2 * k + 1];
-    int ps = dist[s], pt = dist[t];
+    int ps = M::ld(&dist[s]), pt = M::ld(&dist[t]);
     if (ps > pt) {
       const int x = s; s = t; t = x;
       const int y = ps; ps = pt; pt = y;
@@ -204,7 +237,9 @@ __device__ void bfs_net(const BfsParams& P, int net, int32_t* base, int lane) {
     P.pid[off + pt] = pt - ps;
     P.pres[off + pt] = res[k];
   }
-  for (int k = lane; k < n; k += kWave) P.cap_bfs[(int64_t)off + dist[k]] = P.node_cap[(int64_t)off + k];
+  for (int k = lane; k < n; k += kWave)
+    P.cap_bfs[(int64_t)off + M::ld(&dist[k])] = P.node_cap[(int64_t)off + k];
+  M::sync();
 }
 
 __global__ __launch_bounds__(64) void rct_bfs_kernel(BfsParams P) {
@@ -213,11 +248,11 @@ __global__ __launch_bounds__(64) void rct_bfs_kernel(BfsParams P) {
   for (int net = blockIdx.x; net < P.num_nets; net += gridDim.x) {
     const int n = P.nodes_start[net + 1] - P.nodes_start[net];
     if (n <= kLdsNodes) {
-      bfs_net<false>(P, net, lds, lane);
+      bfs_net<false>(P, net, lds, lds, 0, lane);
     } else {
-      bfs_net<true>(P, net, P.gscratch + 6LL * P.nodes_start[net] + 2LL * net, lane);
+      bfs_net<true>(P, net, P.gscratch + 6LL * P.nodes_start[net] + 2LL * net, lds,
+                    6 * kLdsNodes + 2, lane);
     }
-    __syncthreads();
   }
 }
 
@@ -240,9 +275,15 @@ otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t
   P.cap_bfs = reinterpret_cast<float4*>(cap_bfs);
   P.bad = d_bad;
   int32_t* gs = nullptr;
-  OTG_HIP(ctx, hipMallocAsync((void**)&gs, sizeof(int32_t) * (6LL * in->total_num_nodes + 2LL * in->num_nets + 8), s));
+  OTG_HIP(ctx, hipMallocAsync((void**)&gs,
+                              sizeof(int32_t) * (6LL * in->total_num_nodes + 2LL * in->num_nets + 8),
+                              s));
   P.gscratch = gs;
-  const int grid = std::max(1, std::min(in->num_nets, ctx->num_cus * 16));
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&rct_bfs_kernel),
+                                                   64, 0) != hipSuccess || per_cu <= 0)
+    per_cu = 8;
+  const int grid = std::max(1, std::min(in->num_nets, ctx->num_cus * per_cu));
   hipLaunchKernelGGL(rct_bfs_kernel, dim3(grid), dim3(64), 0, s, P);
   hipError_t le = hipGetLastError();
   OTG_HIP(ctx, hipFreeAsync(gs, s));

@@ -11,7 +11,7 @@
 
 struct otg_context {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // default stream of the device
   int num_cus = 0;
   int lds_per_cu = 0;
   std::string last_error;
@@ -64,8 +64,10 @@ void otg_set_error(const otg_context* ctx, const char* fmt, ...);
     }                                                                             \
   } while (0)
 
+// NULL selects the device's default (null) stream.
 inline hipStream_t otg_pick_stream(const otg_context* ctx, otg_stream s) {
-  return s ? (hipStream_t)s : ctx->stream;
+  (void)ctx;
+  return (hipStream_t)s;
 }
 
 // Defaults of the batched launch, resolved against the method.

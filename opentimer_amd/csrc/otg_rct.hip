@@ -254,6 +254,16 @@ struct Ring {
   }
 };
 
+// The workgroup is one wavefront, and LDS operations of a wavefront execute
+// in issue order, so ordering LDS traffic between rounds only needs the
+// compiler to keep program order.  (__syncthreads() would also wait for every
+// outstanding global load and store, serialising memory latency into each
+// round.)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_wave_barrier();
+  __asm__ __volatile__("" ::: "memory");
+}
+
 struct WinIn {
   int32_t pid;
   float r;
@@ -276,14 +286,14 @@ __device__ __forceinline__ void process_item(const RctParams& P, RctLds<W, COMP>
     for (int k = lane; k < nz; k += kWave) R.zero(k);
     if (multi) {
       for (int k = lane; k < W / 32; k += kWave) S.rootbits[k] = 0u;
-      __syncthreads();
+      wave_sync();
       for (int j = it.net_begin + lane; j < it.net_end; j += kWave) {
         const int r = P.nodes_start[j] - pb;
         atomicOr(&S.rootbits[r >> 5], 1u << (r & 31));
       }
     }
     R.fence();
-    __syncthreads();
+    wave_sync();
   }
   auto is_root = [&](int i) -> bool {
     if (multi) return (S.rootbits[(i - pb) >> 5] >> ((i - pb) & 31)) & 1u;
@@ -320,9 +330,9 @@ __device__ __forceinline__ void process_item(const RctParams& P, RctLds<W, COMP>
       S.sres[i - pb] = cur.r;
     }
     S.cnt[lane] = 0;
-    __syncthreads();
+    wave_sync();
     if (pin) atomicAdd(&S.cnt[p - wk], 1);
-    __syncthreads();
+    wave_sync();
     bool done = !valid;
     float cap4[4];
     unpack(cur.c, cap4);
@@ -372,15 +382,15 @@ __device__ __forceinline__ void process_item(const RctParams& P, RctLds<W, COMP>
         done = true;
       }
       R.fence();
-      __syncthreads();
+      wave_sync();
       if (__all(done)) break;
     }
     if (!keep && valid) R.zero(R.slot(i));
-    __syncthreads();
+    wave_sync();
     cur = nxt;
   }
   R.fence();
-  __syncthreads();
+  wave_sync();
 
   // ---- top-down sweep ---------------------------------------------------------
   struct TdIn {
@@ -460,7 +470,7 @@ __device__ __forceinline__ void process_item(const RctParams& P, RctLds<W, COMP>
       }
     }
     S.cnt[lane] = 0;
-    __syncthreads();
+    wave_sync();
     bool done = !valid;
     float d[4], dl[4], m[4], ml[4];
     for (int round = 0; round <= kWave; ++round) {
@@ -489,7 +499,7 @@ __device__ __forceinline__ void process_item(const RctParams& P, RctLds<W, COMP>
         done = true;
       }
       R.fence();
-      __syncthreads();
+      wave_sync();
       if (__all(done)) break;
     }
     if (valid) {
@@ -532,7 +542,7 @@ __device__ __forceinline__ void process_item(const RctParams& P, RctLds<W, COMP>
     tc = tn;
   }
   R.fence();
-  __syncthreads();
+  wave_sync();
 }
 
 template <int W, bool COMP>

@@ -34,7 +34,7 @@ class Levelizer:
         self.num_ends = 0
 
     def run(self, stream=None):
-        stream = stream or torch.cuda.current_stream(self.device).cuda_stream
+        stream = torch.cuda.current_stream(self.device).cuda_stream if stream is None else stream
         ne = C.c_int32()
         self.ctx.check(N.lib.otg_levelize(
             self.ctx.ptr, self.n, self.m, self.start.data_ptr(), self.fanin.data_ptr(),

@@ -13,6 +13,14 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libotg.so")
 
+try:
+    # libotg.so needs libamdhip64.so.7.  When PyTorch is present its bundled
+    # HIP runtime must be the one loaded (one runtime per process): importing
+    # torch first makes the dynamic linker bind libotg to it.
+    import torch  # noqa: F401
+except ImportError:  # the C ABI works without torch
+    torch = None
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"{LIB_PATH} is not built; run `python -m opentimer_amd.build` "

@@ -75,7 +75,7 @@ class RcSolver:
         return self
 
     def bfs(self, edges, roots, res, ncap, stream=None):
-        stream = stream or torch_stream(self.device)
+        stream = torch_stream(self.device) if stream is None else stream
         e = N.RctEdges(self.num_nets, self.num_nodes, self.nodes_start.data_ptr(), edges.data_ptr(),
                        roots.data_ptr(), res.data_ptr(), ncap.data_ptr())
         self.ctx.check(N.lib.otg_rct_bfs(self.ctx.ptr, C.byref(e), self.node2bfs.data_ptr(),
@@ -121,7 +121,7 @@ class RcSolver:
     def solve(self, method="batched", stream=None, names=OUTPUTS, tuning=None):
         m = METHODS[method]
         plan = None if m == N.PLAIN else self.plan(method, tuning)
-        stream = stream or torch_stream(self.device)
+        stream = torch_stream(self.device) if stream is None else stream
         res = self.result(names)
         self.ctx.check(N.lib.otg_rct_solve(self.ctx.ptr, plan, C.byref(self.tree()), C.byref(res),
                                            m, stream))
@@ -130,7 +130,7 @@ class RcSolver:
     def solve_subset(self, net_ids: torch.Tensor, method="batched", stream=None, names=OUTPUTS):
         m = METHODS[method]
         plan = self.plan(method)
-        stream = stream or torch_stream(self.device)
+        stream = torch_stream(self.device) if stream is None else stream
         res = self.result(names)
         self.ctx.check(N.lib.otg_rct_solve_subset(self.ctx.ptr, plan,
                                                   C.byref(self.tree()), C.byref(res), m,
