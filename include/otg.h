@@ -110,7 +110,8 @@ typedef struct otg_rct_result {
  * values persisted by the autotuner, see otg_rct_autotune). */
 typedef struct otg_rct_tuning {
   int32_t ring_slots;     /* on-chip window per wavefront: 256, 512 or 1024  */
-  int32_t waves_per_wg;   /* wavefronts per workgroup: 1, 2 or 4            */
+  int32_t window_k;       /* max positions per lane of a streamed net's    */
+                          /* window (1, 2 or 4; 0 = 4)                      */
   int32_t small_target;   /* node budget of one packed group of small nets  */
   int32_t grid_per_cu;    /* persistent workgroups per CU (0 = occupancy)   */
 } otg_rct_tuning;

@@ -359,9 +359,9 @@ static bool write_tuning(const char* path, const otg_rct_tuning& t, otg_rct_meth
   FILE* f = fopen(path, "w");
   if (!f) return false;
   fprintf(f,
-          "{\"ring_slots\": %d, \"waves_per_wg\": %d, \"small_target\": %d, \"grid_per_cu\": %d, "
+          "{\"ring_slots\": %d, \"window_k\": %d, \"small_target\": %d, \"grid_per_cu\": %d, "
           "\"method\": %d, \"ms\": %.6f}\n",
-          t.ring_slots, t.waves_per_wg, t.small_target, t.grid_per_cu, (int)m, ms);
+          t.ring_slots, t.window_k, t.small_target, t.grid_per_cu, (int)m, ms);
   fclose(f);
   return true;
 }
@@ -389,7 +389,7 @@ extern "C" otg_status otg_rct_tuning_load(otg_context* ctx, const char* path, ot
   fclose(f);
   otg_rct_tuning r{};
   r.ring_slots = json_int(s, "ring_slots", 0);
-  r.waves_per_wg = json_int(s, "waves_per_wg", 1);
+  r.window_k = json_int(s, "window_k", 0);
   r.small_target = json_int(s, "small_target", 0);
   r.grid_per_cu = json_int(s, "grid_per_cu", 0);
   ctx->tuning = r;
@@ -407,17 +407,15 @@ extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tre
   if (st != OTG_OK) return st;
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  const int rings[] = {256, 512, 1024};
-  const int targets_frac[] = {2, 1};  // small_target = ring / frac
   std::vector<otg_rct_tuning> cands;
-  for (int W : rings)
-    for (int fr : targets_frac)
-      for (int g : {0, 2, 4}) {
+  for (int W : {512, 1024, 256})
+    for (int target : {256, 512})
+      for (int kk : {1, 2, 4}) {
         otg_rct_tuning t{};
         t.ring_slots = W;
-        t.waves_per_wg = 1;
-        t.small_target = W / fr;
-        t.grid_per_cu = g;
+        t.window_k = kk;
+        t.small_target = std::min(target, W);
+        t.grid_per_cu = 0;
         cands.push_back(t);
       }
   hipEvent_t e0, e1;

@@ -30,7 +30,7 @@ struct RctItem {
   int32_t pos_begin, pos_end;
   int32_t net_begin, net_end;
   int32_t kind;
-  int32_t pad;
+  int32_t k;        // stream items: positions per lane of a window (1, 2 or 4)
   int64_t scratch;  // float offset into the plan scratch (stream/global items)
 };
 

@@ -129,6 +129,39 @@ __device__ __forceinline__ void unpack(float4 v, float (&a)[4]) {
 }
 __device__ __forceinline__ float4 pack(const float (&a)[4]) { return make_float4(a[0], a[1], a[2], a[3]); }
 
+// Global-memory accesses through address-space-1 pointers: the compiler then
+// always emits global_load/global_store.  (A generic pointer it cannot prove
+// global becomes a FLAT access, which also counts against the LDS counter,
+// so every LDS wait would wait for HBM latency too.)
+#define OTG_GLOBAL __attribute__((address_space(1)))
+typedef float otg_f4v __attribute__((ext_vector_type(4)));
+typedef int otg_i4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int gld(const int32_t* p) { return *(const OTG_GLOBAL int32_t*)p; }
+__device__ __forceinline__ int64_t gld(const int64_t* p) { return *(const OTG_GLOBAL int64_t*)p; }
+__device__ __forceinline__ float gld(const float* p) { return *(const OTG_GLOBAL float*)p; }
+__device__ __forceinline__ float4 gld(const float4* p) {
+  const otg_f4v v = *(const OTG_GLOBAL otg_f4v*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst(float4* p, const float4& v) {
+  otg_f4v t = {v.x, v.y, v.z, v.w};
+  *(OTG_GLOBAL otg_f4v*)p = t;
+}
+__device__ __forceinline__ RctItem gld(const RctItem* p) {
+  static_assert(sizeof(RctItem) == 32, "RctItem layout");
+  const otg_i4v a = *(const OTG_GLOBAL otg_i4v*)p;
+  const otg_i4v b = *((const OTG_GLOBAL otg_i4v*)p + 1);
+  RctItem it;
+  it.pos_begin = a.x;
+  it.pos_end = a.y;
+  it.net_begin = a.z;
+  it.net_end = a.w;
+  it.kind = b.x;
+  it.k = b.y;
+  it.scratch = (int64_t)(((uint64_t)(uint32_t)b.w << 32) | (uint32_t)b.z);
+  return it;
+}
+
 // ---------------------------------------------------------------------------
 // Batched kernel.
 // ---------------------------------------------------------------------------
@@ -146,6 +179,7 @@ struct RctParams {
   float4* slew;
   const RctItem* items;
   int32_t num_items;
+  int32_t num_nets_total;
   int32_t subset_count;
   const int32_t* subset;
   const int32_t* net_maxpid;
@@ -156,6 +190,7 @@ struct RctParams {
 
 template <int W, bool COMP>
 struct RctLds {
+  static constexpr int KP = (W < 256 ? W : 256) / 64;  // positions per lane, packed items
   float4 ra[W];                // bottom-up: load acc (hi) | top-down: delay (hi)
   float4 rb[W];                // bottom-up: L acc (hi)    | top-down: impulse (hi)
   float4 ral[COMP ? W : 1];    // low parts (compensated mode)
@@ -163,7 +198,7 @@ struct RctLds {
   int32_t spid[W];             // parent offsets of a packed item, kept for the top-down sweep
   float sres[W];               // resistances of a packed item
   uint32_t rootbits[W / 32];   // driver positions of a packed group
-  int32_t cnt[kWave];          // per-lane child counters / done flags of a window
+  int32_t cnt[kWave * KP];     // per-position child counters / done flags of a window
 };
 
 // Window storage: either the LDS ring (index mod W) or a per-node region in
@@ -196,6 +231,9 @@ struct Ring {
     if constexpr (COMP) {
       unpack(lda(al, k), xl);
       unpack(lda(bl, k), yl);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) xl[c] = yl[c] = 0.f;
     }
   }
   __device__ __forceinline__ void put(int k, const float (&x)[4], const float (&y)[4],
@@ -264,281 +302,413 @@ __device__ __forceinline__ void wave_sync() {
   __asm__ __volatile__("" ::: "memory");
 }
 
-struct WinIn {
-  int32_t pid;
-  float r;
-  float4 c;
+// Inputs of one window of 64*K positions; lane l holds positions
+// wk + k*64 + l, k = 0..K-1 (each k is a coalesced 64-position block).
+template <int K>
+struct BuIn {
+  int32_t pid[K];
+  float r[K];
+  float4 c[K];
+};
+template <int K, bool KEEP>
+struct TdIn {
+  int32_t pid[K];
+  float r[K];
+  float4 ld[KEEP ? 1 : K];   // streamed items: load and L read back
+  float4 L[KEEP ? 1 : K];
+  float4 si[KEEP ? K : 1];   // driver slew: per position for a packed group
 };
 
-template <int W, bool COMP, bool GLOBAL>
+// Counter slot of window-relative position q: each lane's K counters are
+// contiguous, so a lane reads them with wide LDS loads.
+template <int K>
+__device__ __forceinline__ int cidx(int q) {
+  return (q & (kWave - 1)) * K + (q >> 6);
+}
+
+// Buffer resources (V#) over the item's slice of each array.  Loads past the
+// slice read 0 and stores past it are dropped by the hardware range check,
+// so window code needs no per-lane branch around its memory operations and
+// every window issues a fixed number of them.  A null array gets a
+// zero-size slice (all its stores dropped).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef unsigned int otg_u4v __attribute__((ext_vector_type(4)));
+constexpr int kRsrcWord3 = 0x00020000;  // raw, untyped buffer (gfx9 family)
+
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, int64_t elem0, int elem_bytes,
+                                          int count) {
+  if (!base) return __builtin_amdgcn_make_buffer_rsrc(nullptr, 0, 0, kRsrcWord3);
+  return __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<char*>(static_cast<const char*>(base) + elem0 * elem_bytes), 0,
+      count * elem_bytes, kRsrcWord3);
+}
+__device__ __forceinline__ float4 bload4(Rsrc r, int byte_off) {
+  const otg_u4v v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0);
+  const otg_f4v f = __builtin_bit_cast(otg_f4v, v);
+  return make_float4(f.x, f.y, f.z, f.w);
+}
+__device__ __forceinline__ int bload_i(Rsrc r, int byte_off) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0);
+}
+__device__ __forceinline__ float bload_f(Rsrc r, int byte_off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+__device__ __forceinline__ void bstore4(Rsrc r, int byte_off, const float (&a)[4]) {
+  const otg_f4v f = {a[0], a[1], a[2], a[3]};
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(otg_u4v, f), r, byte_off, 0, 0);
+}
+
+struct ItemBufs {
+  Rsrc pid, res, cap, load, gl, delay, ldelay, beta, imp, slew;
+};
+
+template <int W, bool COMP>
+struct ItemCtx {
+  RctLds<W, COMP>& S;
+  int pb, pe;
+  bool multi;
+  int lane;
+  ItemBufs B;
+  __device__ __forceinline__ bool is_root(int i) const {
+    if (multi) return (S.rootbits[(i - pb) >> 5] >> ((i - pb) & 31)) & 1u;
+    return i == pb;
+  }
+};
+
+// ---- bottom-up window: load_i = C_i + sum load_c, L_i = sum (L_c + R_c load_c^2)
+// Each round reads every position's counter and accumulator slot
+// unconditionally (one LDS round trip for the whole window), computes every
+// position, and writes only the positions that are ready (their in-window
+// children are all done).  Reads of a not-yet-ready slot are discarded.
+template <int K, bool KEEP, int W, bool COMP, bool GLOBAL>
+__device__ __forceinline__ void bu_window(const ItemCtx<W, COMP>& X, const Ring<GLOBAL, COMP>& R,
+                                          const int wk, const BuIn<K>& in) {
+  constexpr bool STASH = KEEP && !GLOBAL;  // a packed group keeps pid/R in LDS
+  const int lane = X.lane;
+  int par[K], slot[K];
+  uint32_t pending = 0, inwin = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = wk + k * kWave + lane;
+    const bool valid = i < X.pe;
+    const bool root = valid && X.is_root(i);
+    par[k] = (valid && !root) ? i - in.pid[k] : -1;
+    slot[k] = R.slot(i);
+    if (valid) pending |= 1u << k;
+    if (par[k] >= wk) inwin |= 1u << k;
+    X.S.cnt[lane * K + k] = 0;
+    if (STASH && valid) {
+      X.S.spid[i - X.pb] = in.pid[k];
+      X.S.sres[i - X.pb] = in.r[k];
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((inwin >> k) & 1u) atomicAdd(&X.S.cnt[cidx<K>(par[k] - wk)], 1);
+  wave_sync();
+  for (int round = 0; round <= kWave * K; ++round) {
+    int cn[K];
+    float xa[K][4], xb[K][4], xal[K][4], xbl[K][4];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      cn[k] = X.S.cnt[lane * K + k];
+      R.get(slot[k], xa[k], xb[k], xal[k], xbl[k]);
+    }
+    uint32_t ready = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (((pending >> k) & 1u) && cn[k] == 0) ready |= 1u << k;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      float cap4[4];
+      unpack(in.c[k], cap4);
+      float ld[4], ldl[4], L[4], Ll[4], w[4], wl[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if constexpr (COMP) {
+          const ff t = ff_add(two_sum(cap4[c], xa[k][c]), ff{xal[k][c], 0.f});
+          ld[c] = t.hi;
+          ldl[c] = t.lo;
+          L[c] = xb[k][c];
+          Ll[c] = xbl[k][c];
+          const ff q = ff_add(ff{L[c], Ll[c]}, ff_mul_f(ff_mul(t, t), in.r[k]));
+          w[c] = q.hi;
+          wl[c] = q.lo;
+        } else {
+          ld[c] = cap4[c] + xa[k][c];
+          ldl[c] = 0.f;
+          L[c] = xb[k][c];
+          Ll[c] = 0.f;
+          w[c] = L[c] + in.r[k] * ld[c] * ld[c];
+          wl[c] = 0.f;
+        }
+      }
+      if (!((ready >> k) & 1u)) continue;
+      R.put(slot[k], ld, L, ldl, Ll);  // final: no child adds to it any more
+      if (par[k] >= 0) {
+        R.add(R.slot(par[k]), ld, w, ldl, wl);
+        if ((inwin >> k) & 1u) atomicSub(&X.S.cnt[cidx<K>(par[k] - wk)], 1);
+      }
+    }
+    pending &= ~ready;
+    R.fence();
+    wave_sync();
+    if (__all(pending == 0)) break;
+  }
+  if constexpr (!KEEP) {
+    // streamed item: flush the window's load and L to global memory (a fixed
+    // number of stores per window, issued after the rounds, so the next
+    // window's prefetched loads can be waited for without waiting for these
+    // stores) and free the slots for positions W further on
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = wk + k * kWave + lane;
+      float ld[4], L[4], ldl[4], Ll[4];
+      R.get(slot[k], ld, L, ldl, Ll);
+      float o1[4], o2[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        o1[c] = COMP ? ld[c] + ldl[c] : ld[c];
+        o2[c] = COMP ? L[c] + Ll[c] : L[c];
+      }
+      bstore4(X.B.load, (i - X.pb) * 16, o1);
+      bstore4(X.B.gl, (i - X.pb) * 16, o2);
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < K; ++k) R.zero(slot[k]);
+  }
+  wave_sync();
+}
+
+// ---- top-down window: delay_i = delay_p + R_i load_i,
+//      impulse_i = impulse_p + (R_i load_i)^2 + 2 R_i L_i.  Each round reads
+//      every position's parent flag and parent slot together and lets the
+//      ready positions (parent done) write their own slot; the outputs of the
+//      whole window are computed and stored after the rounds, a fixed number
+//      of stores per window.
+template <int K, bool KEEP, int W, bool COMP, bool GLOBAL>
+__device__ __forceinline__ void td_window(const ItemCtx<W, COMP>& X, const Ring<GLOBAL, COMP>& R,
+                                          const int wk, const TdIn<K, KEEP>& in) {
+  const int lane = X.lane;
+  int par[K], slot[K], pslot[K], flag_idx[K];
+  float rk[K];
+  float ld[K][4], ldl[K][4], L[K][4], Ll[K][4];
+  uint32_t pending = 0, inwin = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = wk + k * kWave + lane;
+    const bool valid = i < X.pe;
+    const bool root = valid && X.is_root(i);
+    par[k] = (valid && !root) ? i - in.pid[k] : -1;
+    slot[k] = R.slot(i);
+    pslot[k] = par[k] >= 0 ? R.slot(par[k]) : slot[k];
+    rk[k] = root ? 0.f : in.r[k];
+    if (valid) pending |= 1u << k;
+    if (par[k] >= wk) inwin |= 1u << k;
+    flag_idx[k] = par[k] >= wk ? cidx<K>(par[k] - wk) : lane * K + k;
+    X.S.cnt[lane * K + k] = 0;
+    // own load and L: from the slot (whole item on chip) or the prefetch
+    if constexpr (KEEP) {
+      R.get(slot[k], ld[k], L[k], ldl[k], Ll[k]);
+    } else {
+      unpack(in.ld[KEEP ? 0 : k], ld[k]);
+      unpack(in.L[KEEP ? 0 : k], L[k]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) ldl[k][c] = Ll[k][c] = 0.f;
+    }
+  }
+  wave_sync();
+  for (int round = 0; round <= kWave * K; ++round) {
+    int fl[K];
+    float pd[K][4], pdl[K][4], pm[K][4], pml[K][4];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      fl[k] = X.S.cnt[flag_idx[k]];
+      R.get(pslot[k], pd[k], pm[k], pdl[k], pml[k]);
+    }
+    uint32_t ready = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (((pending >> k) & 1u) && (!((inwin >> k) & 1u) || fl[k] != 0)) ready |= 1u << k;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!((ready >> k) & 1u)) continue;
+      const float r = rk[k];
+      const bool has_par = par[k] >= 0;
+      float d[4], dl[4], m[4], ml[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float qd = has_par ? pd[k][c] : 0.f, qdl = has_par ? pdl[k][c] : 0.f;
+        const float qm = has_par ? pm[k][c] : 0.f, qml = has_par ? pml[k][c] : 0.f;
+        if constexpr (COMP) {
+          const ff uu = ff_mul_f(ff{ld[k][c], ldl[k][c]}, r);
+          const ff zz = ff_add(ff_mul(uu, uu), ff_mul_f(ff{L[k][c], Ll[k][c]}, 2.0f * r));
+          const ff dd = ff_add(ff{qd, qdl}, uu);
+          const ff mm = ff_add(ff{qm, qml}, zz);
+          d[c] = dd.hi;
+          dl[c] = dd.lo;
+          m[c] = mm.hi;
+          ml[c] = mm.lo;
+        } else {
+          const float u = r * ld[k][c];
+          d[c] = qd + u;
+          m[c] = qm + (u * u + 2.0f * r * L[k][c]);
+          dl[c] = ml[c] = 0.f;
+        }
+      }
+      R.put(slot[k], d, m, dl, ml);
+      X.S.cnt[lane * K + k] = 1;
+    }
+    pending &= ~ready;
+    R.fence();
+    wave_sync();
+    if (__all(pending == 0)) break;
+  }
+  // outputs of the window
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = wk + k * kWave + lane;
+    float d[4], dl[4], m[4], ml[4];
+    R.get(slot[k], d, m, dl, ml);
+    float sv[4];
+    unpack(in.si[KEEP ? k : 0], sv);
+    float o_ld[4], o_d[4], o_ldel[4], o_b[4], o_m[4], o_s[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if constexpr (COMP) {
+        // final per-node combination in FP64
+        const double dd = (double)d[c] + (double)dl[c];
+        const double md = (double)m[c] + (double)ml[c];
+        const double ldd = (double)ld[k][c] + (double)ldl[k][c];
+        const double Ld = (double)L[k][c] + (double)Ll[k][c];
+        o_ld[c] = (float)ldd;
+        o_d[c] = (float)dd;
+        o_m[c] = (float)md;
+        o_ldel[c] = (float)(dd * ldd + Ld);
+        o_b[c] = (float)(0.5 * (dd * dd + md));
+        const double q = sqrt((double)sv[c] * (double)sv[c] + md);
+        o_s[c] = (float)(sv[c] < 0.f ? -q : q);
+      } else {
+        o_ld[c] = ld[k][c];
+        o_d[c] = d[c];
+        o_m[c] = m[c];
+        o_ldel[c] = d[c] * ld[k][c] + L[k][c];
+        o_b[c] = 0.5f * (d[c] * d[c] + m[c]);
+        const float q = sqrtf(sv[c] * sv[c] + m[c]);
+        o_s[c] = sv[c] < 0.f ? -q : q;
+      }
+    }
+    const int off = (i - X.pb) * 16;
+    if constexpr (KEEP) bstore4(X.B.load, off, o_ld);
+    bstore4(X.B.delay, off, o_d);
+    bstore4(X.B.ldelay, off, o_ldel);
+    bstore4(X.B.beta, off, o_b);
+    bstore4(X.B.imp, off, o_m);
+    bstore4(X.B.slew, off, o_s);
+  }
+  wave_sync();
+}
+
+template <int K, int W, bool COMP>
+__device__ __forceinline__ void load_bu(const ItemCtx<W, COMP>& X, int wk, BuIn<K>& in) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int q = wk + k * kWave + X.lane - X.pb;  // item-relative position
+    in.pid[k] = bload_i(X.B.pid, q * 4);
+    in.r[k] = bload_f(X.B.res, q * 4);
+    in.c[k] = bload4(X.B.cap, q * 16);
+  }
+}
+
+// One work item: windows of 64*K positions, bottom-up over the windows from
+// last to first, then top-down from first to last.  The next window's
+// global inputs are requested before the current window is processed.
+// KEEP: the whole item fits the on-chip window (packed groups, global-window
+// nets), so load and L stay on chip between the sweeps; otherwise (streamed
+// nets) they go to global memory and are read back.
+template <int K, bool KEEP, int W, bool COMP, bool GLOBAL>
 __device__ __forceinline__ void process_item(const RctParams& P, RctLds<W, COMP>& S,
                                              const Ring<GLOBAL, COMP>& R, const RctItem& it,
                                              const int lane) {
+  constexpr int WIN = kWave * K;
+  constexpr bool STASH = KEEP && !GLOBAL;
   const int pb = it.pos_begin, pe = it.pos_end, n = pe - pb;
-  const bool keep = it.kind != ITEM_STREAM;             // window holds the whole item
-  const bool stash = it.kind == ITEM_PACKED;            // pid/R kept in LDS
-  const bool multi = it.net_end - it.net_begin > 1;     // packed group of several nets
-  float4* gL = reinterpret_cast<float4*>(P.scratch + (it.kind == ITEM_STREAM ? it.scratch : 0));
-
-  // ---- set-up: clear the window, mark drivers -------------------------------
+  const bool multi = it.net_end - it.net_begin > 1;
+  const float* gl = P.scratch + (KEEP ? 0 : it.scratch);
+  ItemCtx<W, COMP> X{S, pb, pe, multi, lane,
+                     ItemBufs{make_rsrc(P.pid, pb, 4, n), make_rsrc(P.pres, pb, 4, n),
+                              make_rsrc(P.cap, pb, 16, n), make_rsrc(P.load, pb, 16, n),
+                              make_rsrc(KEEP ? nullptr : gl, 0, 16, n),
+                              make_rsrc(P.delay, pb, 16, n), make_rsrc(P.ldelay, pb, 16, n),
+                              make_rsrc(P.beta, pb, 16, n), make_rsrc(P.impulse, pb, 16, n),
+                              make_rsrc(P.slew, pb, 16, n)}};
+  const Rsrc rsi = make_rsrc(P.slew_in, 0, 16, P.num_nets_total);
+  const int nwin = (n + WIN - 1) / WIN;
+  // first window's inputs go out before the set-up work
+  BuIn<K> cur, nxt;
+  load_bu<K>(X, pb + (nwin - 1) * WIN, cur);
+  // ---- set-up: clear the window, mark drivers ------------------------------
   {
-    const int nz = keep ? n : W;
+    const int nz = KEEP ? n : W;
     for (int k = lane; k < nz; k += kWave) R.zero(k);
     if (multi) {
       for (int k = lane; k < W / 32; k += kWave) S.rootbits[k] = 0u;
       wave_sync();
       for (int j = it.net_begin + lane; j < it.net_end; j += kWave) {
-        const int r = P.nodes_start[j] - pb;
+        const int r = gld(P.nodes_start + j) - pb;
         atomicOr(&S.rootbits[r >> 5], 1u << (r & 31));
       }
     }
     R.fence();
     wave_sync();
   }
-  auto is_root = [&](int i) -> bool {
-    if (multi) return (S.rootbits[(i - pb) >> 5] >> ((i - pb) & 31)) & 1u;
-    return i == pb;
-  };
-
-  const int nwin = (n + kWave - 1) / kWave;
-
-  // ---- bottom-up sweep --------------------------------------------------------
-  auto load_bu = [&](int w, WinIn& in) {
-    const int i = pb + w * kWave + lane;
-    if (i < pe) {
-      in.pid = P.pid[i];
-      in.r = P.pres[i];
-      in.c = P.cap[i];
-    } else {
-      in.pid = 0;
-      in.r = 0.f;
-      in.c = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-  };
-  WinIn cur, nxt;
-  load_bu(nwin - 1, cur);
+  // ---- bottom-up sweep ----------------------------------------------------
   for (int w = nwin - 1; w >= 0; --w) {
-    if (w > 0) load_bu(w - 1, nxt);
-    const int wk = pb + w * kWave;
-    const int i = wk + lane;
-    const bool valid = i < pe;
-    const bool root = valid && is_root(i);
-    const int p = (valid && !root) ? i - cur.pid : -1;
-    const bool pin = p >= wk;
-    if (stash && valid) {
-      S.spid[i - pb] = cur.pid;
-      S.sres[i - pb] = cur.r;
-    }
-    S.cnt[lane] = 0;
-    wave_sync();
-    if (pin) atomicAdd(&S.cnt[p - wk], 1);
-    wave_sync();
-    bool done = !valid;
-    float cap4[4];
-    unpack(cur.c, cap4);
-    for (int round = 0; round <= kWave; ++round) {
-      const bool ready = !done && S.cnt[lane] == 0;
-      if (ready) {
-        const int s = R.slot(i);
-        float xa[4], xb[4], xal[4], xbl[4];
-        R.get(s, xa, xb, xal, xbl);
-        float ld[4], ldl[4], L[4], Ll[4], w[4], wl[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          if constexpr (COMP) {
-            const ff t = ff_add(two_sum(cap4[c], xa[c]), ff{xal[c], 0.f});
-            ld[c] = t.hi;
-            ldl[c] = t.lo;
-            L[c] = xb[c];
-            Ll[c] = xbl[c];
-            const ff q = ff_add(ff{L[c], Ll[c]}, ff_mul_f(ff_mul(t, t), cur.r));
-            w[c] = q.hi;
-            wl[c] = q.lo;
-          } else {
-            ld[c] = cap4[c] + xa[c];
-            ldl[c] = 0.f;
-            L[c] = xb[c];
-            Ll[c] = 0.f;
-            w[c] = L[c] + cur.r * ld[c] * ld[c];
-            wl[c] = 0.f;
-          }
-        }
-        if (keep) {
-          R.put(s, ld, L, ldl, Ll);
-        } else {
-          float o1[4], o2[4];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            o1[c] = COMP ? ld[c] + ldl[c] : ld[c];
-            o2[c] = COMP ? L[c] + Ll[c] : L[c];
-          }
-          P.load[i] = pack(o1);
-          gL[i - pb] = pack(o2);
-        }
-        if (p >= 0) {
-          R.add(R.slot(p), ld, w, ldl, wl);
-          if (pin) atomicSub(&S.cnt[p - wk], 1);
-        }
-        done = true;
-      }
-      R.fence();
-      wave_sync();
-      if (__all(done)) break;
-    }
-    if (!keep && valid) R.zero(R.slot(i));
-    wave_sync();
+    load_bu<K>(X, pb + (w > 0 ? w - 1 : 0) * WIN, nxt);
+    bu_window<K, KEEP, W, COMP, GLOBAL>(X, R, pb + w * WIN, cur);
     cur = nxt;
   }
   R.fence();
   wave_sync();
-
-  // ---- top-down sweep ---------------------------------------------------------
-  struct TdIn {
-    int32_t pid;
-    float r;
-    float4 ld, L;
+  // ---- top-down sweep -----------------------------------------------------
+  // driver slew of each position's net: counted from the driver bitmap in
+  // window order for a packed group; the single net otherwise
+  int net_carry = 0;
+  const float4 si0 = bload4(rsi, it.net_begin * 16);
+  auto load_td = [&](int w, TdIn<K, KEEP>& in) {
+    if constexpr (!KEEP) in.si[0] = si0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = pb + w * WIN + k * kWave + lane;
+      const int q = i - pb;
+      if constexpr (KEEP) {
+        const bool root = i < pe && X.is_root(i);
+        const uint64_t rm = __ballot(root);
+        const int net = multi ? it.net_begin + net_carry + __popcll(rm & ((2ull << lane) - 1ull)) - 1
+                              : it.net_begin;
+        net_carry += __popcll(rm);
+        in.si[k] = bload4(rsi, (i < pe ? net : P.num_nets_total) * 16);
+      }
+      if constexpr (STASH) {
+        in.pid[k] = i < pe ? S.spid[q] : 0;
+        in.r[k] = i < pe ? S.sres[q] : 0.f;
+      } else {
+        in.pid[k] = bload_i(X.B.pid, q * 4);
+        in.r[k] = bload_f(X.B.res, q * 4);
+      }
+      if constexpr (!KEEP) {
+        in.ld[KEEP ? 0 : k] = bload4(X.B.load, q * 16);
+        in.L[KEEP ? 0 : k] = bload4(X.B.gl, q * 16);
+      }
+    }
   };
-  auto load_td = [&](int w, TdIn& in) {
-    const int i = pb + w * kWave + lane;
-    in.ld = make_float4(0.f, 0.f, 0.f, 0.f);
-    in.L = in.ld;
-    if (i >= pe) {
-      in.pid = 0;
-      in.r = 0.f;
-      return;
-    }
-    if (stash) {
-      in.pid = S.spid[i - pb];
-      in.r = S.sres[i - pb];
-    } else {
-      in.pid = P.pid[i];
-      in.r = P.pres[i];
-    }
-    if (!keep) {
-      in.ld = P.load[i];
-      in.L = gL[i - pb];
-    }
-  };
-  TdIn tc, tn;
+  TdIn<K, KEEP> tc, tn;
   load_td(0, tc);
-  int carry = 0;
   for (int w = 0; w < nwin; ++w) {
-    if (w + 1 < nwin) load_td(w + 1, tn);
-    const int wk = pb + w * kWave;
-    const int i = wk + lane;
-    const bool valid = i < pe;
-    const bool root = valid && is_root(i);
-    const int p = (valid && !root) ? i - tc.pid : -1;
-    const bool pin = p >= wk;
-    int net = it.net_begin;
-    if (multi) {
-      const uint64_t rm = __ballot(root);
-      net += carry + __popcll(rm & ((2ull << lane) - 1ull)) - 1;
-      carry += __popcll(rm);
-    }
-    // own load and L
-    float ld[4], ldl[4], L[4], Ll[4];
-    if (keep) {
-      if (valid) {
-        R.get(R.slot(i), ld, L, ldl, Ll);
-      } else {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) ld[c] = ldl[c] = L[c] = Ll[c] = 0.f;
-      }
-    } else {
-      unpack(tc.ld, ld);
-      unpack(tc.L, L);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) ldl[c] = Ll[c] = 0.f;
-    }
-    const float r = root ? 0.f : tc.r;
-    // per-node path terms u = R load, z = u^2 + 2 R L
-    float u[4], ul[4], z[4], zl[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if constexpr (COMP) {
-        const ff uu = ff_mul_f(ff{ld[c], ldl[c]}, r);
-        const ff zz = ff_add(ff_mul(uu, uu), ff_mul_f(ff{L[c], Ll[c]}, 2.0f * r));
-        u[c] = uu.hi;
-        ul[c] = uu.lo;
-        z[c] = zz.hi;
-        zl[c] = zz.lo;
-      } else {
-        u[c] = r * ld[c];
-        z[c] = u[c] * u[c] + 2.0f * r * L[c];
-        ul[c] = zl[c] = 0.f;
-      }
-    }
-    S.cnt[lane] = 0;
-    wave_sync();
-    bool done = !valid;
-    float d[4], dl[4], m[4], ml[4];
-    for (int round = 0; round <= kWave; ++round) {
-      const bool ready = !done && (!pin || S.cnt[p - wk] != 0);
-      if (ready) {
-        float pd[4] = {0.f, 0.f, 0.f, 0.f}, pm[4] = {0.f, 0.f, 0.f, 0.f};
-        float pdl[4] = {0.f, 0.f, 0.f, 0.f}, pml[4] = {0.f, 0.f, 0.f, 0.f};
-        if (p >= 0) R.get(R.slot(p), pd, pm, pdl, pml);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          if constexpr (COMP) {
-            const ff dd = ff_add(ff{pd[c], pdl[c]}, ff{u[c], ul[c]});
-            const ff mm = ff_add(ff{pm[c], pml[c]}, ff{z[c], zl[c]});
-            d[c] = dd.hi;
-            dl[c] = dd.lo;
-            m[c] = mm.hi;
-            ml[c] = mm.lo;
-          } else {
-            d[c] = pd[c] + u[c];
-            m[c] = pm[c] + z[c];
-            dl[c] = ml[c] = 0.f;
-          }
-        }
-        R.put(R.slot(i), d, m, dl, ml);
-        S.cnt[lane] = 1;
-        done = true;
-      }
-      R.fence();
-      wave_sync();
-      if (__all(done)) break;
-    }
-    if (valid) {
-      float4 si4 = P.slew_in ? P.slew_in[net] : make_float4(0.f, 0.f, 0.f, 0.f);
-      float si[4];
-      unpack(si4, si);
-      float o_ld[4], o_d[4], o_ldel[4], o_b[4], o_m[4], o_s[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if constexpr (COMP) {
-          // final per-node combination in FP64
-          const double dd = (double)d[c] + (double)dl[c];
-          const double md = (double)m[c] + (double)ml[c];
-          const double ldd = (double)ld[c] + (double)ldl[c];
-          const double Ld = (double)L[c] + (double)Ll[c];
-          o_ld[c] = (float)ldd;
-          o_d[c] = (float)dd;
-          o_m[c] = (float)md;
-          o_ldel[c] = (float)(dd * ldd + Ld);
-          o_b[c] = (float)(0.5 * (dd * dd + md));
-          const double sv = sqrt((double)si[c] * (double)si[c] + md);
-          o_s[c] = (float)(si[c] < 0.f ? -sv : sv);
-        } else {
-          o_ld[c] = ld[c];
-          o_d[c] = d[c];
-          o_m[c] = m[c];
-          o_ldel[c] = d[c] * ld[c] + L[c];
-          o_b[c] = 0.5f * (d[c] * d[c] + m[c]);
-          const float sv = sqrtf(si[c] * si[c] + m[c]);
-          o_s[c] = si[c] < 0.f ? -sv : sv;
-        }
-      }
-      if (keep && P.load) P.load[i] = pack(o_ld);
-      if (P.delay) P.delay[i] = pack(o_d);
-      if (P.ldelay) P.ldelay[i] = pack(o_ldel);
-      if (P.beta) P.beta[i] = pack(o_b);
-      if (P.impulse) P.impulse[i] = pack(o_m);
-      if (P.slew) P.slew[i] = pack(o_s);
-    }
+    load_td(w + 1 < nwin ? w + 1 : w, tn);
+    td_window<K, KEEP, W, COMP, GLOBAL>(X, R, pb + w * WIN, tc);
     tc = tn;
   }
   R.fence();
@@ -548,6 +718,7 @@ __device__ __forceinline__ void process_item(const RctParams& P, RctLds<W, COMP>
 template <int W, bool COMP>
 __global__ __launch_bounds__(64) void rct_batched_kernel(RctParams P) {
   __shared__ RctLds<W, COMP> S;
+  constexpr int KP = RctLds<W, COMP>::KP;
   const int lane = threadIdx.x;
   const int total = P.subset ? P.subset_count : P.num_items;
   for (;;) {
@@ -557,17 +728,18 @@ __global__ __launch_bounds__(64) void rct_batched_kernel(RctParams P) {
     if (idx >= total) break;
     RctItem it;
     if (P.subset) {
-      const int net = P.subset[idx];
-      it.pos_begin = P.nodes_start[net];
-      it.pos_end = P.nodes_start[net + 1];
+      const int net = gld(P.subset + idx);
+      it.pos_begin = gld(P.nodes_start + net);
+      it.pos_end = gld(P.nodes_start + net + 1);
       it.net_begin = net;
       it.net_end = net + 1;
       const int nn = it.pos_end - it.pos_begin;
-      it.kind = nn <= W ? ITEM_PACKED : (P.net_maxpid[net] <= W - kWave ? ITEM_STREAM : ITEM_GLOBAL);
-      it.pad = 0;
-      it.scratch = P.net_scratch[net];
+      const int mp = gld(P.net_maxpid + net);
+      it.kind = nn <= W ? ITEM_PACKED : (mp <= W - kWave ? ITEM_STREAM : ITEM_GLOBAL);
+      it.k = it.kind != ITEM_STREAM ? 1 : (mp <= W - 4 * kWave ? 4 : (mp <= W - 2 * kWave ? 2 : 1));
+      it.scratch = gld(P.net_scratch + net);
     } else {
-      it = P.items[idx];
+      it = gld(P.items + idx);
     }
     if (it.pos_end <= it.pos_begin) continue;
     if (it.kind == ITEM_GLOBAL) {
@@ -575,10 +747,13 @@ __global__ __launch_bounds__(64) void rct_batched_kernel(RctParams P) {
       float4* base = reinterpret_cast<float4*>(P.scratch + it.scratch);
       Ring<true, COMP> R{base, base + nn, COMP ? base + 2 * nn : base, COMP ? base + 3 * nn : base,
                          it.pos_begin, -1};
-      process_item<W, COMP, true>(P, S, R, it, lane);
+      process_item<1, true, W, COMP, true>(P, S, R, it, lane);
     } else {
       Ring<false, COMP> R{S.ra, S.rb, S.ral, S.rbl, it.pos_begin, W - 1};
-      process_item<W, COMP, false>(P, S, R, it, lane);
+      if (it.kind == ITEM_PACKED) process_item<KP, true, W, COMP, false>(P, S, R, it, lane);
+      else if (it.k >= 4) process_item<4, false, W, COMP, false>(P, S, R, it, lane);
+      else if (it.k == 2) process_item<2, false, W, COMP, false>(P, S, R, it, lane);
+      else process_item<1, false, W, COMP, false>(P, S, R, it, lane);
     }
   }
   // The last workgroup out resets the queue for the next launch.
@@ -640,9 +815,11 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   if (!r.ring_slots) r.ring_slots = d.ring_slots ? d.ring_slots : 512;
   if (r.ring_slots != 256 && r.ring_slots != 512 && r.ring_slots != 1024) r.ring_slots = 512;
   if (!r.small_target) r.small_target = d.small_target ? d.small_target : r.ring_slots;
-  r.small_target = std::max(64, std::min(r.small_target, r.ring_slots));
+  // a packed group is one window of at most 512 positions (8 per lane)
+  r.small_target = std::max(64, std::min(r.small_target, std::min(r.ring_slots, 512)));
   if (!r.grid_per_cu) r.grid_per_cu = d.grid_per_cu;
-  r.waves_per_wg = 1;
+  if (!r.window_k) r.window_k = d.window_k ? d.window_k : 4;
+  r.window_k = r.window_k >= 4 ? 4 : (r.window_k >= 2 ? 2 : 1);
   (void)m;
   return r;
 }
@@ -694,6 +871,7 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   P.slew = reinterpret_cast<float4*>(out->slew);
   P.items = plan->d_items;
   P.num_items = plan->num_items;
+  P.num_nets_total = tree->num_nets;
   P.subset = subset;
   P.subset_count = count;
   P.net_maxpid = plan->d_net_maxpid;
@@ -795,15 +973,18 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
         cur.net_end = k + 1;
       } else {
         flush();
-        cur = RctItem{hstart[k], hstart[k + 1], k, k + 1, ITEM_PACKED, 0, 0};
+        cur = RctItem{hstart[k], hstart[k + 1], k, k + 1, ITEM_PACKED, 1, 0};
         open = true;
       }
       continue;
     }
     flush();
-    RctItem it{hstart[k], hstart[k + 1], k, k + 1, ITEM_STREAM, 0, 0};
+    RctItem it{hstart[k], hstart[k + 1], k, k + 1, ITEM_STREAM, 1, 0};
     if (hmax[k] <= W - kWave) {
       it.kind = ITEM_STREAM;
+      const int kmax = p->tuning.window_k;
+      it.k = (kmax >= 4 && hmax[k] <= W - 4 * kWave) ? 4
+             : ((kmax >= 2 && hmax[k] <= W - 2 * kWave) ? 2 : 1);
       it.scratch = scratch;
       scratch += 4LL * n;
       ++n_stream;
