@@ -135,6 +135,18 @@ def main():
     if args.only == "levelize":
         print(json.dumps(run_levelize(ctx, stream, sh)))
         return
+    if args.only == "copy":
+        # a plain device copy of known size: the yardstick that converts the
+        # FETCH_SIZE / WRITE_SIZE counters of the RC kernel into HBM bytes
+        nbytes = 1 << 30
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"only": "copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "launches": args.steps}))
+        return
 
     cfg = W.config1(num_nets=args.nets)
     t0 = time.perf_counter()

@@ -818,7 +818,7 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   // a packed group is one window of at most 512 positions (8 per lane)
   r.small_target = std::max(64, std::min(r.small_target, std::min(r.ring_slots, 512)));
   if (!r.grid_per_cu) r.grid_per_cu = d.grid_per_cu;
-  if (!r.window_k) r.window_k = d.window_k ? d.window_k : 4;
+  if (!r.window_k) r.window_k = d.window_k ? d.window_k : 1;
   r.window_k = r.window_k >= 4 ? 4 : (r.window_k >= 2 ? 2 : 1);
   (void)m;
   return r;
