@@ -359,9 +359,9 @@ static bool write_tuning(const char* path, const otg_rct_tuning& t, otg_rct_meth
   FILE* f = fopen(path, "w");
   if (!f) return false;
   fprintf(f,
-          "{\"ring_slots\": %d, \"window_k\": %d, \"small_target\": %d, \"grid_per_cu\": %d, "
+          "{\"ring_slots\": %d, \"small_target\": %d, \"grid_per_cu\": %d, "
           "\"method\": %d, \"ms\": %.6f}\n",
-          t.ring_slots, t.window_k, t.small_target, t.grid_per_cu, (int)m, ms);
+          t.ring_slots, t.small_target, t.grid_per_cu, (int)m, ms);
   fclose(f);
   return true;
 }
@@ -389,7 +389,6 @@ extern "C" otg_status otg_rct_tuning_load(otg_context* ctx, const char* path, ot
   fclose(f);
   otg_rct_tuning r{};
   r.ring_slots = json_int(s, "ring_slots", 0);
-  r.window_k = json_int(s, "window_k", 0);
   r.small_target = json_int(s, "small_target", 0);
   r.grid_per_cu = json_int(s, "grid_per_cu", 0);
   ctx->tuning = r;
@@ -408,14 +407,16 @@ extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tre
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   std::vector<otg_rct_tuning> cands;
-  for (int W : {512, 1024, 256})
-    for (int target : {256, 512})
-      for (int kk : {1, 2, 4}) {
+  // ring size (LDS per wavefront, hence occupancy, against the nets that
+  // fall back to the global-window path), packed-group size, and the number
+  // of persistent workgroups per CU (0: as many as fit)
+  for (int W : {512, 256, 1024})
+    for (int target : {256, 192, 128})
+      for (int g : {0, 4, 6}) {
         otg_rct_tuning t{};
         t.ring_slots = W;
-        t.window_k = kk;
-        t.small_target = std::min(target, W);
-        t.grid_per_cu = 0;
+        t.small_target = target;
+        t.grid_per_cu = g;
         cands.push_back(t);
       }
   hipEvent_t e0, e1;
@@ -426,8 +427,6 @@ extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tre
   for (const auto& t : cands) {
     otg_rct_plan* plan = nullptr;
     if (otg_rct_plan_create(ctx, tree, &t, method, &plan) != OTG_OK) continue;
-    // skip candidates that push nets to the global-memory window when a
-    // larger ring keeps them on chip
     st = otg_rct_launch_batched(ctx, plan, tree, scratch_out, nullptr, 0, s);  // warm-up
     if (st != OTG_OK) {
       otg_rct_plan_destroy(plan);

@@ -21,8 +21,8 @@ struct otg_context {
 // One unit of work of the batched P1 launch: a contiguous range of whole
 // nets [net_begin, net_end) occupying positions [pos_begin, pos_end).
 enum RctItemKind : int32_t {
-  ITEM_PACKED = 0,  // group of nets that fits the on-chip window entirely
-  ITEM_STREAM = 1,  // one net larger than the window, parent offsets < window-64
+  ITEM_PACKED = 0,  // group of whole nets of at most 256 positions, held on chip
+  ITEM_STREAM = 1,  // one larger net whose parent offsets fit the ring (< W-64)
   ITEM_GLOBAL = 2   // one net that needs a per-node window in global memory
 };
 
@@ -30,7 +30,7 @@ struct RctItem {
   int32_t pos_begin, pos_end;
   int32_t net_begin, net_end;
   int32_t kind;
-  int32_t k;        // stream items: positions per lane of a window (1, 2 or 4)
+  int32_t k;        // positions per lane of a window (global-window path: 1)
   int64_t scratch;  // float offset into the plan scratch (stream/global items)
 };
 
@@ -41,8 +41,12 @@ struct otg_rct_plan {
   int32_t num_nets = 0;
   int32_t total_nodes = 0;
   int32_t num_items = 0;
-  RctItem* d_items = nullptr;
-  int32_t* d_queue = nullptr;      // [0] next item, [1] finished workgroups
+  RctItem* d_items = nullptr;      // packed groups and streamed nets
+  int32_t num_gitems = 0;
+  RctItem* d_gitems = nullptr;     // global-window nets
+  RctItem* d_sub_items = nullptr;  // [num_nets] items of a subset solve
+  int32_t* d_queue = nullptr;      // [0],[1] streaming kernel, [2],[3] global-window
+                                   // kernel: next item, finished workgroups
   float* d_scratch = nullptr;      // window storage of stream/global items
   int64_t scratch_floats = 0;
   int32_t* d_net_maxpid = nullptr;  // per net, for subset solves

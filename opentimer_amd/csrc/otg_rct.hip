@@ -186,7 +186,11 @@ struct RctParams {
   const int64_t* net_scratch;
   float* scratch;
   int32_t* queue;
+  int32_t long_maxpid;  // largest parent offset the streaming kernel's ring takes
 };
+
+// Nets of at most this many nodes are solved as packed groups held on chip.
+constexpr int kShortMax = 256;
 
 template <int W, bool COMP>
 struct RctLds {
@@ -715,10 +719,26 @@ __device__ __forceinline__ void process_item(const RctParams& P, RctLds<W, COMP>
   wave_sync();
 }
 
-template <int W, bool COMP>
-__global__ __launch_bounds__(64) void rct_batched_kernel(RctParams P) {
+// The last workgroup out resets its queue for the next launch.
+__device__ __forceinline__ void queue_release(int32_t* queue, int lane) {
+  __syncthreads();
+  if (lane == 0) {
+    __threadfence();
+    const int old = atomicAdd(&queue[1], 1);
+    if (old == (int)gridDim.x - 1) {
+      atomicExch(&queue[0], 0);
+      atomicExch(&queue[1], 0);
+    }
+  }
+}
+
+// Nets whose parent offsets exceed what the streaming kernel's ring can hold:
+// windows of 64 with the per-node window storage in global memory.  In subset
+// mode every listed net is classified and only those nets are taken.
+template <bool COMP>
+__global__ __launch_bounds__(64) void rct_global_kernel(RctParams P) {
+  constexpr int W = 256;  // LDS of this path: counters only, the ring is in global memory
   __shared__ RctLds<W, COMP> S;
-  constexpr int KP = RctLds<W, COMP>::KP;
   const int lane = threadIdx.x;
   const int total = P.subset ? P.subset_count : P.num_items;
   for (;;) {
@@ -735,37 +755,661 @@ __global__ __launch_bounds__(64) void rct_batched_kernel(RctParams P) {
       it.net_end = net + 1;
       const int nn = it.pos_end - it.pos_begin;
       const int mp = gld(P.net_maxpid + net);
-      it.kind = nn <= W ? ITEM_PACKED : (mp <= W - kWave ? ITEM_STREAM : ITEM_GLOBAL);
-      it.k = it.kind != ITEM_STREAM ? 1 : (mp <= W - 4 * kWave ? 4 : (mp <= W - 2 * kWave ? 2 : 1));
+      it.kind = (nn <= kShortMax || mp <= P.long_maxpid) ? ITEM_PACKED : ITEM_GLOBAL;
+      it.k = 1;
       it.scratch = gld(P.net_scratch + net);
     } else {
       it = gld(P.items + idx);
     }
-    if (it.pos_end <= it.pos_begin) continue;
-    if (it.kind == ITEM_GLOBAL) {
-      const int64_t nn = it.pos_end - it.pos_begin;
-      float4* base = reinterpret_cast<float4*>(P.scratch + it.scratch);
-      Ring<true, COMP> R{base, base + nn, COMP ? base + 2 * nn : base, COMP ? base + 3 * nn : base,
-                         it.pos_begin, -1};
-      process_item<1, true, W, COMP, true>(P, S, R, it, lane);
+    if (it.pos_end <= it.pos_begin || it.kind != ITEM_GLOBAL) continue;
+    const int64_t nn = it.pos_end - it.pos_begin;
+    float4* base = reinterpret_cast<float4*>(P.scratch + it.scratch);
+    Ring<true, COMP> R{base, base + nn, COMP ? base + 2 * nn : base, COMP ? base + 3 * nn : base,
+                       it.pos_begin, -1};
+    process_item<1, true, W, COMP, true>(P, S, R, it, lane);
+  }
+  queue_release(P.queue, lane);
+}
+
+// ===========================================================================
+// Streaming kernel: packed groups of small nets and streamed large nets.
+//
+// Persistent, one wavefront per workgroup, items from a largest-first queue.
+// Positions are visited in windows of 64, one lane per position.  Inside a
+// window the dependency order is resolved by in-window depth: a lane whose
+// parent lies before the window has depth 0, otherwise its parent's depth
+// plus one (computed with one ballot per level, no memory traffic).  The
+// bottom-up sweep visits depths from the deepest up, the top-down sweep from
+// 0 down; every step touches only the lanes of that depth.  Parents in
+// earlier windows are reached through a ring of W per-position slots in LDS
+// (slot = position mod W), which holds every parent as long as parent
+// offsets stay below W - 64.
+//
+// Global inputs arrive in chunks of four windows (256 positions) that are
+// requested one chunk ahead.  A wavefront that both loads and stores can
+// only wait for all of its memory operations at once, so the prefetch
+// distance is one chunk of work rather than one window.
+//
+//  * packed group (<= 256 positions, <= 128 whole nets): one chunk; load and
+//    L stay in the ring between the sweeps.
+//  * streamed net: bottom-up over chunks last to first, spilling the
+//    (rounded) load into the load output and L into scratch; then top-down
+//    over chunks first to last, reading them back.
+// ===========================================================================
+constexpr int kChunkWin = 4;
+constexpr int kChunkPos = kChunkWin * kWave;  // == kShortMax
+constexpr int kGroupNets = 2 * kWave;
+constexpr int kClaim = 256;  // claim table entries (parent slot mod 256)
+static_assert(kChunkPos == kShortMax, "a packed group is one chunk");
+
+template <int W, bool COMP>
+struct StreamLds {
+  float4 a[W];   // bottom-up: load accumulator (hi) | top-down: delay (hi)
+  float4 b[W];   // bottom-up: L accumulator (hi)    | top-down: impulse (hi)
+  float4 al[COMP ? W : 1];
+  float4 bl[COMP ? W : 1];
+  float4 si[kGroupNets];               // driver slew of a packed group's nets
+  uint32_t roots[kChunkPos / 32];      // driver positions of a packed group
+  int32_t claim[kClaim];               // one writer per parent slot per round
+};
+
+// Inputs of one chunk: lane l holds positions q0 + 64 s + l, s = 0..3.
+struct Chunk {
+  int32_t pid[kChunkWin];
+  float r[kChunkWin];
+  float4 x[kChunkWin];  // cap (bottom-up) | load read back (top-down, streamed)
+  float4 y[kChunkWin];  // L read back (top-down, streamed)
+  float4 si[2];         // packed: driver slew of nets l and 64+l | streamed: the net's
+  int32_t ns[2];        // packed: first position of nets l and 64+l
+};
+
+struct Item {
+  int pb, pe, nb, ne, kind;
+  int64_t scratch;
+};
+
+enum : int { PH_GROUP = 0, PH_BU = 1, PH_TD = 2 };
+
+__device__ __forceinline__ int num_chunks(const Item& it) {
+  return (it.pe - it.pb + kChunkPos - 1) / kChunkPos;
+}
+
+// Loads that must not hit a stale L1 line: the streamed top-down sweep reads
+// back what the same lanes stored during the bottom-up sweep (glc: read from
+// L2).
+__device__ __forceinline__ float4 bload4_l2(Rsrc r, int byte_off) {
+  const otg_u4v v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 1);
+  const otg_f4v f = __builtin_bit_cast(otg_f4v, v);
+  return make_float4(f.x, f.y, f.z, f.w);
+}
+
+__device__ __forceinline__ void load_chunk(const RctParams& P, const Item& it, int ph, int c,
+                                           Chunk& ch, int lane) {
+  const int n = it.pe - it.pb;
+  const int q0 = ph == PH_GROUP ? 0 : c * kChunkPos;
+  const Rsrc rp = make_rsrc(P.pid, it.pb, 4, n);
+  const Rsrc rr = make_rsrc(P.pres, it.pb, 4, n);
+  if (ph == PH_TD) {
+    const Rsrc rl = make_rsrc(P.load, it.pb, 16, n);
+    const Rsrc rg = make_rsrc(P.scratch + it.scratch, 0, 16, n);
+#pragma unroll
+    for (int s = 0; s < kChunkWin; ++s) {
+      const int q = q0 + s * kWave + lane;
+      ch.pid[s] = bload_i(rp, q * 4);
+      ch.r[s] = bload_f(rr, q * 4);
+      ch.x[s] = bload4_l2(rl, q * 16);
+      ch.y[s] = bload4_l2(rg, q * 16);
+    }
+    ch.si[0] = bload4(make_rsrc(P.slew_in, it.nb, 16, 1), 0);
+  } else {
+    const Rsrc rc = make_rsrc(P.cap, it.pb, 16, n);
+#pragma unroll
+    for (int s = 0; s < kChunkWin; ++s) {
+      const int q = q0 + s * kWave + lane;
+      ch.pid[s] = bload_i(rp, q * 4);
+      ch.r[s] = bload_f(rr, q * 4);
+      ch.x[s] = bload4(rc, q * 16);
+    }
+    if (ph == PH_GROUP) {
+      const int nn = it.ne - it.nb;
+      const Rsrc rs = make_rsrc(P.slew_in, it.nb, 16, nn);
+      const Rsrc rn = make_rsrc(P.nodes_start, it.nb, 4, nn);
+      ch.si[0] = bload4(rs, lane * 16);
+      ch.si[1] = bload4(rs, (lane + kWave) * 16);
+      ch.ns[0] = bload_i(rn, lane * 4);
+      ch.ns[1] = bload_i(rn, (lane + kWave) * 4);
+    }
+  }
+}
+
+// In-window depth of this lane's position (-1: no position).  Returns the
+// largest depth in the window.  One ballot per level; a malformed parent
+// (cycle) leaves its lanes at -1 instead of looping.
+__device__ __forceinline__ int window_depth(bool valid, int par, int ws, int& dw) {
+  const bool inw = valid && par >= ws;
+  const int pl = (par - ws) & (kWave - 1);
+  dw = (valid && !inw) ? 0 : -1;
+  uint64_t front = __ballot(valid && !inw);
+  int t = 0;
+  for (;;) {
+    const bool mine = inw && dw < 0 && ((front >> pl) & 1ull);
+    const uint64_t nf = __ballot(mine);
+    if (!nf) break;
+    ++t;
+    if (mine) dw = t;
+    front = nf;
+  }
+  return t;
+}
+
+template <int W, bool COMP>
+struct SRing {
+  StreamLds<W, COMP>& S;
+  int base;
+  __device__ __forceinline__ int slot(int i) const { return (i - base) & (W - 1); }
+  __device__ __forceinline__ void get(int k, float (&x)[4], float (&y)[4], float (&xl)[4],
+                                      float (&yl)[4]) const {
+    unpack(S.a[k], x);
+    unpack(S.b[k], y);
+    if constexpr (COMP) {
+      unpack(S.al[k], xl);
+      unpack(S.bl[k], yl);
     } else {
-      Ring<false, COMP> R{S.ra, S.rb, S.ral, S.rbl, it.pos_begin, W - 1};
-      if (it.kind == ITEM_PACKED) process_item<KP, true, W, COMP, false>(P, S, R, it, lane);
-      else if (it.k >= 4) process_item<4, false, W, COMP, false>(P, S, R, it, lane);
-      else if (it.k == 2) process_item<2, false, W, COMP, false>(P, S, R, it, lane);
-      else process_item<1, false, W, COMP, false>(P, S, R, it, lane);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) xl[c] = yl[c] = 0.f;
     }
   }
-  // The last workgroup out resets the queue for the next launch.
-  __syncthreads();
-  if (lane == 0) {
-    __threadfence();
-    const int old = atomicAdd(&P.queue[1], 1);
-    if (old == (int)gridDim.x - 1) {
-      atomicExch(&P.queue[0], 0);
-      atomicExch(&P.queue[1], 0);
+  __device__ __forceinline__ void put(int k, const float (&x)[4], const float (&y)[4],
+                                      const float (&xl)[4], const float (&yl)[4]) const {
+    S.a[k] = pack(x);
+    S.b[k] = pack(y);
+    if constexpr (COMP) {
+      S.al[k] = pack(xl);
+      S.bl[k] = pack(yl);
     }
   }
+  __device__ __forceinline__ void zero(int k) const {
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    S.a[k] = z;
+    S.b[k] = z;
+    if constexpr (COMP) {
+      S.al[k] = z;
+      S.bl[k] = z;
+    }
+  }
+  // a[k] += (x, xl), b[k] += (y, yl); compensated: the rounding error of each
+  // high-part add is recovered from the returned old value and added to the
+  // low part.
+  __device__ __forceinline__ void add(int k, const float (&x)[4], const float (&y)[4],
+                                      const float (&xl)[4], const float (&yl)[4]) const {
+    float* pa = reinterpret_cast<float*>(&S.a[k]);
+    float* pb = reinterpret_cast<float*>(&S.b[k]);
+    if constexpr (COMP) {
+      float* pal = reinterpret_cast<float*>(&S.al[k]);
+      float* pbl = reinterpret_cast<float*>(&S.bl[k]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float oa = atomicAdd(pa + c, x[c]);
+        const float ob = atomicAdd(pb + c, y[c]);
+        atomicAdd(pal + c, two_sum_err(oa, x[c], oa + x[c]) + xl[c]);
+        atomicAdd(pbl + c, two_sum_err(ob, y[c], ob + y[c]) + yl[c]);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        atomicAdd(pa + c, x[c]);
+        atomicAdd(pb + c, y[c]);
+      }
+    }
+  }
+};
+
+// Bottom-up window: load_i = C_i + sum load_c, L_i = sum (L_c + R_c load_c^2).
+// Returns the lane's final load and L rounded to FP32 (ldo, Lo).  KEEP
+// (packed group): the final values stay in the lane's slot for the top-down
+// sweep; otherwise the slot is cleared for the position W further back.
+// Both happen once for the whole window at its end: a slot is read only at
+// its own step, and the next user of a cleared slot (position i - W) gets
+// its first child only in a later window.
+//
+// A child adds into its parent's slot with a plain read-modify-write (LDS
+// float atomics cost several cycles per lane).  Siblings of one step would
+// race, so each step runs in rounds: every pending lane writes its lane id
+// to claim[parent slot mod 256], and the lanes that read their own id back
+// own their parent for the round.  Each round retires at least one lane, and
+// two winners never share a parent.
+template <int W, bool COMP, bool KEEP>
+__device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int pe, int lane,
+                                          int pidv, float r, float4 capv, bool root,
+                                          float (&ldo)[4], float (&Lo)[4]) {
+  const int i = ws + lane;
+  const bool valid = i < pe;
+  const int par = (valid && !root) ? i - pidv : -1;
+  int dw;
+  const int maxd = window_depth(valid, par, ws, dw);
+  const int sl = R.slot(i);
+  const int ps = R.slot(par);
+  const int cs = ps & (kClaim - 1);
+  float cap4[4];
+  unpack(capv, cap4);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) ldo[c] = Lo[c] = 0.f;
+  for (int t = maxd; t >= 0; --t) {
+    float ld[4], ldl[4], w[4], wl[4];
+    const bool act = dw == t;
+    if (act) {
+      float xa[4], xb[4], xal[4], xbl[4];
+      R.get(sl, xa, xb, xal, xbl);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if constexpr (COMP) {
+          const ff tt = ff_add(two_sum(cap4[c], xa[c]), ff{xal[c], 0.f});
+          ld[c] = tt.hi;
+          ldl[c] = tt.lo;
+          const ff q = ff_add(ff{xb[c], xbl[c]}, ff_mul_f(ff_mul(tt, tt), r));
+          w[c] = q.hi;
+          wl[c] = q.lo;
+          ldo[c] = ld[c] + ldl[c];
+          Lo[c] = xb[c] + xbl[c];
+        } else {
+          ld[c] = cap4[c] + xa[c];
+          w[c] = xb[c] + r * ld[c] * ld[c];
+          ldl[c] = wl[c] = 0.f;
+          ldo[c] = ld[c];
+          Lo[c] = xb[c];
+        }
+      }
+    }
+    uint64_t pend = __ballot(act && par >= 0);
+    while (pend) {
+      const bool mine = (pend >> lane) & 1ull;
+      if (mine) R.S.claim[cs] = lane;
+      wave_sync();
+      const bool win = mine && R.S.claim[cs] == lane;
+      if (win) {
+        float pa[4], pb[4], pal[4], pbl[4];
+        R.get(ps, pa, pb, pal, pbl);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if constexpr (COMP) {
+            const ff sa = ff_add(ff{pa[c], pal[c]}, ff{ld[c], ldl[c]});
+            const ff sb = ff_add(ff{pb[c], pbl[c]}, ff{w[c], wl[c]});
+            pa[c] = sa.hi;
+            pal[c] = sa.lo;
+            pb[c] = sb.hi;
+            pbl[c] = sb.lo;
+          } else {
+            pa[c] += ld[c];
+            pb[c] += w[c];
+          }
+        }
+        R.put(ps, pa, pb, pal, pbl);
+      }
+      pend &= ~__ballot(win);
+      wave_sync();
+    }
+  }
+  if constexpr (KEEP) {
+    float z[4] = {0.f, 0.f, 0.f, 0.f};
+    R.put(sl, ldo, Lo, z, z);
+  } else {
+    R.zero(sl);
+  }
+  wave_sync();
+}
+
+// Top-down window: delay_i = delay_p + R_i load_i,
+// impulse_i = impulse_p + (R_i load_i)^2 + 2 R_i L_i; then the lane's six
+// outputs.  ld/L are the lane's rounded bottom-up results.
+template <int W, bool COMP>
+__device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int pe, int lane,
+                                          int pidv, float r, bool root, const float (&ld)[4],
+                                          const float (&L)[4], float (&o_d)[4],
+                                          float (&o_ldel)[4], float (&o_b)[4], float (&o_m)[4],
+                                          float (&o_s)[4], const float4 siv) {
+  const int i = ws + lane;
+  const bool valid = i < pe;
+  const int par = (valid && !root) ? i - pidv : -1;
+  const float rr = root ? 0.f : r;
+  int dw;
+  const int maxd = window_depth(valid, par, ws, dw);
+  const int sl = R.slot(i);
+  const int ps = R.slot(par);
+  float d[4], dl[4], m[4], ml[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) d[c] = dl[c] = m[c] = ml[c] = 0.f;
+  for (int t = 0; t <= maxd; ++t) {
+    if (dw == t) {
+      float pd[4], pm[4], pdl[4], pml[4];
+      if (par >= 0) {
+        R.get(ps, pd, pm, pdl, pml);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pd[c] = pm[c] = pdl[c] = pml[c] = 0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if constexpr (COMP) {
+          const ff uu = ff_mul_f(ff{ld[c], 0.f}, rr);
+          const ff zz = ff_add(ff_mul(uu, uu), ff_mul_f(ff{L[c], 0.f}, 2.0f * rr));
+          const ff dd = ff_add(ff{pd[c], pdl[c]}, uu);
+          const ff mm = ff_add(ff{pm[c], pml[c]}, zz);
+          d[c] = dd.hi;
+          dl[c] = dd.lo;
+          m[c] = mm.hi;
+          ml[c] = mm.lo;
+        } else {
+          const float u = rr * ld[c];
+          d[c] = pd[c] + u;
+          m[c] = pm[c] + (u * u + 2.0f * rr * L[c]);
+        }
+      }
+      R.put(sl, d, m, dl, ml);
+    }
+    wave_sync();
+  }
+  float sv[4];
+  unpack(siv, sv);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if constexpr (COMP) {
+      // final per-node combination in FP64
+      const double dd = (double)d[c] + (double)dl[c];
+      const double md = (double)m[c] + (double)ml[c];
+      const double ldd = (double)ld[c];
+      o_d[c] = (float)dd;
+      o_m[c] = (float)md;
+      o_ldel[c] = (float)(dd * ldd + (double)L[c]);
+      o_b[c] = (float)(0.5 * (dd * dd + md));
+      const double q = sqrt((double)sv[c] * (double)sv[c] + md);
+      o_s[c] = (float)(sv[c] < 0.f ? -q : q);
+    } else {
+      o_d[c] = d[c];
+      o_m[c] = m[c];
+      o_ldel[c] = d[c] * ld[c] + L[c];
+      o_b[c] = 0.5f * (d[c] * d[c] + m[c]);
+      const float q = sqrtf(sv[c] * sv[c] + m[c]);
+      o_s[c] = sv[c] < 0.f ? -q : q;
+    }
+  }
+}
+
+struct OutBufs {
+  Rsrc load, delay, ldelay, beta, imp, slew, gl;
+};
+
+__device__ __forceinline__ OutBufs out_bufs(const RctParams& P, const Item& it) {
+  const int n = it.pe - it.pb;
+  return OutBufs{make_rsrc(P.load, it.pb, 16, n),    make_rsrc(P.delay, it.pb, 16, n),
+                 make_rsrc(P.ldelay, it.pb, 16, n),  make_rsrc(P.beta, it.pb, 16, n),
+                 make_rsrc(P.impulse, it.pb, 16, n), make_rsrc(P.slew, it.pb, 16, n),
+                 it.kind == ITEM_STREAM ? make_rsrc(P.scratch + it.scratch, 0, 16, n)
+                                        : make_rsrc(nullptr, 0, 16, 0)};
+}
+
+// A packed group: the whole item is one chunk held on chip.
+template <int W, bool COMP>
+__device__ __forceinline__ void run_group(StreamLds<W, COMP>& S, const RctParams& P,
+                                          const Item& it, const Chunk& ch, int lane) {
+  const SRing<W, COMP> R{S, it.pb};
+  const int n = it.pe - it.pb;
+  const int nn = it.ne - it.nb;
+  const int nw = (n + kWave - 1) / kWave;
+  for (int k = lane; k < nw * kWave; k += kWave) R.zero(k);
+  if (lane < kChunkPos / 32) S.roots[lane] = 0u;
+  wave_sync();
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int j = h * kWave + lane;
+    if (j < nn) {
+      const int q = ch.ns[h] - it.pb;
+      atomicOr(&S.roots[q >> 5], 1u << (q & 31));
+      S.si[j] = ch.si[h];
+    }
+  }
+  wave_sync();
+  uint64_t rmask[kChunkWin];
+#pragma unroll
+  for (int s = 0; s < kChunkWin; ++s)
+    rmask[s] = (uint64_t)S.roots[2 * s] | ((uint64_t)S.roots[2 * s + 1] << 32);
+  const OutBufs B = out_bufs(P, it);
+  // bottom-up, windows last to first
+#pragma unroll
+  for (int s = kChunkWin - 1; s >= 0; --s) {
+    if (s < nw) {
+      float ldo[4], Lo[4];
+      bu_window<W, COMP, true>(R, it.pb + s * kWave, it.pe, lane, ch.pid[s], ch.r[s], ch.x[s],
+                               (rmask[s] >> lane) & 1ull, ldo, Lo);
+    }
+  }
+  // top-down, windows first to last
+  int carry = 0;
+#pragma unroll
+  for (int s = 0; s < kChunkWin; ++s) {
+    if (s < nw) {
+      const int q = s * kWave + lane;
+      const int net = carry + __popcll(rmask[s] & ((2ull << lane) - 1ull)) - 1;
+      carry += __popcll(rmask[s]);
+      const float4 siv = S.si[net < 0 ? 0 : (net < kGroupNets ? net : kGroupNets - 1)];
+      float ld[4], L[4], xl[4], yl[4];
+      R.get(R.slot(it.pb + q), ld, L, xl, yl);
+      float o_d[4], o_ldel[4], o_b[4], o_m[4], o_s[4];
+      td_window<W, COMP>(R, it.pb + s * kWave, it.pe, lane, ch.pid[s], ch.r[s],
+                         (rmask[s] >> lane) & 1ull, ld, L, o_d, o_ldel, o_b, o_m, o_s, siv);
+      const int off = q * 16;
+      bstore4(B.load, off, ld);
+      bstore4(B.delay, off, o_d);
+      bstore4(B.ldelay, off, o_ldel);
+      bstore4(B.beta, off, o_b);
+      bstore4(B.imp, off, o_m);
+      bstore4(B.slew, off, o_s);
+    }
+  }
+}
+
+// One chunk of a streamed net's bottom-up sweep (windows last to first).
+template <int W, bool COMP>
+__device__ __forceinline__ void run_bu_chunk(StreamLds<W, COMP>& S, const RctParams& P,
+                                             const Item& it, int c, const Chunk& ch, int lane) {
+  const SRing<W, COMP> R{S, it.pb};
+  const OutBufs B = out_bufs(P, it);
+  const int n = it.pe - it.pb;
+#pragma unroll
+  for (int s = kChunkWin - 1; s >= 0; --s) {
+    const int q0 = c * kChunkPos + s * kWave;
+    if (q0 < n) {
+      float ldo[4], Lo[4];
+      bu_window<W, COMP, false>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], ch.x[s],
+                                q0 + lane == 0, ldo, Lo);
+      bstore4(B.load, (q0 + lane) * 16, ldo);
+      bstore4(B.gl, (q0 + lane) * 16, Lo);
+    }
+  }
+}
+
+// One chunk of a streamed net's top-down sweep (windows first to last).
+template <int W, bool COMP>
+__device__ __forceinline__ void run_td_chunk(StreamLds<W, COMP>& S, const RctParams& P,
+                                             const Item& it, int c, const Chunk& ch, int lane) {
+  const SRing<W, COMP> R{S, it.pb};
+  const OutBufs B = out_bufs(P, it);
+  const int n = it.pe - it.pb;
+#pragma unroll
+  for (int s = 0; s < kChunkWin; ++s) {
+    const int q0 = c * kChunkPos + s * kWave;
+    if (q0 < n) {
+      float ld[4], L[4];
+      unpack(ch.x[s], ld);
+      unpack(ch.y[s], L);
+      float o_d[4], o_ldel[4], o_b[4], o_m[4], o_s[4];
+      td_window<W, COMP>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], q0 + lane == 0, ld, L,
+                         o_d, o_ldel, o_b, o_m, o_s, ch.si[0]);
+      const int off = (q0 + lane) * 16;
+      bstore4(B.delay, off, o_d);
+      bstore4(B.ldelay, off, o_ldel);
+      bstore4(B.beta, off, o_b);
+      bstore4(B.imp, off, o_m);
+      bstore4(B.slew, off, o_s);
+    }
+  }
+}
+
+__device__ __forceinline__ Item item_from(int d) {
+  Item it;
+  it.pb = __builtin_amdgcn_readlane(d, 0);
+  it.pe = __builtin_amdgcn_readlane(d, 1);
+  it.nb = __builtin_amdgcn_readlane(d, 2);
+  it.ne = __builtin_amdgcn_readlane(d, 3);
+  it.kind = __builtin_amdgcn_readlane(d, 4);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(d, 6);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(d, 7);
+  it.scratch = (int64_t)(((uint64_t)hi << 32) | lo);
+  return it;
+}
+
+// The first chunk of an item: the group, or the last chunk of a streamed
+// net's bottom-up sweep.  Empty items and global-window nets have none.
+__device__ __forceinline__ bool first_chunk(const Item& it, int& ph, int& c) {
+  if (it.pe <= it.pb || (it.kind != ITEM_PACKED && it.kind != ITEM_STREAM)) return false;
+  ph = it.kind == ITEM_PACKED ? PH_GROUP : PH_BU;
+  c = ph == PH_GROUP ? 0 : num_chunks(it) - 1;
+  return true;
+}
+
+template <int W, bool COMP>
+__global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
+  __shared__ StreamLds<W, COMP> S;
+  const int lane = threadIdx.x;
+  const int total = P.num_items;
+  const Rsrc ritems = make_rsrc(P.items, 0, (int)sizeof(RctItem), total);
+  auto pop = [&]() {
+    int v = 0;
+    if (lane == 0) v = atomicAdd(&P.queue[0], 1);
+    return v;  // valid in lane 0
+  };
+  // lanes 0..7 hold the item's eight 32-bit fields; past the end: zeros
+  auto load_desc = [&](int idx) { return bload_i(ritems, idx * (int)sizeof(RctItem) + (lane & 7) * 4); };
+
+  // Item pipeline: the current item, the next two descriptors and the index
+  // after them are always known or in flight, so that the chunk after the
+  // current one can be requested whichever item it belongs to.
+  int q0 = __builtin_amdgcn_readfirstlane(pop());
+  int q1 = pop(), q2 = pop();
+  q1 = __builtin_amdgcn_readfirstlane(q1);
+  q2 = __builtin_amdgcn_readfirstlane(q2);
+  int d1 = load_desc(q1 < total ? q1 : total);
+  int d2 = load_desc(q2 < total ? q2 : total);
+  int q3 = pop();
+  Item it = item_from(load_desc(q0 < total ? q0 : total));
+  bool v1 = q1 < total, v2 = q2 < total;
+  bool have = q0 < total;
+  int ph = 0, c = 0;
+  // skip leading items without chunks
+  while (have && !first_chunk(it, ph, c)) {
+    if (!v1) {
+      have = false;
+      break;
+    }
+    it = item_from(d1);
+    d1 = d2;
+    v1 = v2;
+    const int qq = __builtin_amdgcn_readfirstlane(q3);
+    v2 = qq < total;
+    d2 = load_desc(v2 ? qq : total);
+    q3 = pop();
+  }
+  // Chunk data: `cur` is being processed, `nxt` is in flight.  A wavefront
+  // that has both loads and stores outstanding can only wait for all of
+  // them, so the wait for the next chunk happens once, at the top of the
+  // loop, before the following chunk is requested; the processing itself
+  // then reads only registers with no load in flight and never waits on
+  // global memory.
+  Chunk cur, nxt;
+  int nxt_pb = -1, nxt_ph = -1, nxt_c = -1;  // what nxt holds
+  while (have) {
+    // ---- the chunk to process: prefetched, or read now ----------------------
+    __builtin_amdgcn_s_waitcnt(0);
+    if (it.pb == nxt_pb && ph == nxt_ph && c == nxt_c) {
+      cur = nxt;
+      nxt_pb = -1;
+    } else {
+      // read now (first chunk, or a streamed net's top-down chunk 0, which
+      // reads back the bottom-up results stored just before: the wait above
+      // has completed those stores)
+      load_chunk(P, it, ph, c, cur, lane);
+      __builtin_amdgcn_s_waitcnt(0);
+    }
+    // ---- request the chunk after this one --------------------------------
+    {
+      int pph = -1, pc = -1;
+      Item pit = it;
+      bool pv = false;
+      if (ph == PH_BU && c > 0) {
+        pph = PH_BU;
+        pc = c - 1;
+        pv = true;
+      } else if (ph == PH_BU && c == 0 && num_chunks(it) > 1) {
+        // top-down chunk 0 is read back after this chunk's stores; chunk 1
+        // was stored by the previous chunk, complete since the wait above
+        pph = PH_TD;
+        pc = 1;
+        pv = true;
+      } else if (ph == PH_TD && c + 1 < num_chunks(it)) {
+        pph = PH_TD;
+        pc = c + 1;
+        pv = true;
+      } else if (v1) {
+        pit = item_from(d1);
+        pv = first_chunk(pit, pph, pc);
+      }
+      if (pv && !(pit.pb == nxt_pb && pph == nxt_ph && pc == nxt_c)) {
+        load_chunk(P, pit, pph, pc, nxt, lane);
+        nxt_pb = pit.pb;
+        nxt_ph = pph;
+        nxt_c = pc;
+      }
+    }
+    // ---- process ----------------------------------------------------------
+    if (ph == PH_GROUP) {
+      run_group<W, COMP>(S, P, it, cur, lane);
+    } else if (ph == PH_BU) {
+      if (c == num_chunks(it) - 1) {
+        const SRing<W, COMP> R{S, it.pb};
+        for (int k = lane; k < W; k += kWave) R.zero(k);
+        wave_sync();
+      }
+      run_bu_chunk<W, COMP>(S, P, it, c, cur, lane);
+    } else {
+      run_td_chunk<W, COMP>(S, P, it, c, cur, lane);
+    }
+    // ---- advance ------------------------------------------------------------
+    if (ph == PH_BU && c == 0) {
+      ph = PH_TD;
+      c = 0;
+    } else if (ph == PH_BU) {
+      --c;
+    } else if (ph == PH_TD && c + 1 < num_chunks(it)) {
+      ++c;
+    } else {
+      // next item
+      bool found = false;
+      while (v1) {
+        it = item_from(d1);
+        d1 = d2;
+        v1 = v2;
+        const int qq = __builtin_amdgcn_readfirstlane(q3);
+        v2 = qq < total;
+        d2 = load_desc(v2 ? qq : total);
+        q3 = pop();
+        if (first_chunk(it, ph, c)) {
+          found = true;
+          break;
+        }
+      }
+      if (!found) break;
+    }
+  }
+  queue_release(P.queue, lane);
 }
 
 // Per-net validation and largest parent offset (plan building).
@@ -786,20 +1430,56 @@ __global__ void rct_netinfo_kernel(int32_t num_nets, const int32_t* __restrict__
   if (!ok) atomicMin(bad, net);
 }
 
-template <int W, bool COMP>
-void* batched_kernel_ptr() {
-  return reinterpret_cast<void*>(&rct_batched_kernel<W, COMP>);
+// Subset solves: one item per listed net, classified as the plan would.
+// Nets outside [0, num_nets) and global-window nets become empty items
+// (the latter are taken by rct_global_kernel).
+__global__ void rct_subset_items_kernel(int32_t count, const int32_t* __restrict__ subset,
+                                        int32_t num_nets, const int32_t* __restrict__ nodes_start,
+                                        const int32_t* __restrict__ net_maxpid,
+                                        const int64_t* __restrict__ net_scratch,
+                                        int32_t long_maxpid, RctItem* __restrict__ items) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int net = subset[j];
+  RctItem it{0, 0, 0, 0, ITEM_PACKED, 1, 0};
+  if (net >= 0 && net < num_nets) {
+    it.pos_begin = nodes_start[net];
+    it.pos_end = nodes_start[net + 1];
+    it.net_begin = net;
+    it.net_end = net + 1;
+    const int n = it.pos_end - it.pos_begin;
+    if (n > kShortMax) {
+      if (net_maxpid[net] <= long_maxpid) {
+        it.kind = ITEM_STREAM;
+        it.scratch = net_scratch[net];
+      } else {
+        it.kind = ITEM_GLOBAL;
+        it.pos_end = it.pos_begin;
+      }
+    }
+  }
+  items[j] = it;
 }
 
-void* pick_batched(int W, bool comp) {
+template <int W, bool COMP>
+void* stream_kernel_ptr() {
+  return reinterpret_cast<void*>(&rct_stream_kernel<W, COMP>);
+}
+
+void* pick_stream(int W, bool comp) {
   if (comp) {
-    if (W == 256) return batched_kernel_ptr<256, true>();
-    if (W == 1024) return batched_kernel_ptr<1024, true>();
-    return batched_kernel_ptr<512, true>();
+    if (W == 256) return stream_kernel_ptr<256, true>();
+    if (W == 1024) return stream_kernel_ptr<1024, true>();
+    return stream_kernel_ptr<512, true>();
   }
-  if (W == 256) return batched_kernel_ptr<256, false>();
-  if (W == 1024) return batched_kernel_ptr<1024, false>();
-  return batched_kernel_ptr<512, false>();
+  if (W == 256) return stream_kernel_ptr<256, false>();
+  if (W == 1024) return stream_kernel_ptr<1024, false>();
+  return stream_kernel_ptr<512, false>();
+}
+
+void* pick_global(bool comp) {
+  return comp ? reinterpret_cast<void*>(&rct_global_kernel<true>)
+              : reinterpret_cast<void*>(&rct_global_kernel<false>);
 }
 
 }  // namespace
@@ -814,19 +1494,18 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   const otg_rct_tuning& d = ctx->tuning;
   if (!r.ring_slots) r.ring_slots = d.ring_slots ? d.ring_slots : 512;
   if (r.ring_slots != 256 && r.ring_slots != 512 && r.ring_slots != 1024) r.ring_slots = 512;
-  if (!r.small_target) r.small_target = d.small_target ? d.small_target : r.ring_slots;
-  // a packed group is one window of at most 512 positions (8 per lane)
-  r.small_target = std::max(64, std::min(r.small_target, std::min(r.ring_slots, 512)));
+  if (!r.small_target) r.small_target = d.small_target ? d.small_target : kShortMax;
+  // a packed group is one chunk: at most 256 positions
+  r.small_target = std::max(64, std::min(r.small_target, kShortMax));
   if (!r.grid_per_cu) r.grid_per_cu = d.grid_per_cu;
-  if (!r.window_k) r.window_k = d.window_k ? d.window_k : 1;
-  r.window_k = r.window_k >= 4 ? 4 : (r.window_k >= 2 ? 2 : 1);
+  r.reserved = 0;
   (void)m;
   return r;
 }
 
 int otg_rct_blocks_per_cu(otg_context* ctx, const otg_rct_tuning& t, bool comp) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_batched(t.ring_slots, comp), 64, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_stream(t.ring_slots, comp), 64, 0) !=
           hipSuccess ||
       nb <= 0)
     nb = 1;
@@ -857,6 +1536,10 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
     OTG_SET_ERR(ctx, "load output is required: the tree holds nets larger than the window");
     return OTG_ERR_INVALID_ARGUMENT;
   }
+  if (subset && count > plan->num_nets) {
+    OTG_SET_ERR(ctx, "subset of %d nets exceeds the plan's %d nets", count, plan->num_nets);
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
   RctParams P{};
   P.nodes_start = tree->nodes_start;
   P.pid = tree->pid;
@@ -869,22 +1552,49 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   P.beta = reinterpret_cast<float4*>(out->beta);
   P.impulse = reinterpret_cast<float4*>(out->impulse);
   P.slew = reinterpret_cast<float4*>(out->slew);
-  P.items = plan->d_items;
-  P.num_items = plan->num_items;
   P.num_nets_total = tree->num_nets;
-  P.subset = subset;
-  P.subset_count = count;
   P.net_maxpid = plan->d_net_maxpid;
   P.net_scratch = plan->d_net_scratch;
   P.scratch = plan->d_scratch;
+  P.long_maxpid = plan->tuning.ring_slots - kWave;
+  // ---- packed groups and streamed nets ------------------------------------
+  int total = plan->num_items;
+  P.items = plan->d_items;
+  if (subset) {
+    if (count <= 0) return OTG_OK;
+    hipLaunchKernelGGL(rct_subset_items_kernel, dim3((count + 255) / 256), dim3(256), 0, s, count,
+                       subset, plan->num_nets, tree->nodes_start, plan->d_net_maxpid,
+                       plan->d_net_scratch, P.long_maxpid, plan->d_sub_items);
+    OTG_HIP(ctx, hipGetLastError());
+    P.items = plan->d_sub_items;
+    total = count;
+  }
+  P.num_items = total;
   P.queue = plan->d_queue;
-  const int total = subset ? count : plan->num_items;
-  if (total <= 0) return OTG_OK;
-  int grid = plan->grid;
-  if (grid > total) grid = total;
-  void* fn = pick_batched(plan->tuning.ring_slots, comp);
-  void* args[] = {&P};
-  OTG_HIP(ctx, hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, s));
+  if (total > 0) {
+    const int grid = std::min(plan->grid, total);
+    void* args[] = {&P};
+    OTG_HIP(ctx, hipLaunchKernel(pick_stream(plan->tuning.ring_slots, comp), dim3(grid), dim3(64),
+                                 args, 0, s));
+  }
+  // ---- nets whose parent offsets exceed the ring ------------------------------
+  if (plan->stats[3] > 0) {
+    RctParams G = P;
+    G.queue = plan->d_queue + 2;
+    if (subset) {
+      G.subset = subset;
+      G.subset_count = count;
+      G.items = nullptr;
+      G.num_items = 0;
+    } else {
+      G.items = plan->d_gitems;
+      G.num_items = plan->num_gitems;
+    }
+    const int gt = subset ? count : plan->num_gitems;
+    const int grid = std::min(plan->grid, std::max(1, gt));
+    void* args[] = {&G};
+    OTG_HIP(ctx, hipLaunchKernel(pick_global(comp), dim3(grid), dim3(64), args, 0, s));
+  }
   return OTG_OK;
 }
 
@@ -948,7 +1658,13 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
     }
   }
   // ---- binning --------------------------------------------------------------
-  std::vector<RctItem> big_items, packed;
+  // Nets of up to 256 nodes go into packed groups (consecutive nets, at most
+  // small_target positions and 128 nets per group; a larger net alone).
+  // Larger nets are streamed when their parent offsets fit the ring
+  // (maxpid <= W - 64) and take the global-window path otherwise.  Streamed
+  // nets get 4 floats of scratch per node (their L), global-window nets 8 (16
+  // compensated).  Streamed nets are queued largest first, ahead of the groups.
+  std::vector<RctItem> big_items, packed, gitems;
   std::vector<int64_t> net_scratch(N, -1);
   int64_t scratch = 0;
   const int64_t slot_floats = comp ? 16 : 8;
@@ -966,9 +1682,10 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
       flush();
       continue;
     }
-    if (n <= W) {
+    if (n <= kShortMax) {
       ++n_packed_nets;
-      if (open && (cur.pos_end - cur.pos_begin) + n <= p->tuning.small_target) {
+      if (open && (cur.pos_end - cur.pos_begin) + n <= p->tuning.small_target &&
+          cur.net_end - cur.net_begin < 2 * kWave) {
         cur.pos_end += n;
         cur.net_end = k + 1;
       } else {
@@ -979,39 +1696,37 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
       continue;
     }
     flush();
-    RctItem it{hstart[k], hstart[k + 1], k, k + 1, ITEM_STREAM, 1, 0};
+    RctItem it{hstart[k], hstart[k + 1], k, k + 1, ITEM_STREAM, 1, scratch};
     if (hmax[k] <= W - kWave) {
-      it.kind = ITEM_STREAM;
-      const int kmax = p->tuning.window_k;
-      it.k = (kmax >= 4 && hmax[k] <= W - 4 * kWave) ? 4
-             : ((kmax >= 2 && hmax[k] <= W - 2 * kWave) ? 2 : 1);
-      it.scratch = scratch;
       scratch += 4LL * n;
       ++n_stream;
+      big_items.push_back(it);
     } else {
       it.kind = ITEM_GLOBAL;
-      it.scratch = scratch;
       scratch += slot_floats * n;
       ++n_global;
+      gitems.push_back(it);
     }
     net_scratch[k] = it.scratch;
-    big_items.push_back(it);
   }
   flush();
-  std::stable_sort(big_items.begin(), big_items.end(), [](const RctItem& a, const RctItem& b) {
+  auto larger = [](const RctItem& a, const RctItem& b) {
     return (a.pos_end - a.pos_begin) > (b.pos_end - b.pos_begin);
-  });
+  };
+  std::stable_sort(big_items.begin(), big_items.end(), larger);
+  std::stable_sort(gitems.begin(), gitems.end(), larger);
   std::vector<RctItem> items;
   items.reserve(big_items.size() + packed.size());
   items.insert(items.end(), big_items.begin(), big_items.end());
   items.insert(items.end(), packed.begin(), packed.end());
   p->num_items = (int32_t)items.size();
+  p->num_gitems = (int32_t)gitems.size();
   p->stats[0] = (int64_t)packed.size();
   p->stats[1] = n_packed_nets;
   p->stats[2] = n_stream;
   p->stats[3] = n_global;
   p->stats[4] = maxn;
-  p->stats[5] = p->num_items;
+  p->stats[5] = p->num_items + p->num_gitems;
   p->stats[6] = scratch;
   p->stats[7] = W;
   if (!items.empty()) {
@@ -1019,9 +1734,16 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
     PLAN_HIP(hipMemcpyAsync(p->d_items, items.data(), sizeof(RctItem) * items.size(),
                             hipMemcpyHostToDevice, ctx->stream));
   }
-  if (N > 0)
+  if (!gitems.empty()) {
+    PLAN_HIP(hipMalloc(&p->d_gitems, sizeof(RctItem) * gitems.size()));
+    PLAN_HIP(hipMemcpyAsync(p->d_gitems, gitems.data(), sizeof(RctItem) * gitems.size(),
+                            hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (N > 0) {
     PLAN_HIP(hipMemcpyAsync(p->d_net_scratch, net_scratch.data(), sizeof(int64_t) * N,
                             hipMemcpyHostToDevice, ctx->stream));
+    PLAN_HIP(hipMalloc(&p->d_sub_items, sizeof(RctItem) * N));
+  }
   p->scratch_floats = scratch;
   if (scratch > 0) PLAN_HIP(hipMalloc(&p->d_scratch, sizeof(float) * scratch));
   PLAN_HIP(hipMalloc(&p->d_queue, sizeof(int32_t) * 4));
@@ -1038,6 +1760,8 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
 extern "C" otg_status otg_rct_plan_destroy(otg_rct_plan* p) {
   if (!p) return OTG_OK;
   if (p->d_items) hipFree(p->d_items);
+  if (p->d_gitems) hipFree(p->d_gitems);
+  if (p->d_sub_items) hipFree(p->d_sub_items);
   if (p->d_queue) hipFree(p->d_queue);
   if (p->d_scratch) hipFree(p->d_scratch);
   if (p->d_net_maxpid) hipFree(p->d_net_maxpid);
