@@ -21,7 +21,7 @@ struct otg_context {
 // One unit of work of the batched P1 launch: a contiguous range of whole
 // nets [net_begin, net_end) occupying positions [pos_begin, pos_end).
 enum RctItemKind : int32_t {
-  ITEM_PACKED = 0,  // group of whole nets of at most 256 positions, held on chip
+  ITEM_PACKED = 0,  // group of whole small nets, solved on chip from a static schedule
   ITEM_STREAM = 1,  // one larger net whose parent offsets fit the ring (< W-64)
   ITEM_GLOBAL = 2   // one net that needs a per-node window in global memory
 };
@@ -41,17 +41,26 @@ struct otg_rct_plan {
   int32_t num_nets = 0;
   int32_t total_nodes = 0;
   int32_t num_items = 0;
-  RctItem* d_items = nullptr;      // packed groups and streamed nets
+  RctItem* d_items = nullptr;      // streamed nets, largest first
+  int32_t num_groups = 0;
+  int32_t group_cap = 256;         // largest net of a packed group
+  RctItem* d_groups = nullptr;     // packed groups (k: group index, scratch: schedule offset)
+  uint32_t* d_sched = nullptr;     // schedule words of the packed groups
+  int32_t* d_net_group = nullptr;  // [num_nets] group of a packed net, else -1
+  int32_t* d_group_flag = nullptr; // [num_groups] subset solves: group listed in this call
+  RctItem* d_sub_groups = nullptr; // [num_groups] groups of a subset solve
+  int32_t* d_sub_count = nullptr;  // number of them
   int32_t num_gitems = 0;
   RctItem* d_gitems = nullptr;     // global-window nets
   RctItem* d_sub_items = nullptr;  // [num_nets] items of a subset solve
-  int32_t* d_queue = nullptr;      // [0],[1] streaming kernel, [2],[3] global-window
-                                   // kernel: next item, finished workgroups
+  int32_t* d_queue = nullptr;      // next item and finished workgroups of the streaming
+                                   // [0,1], global-window [2,3] and group [4,5] kernels
   float* d_scratch = nullptr;      // window storage of stream/global items
   int64_t scratch_floats = 0;
   int32_t* d_net_maxpid = nullptr;  // per net, for subset solves
   int64_t* d_net_scratch = nullptr; // per net scratch offset (-1: none)
-  int grid = 0;                     // persistent workgroups
+  int grid = 0;                     // persistent workgroups (streaming, global-window)
+  int grid_group = 0;               // persistent workgroups (packed groups)
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 

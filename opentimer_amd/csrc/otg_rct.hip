@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "otg_internal.hpp"
@@ -186,11 +187,13 @@ struct RctParams {
   const int64_t* net_scratch;
   float* scratch;
   int32_t* queue;
-  int32_t long_maxpid;  // largest parent offset the streaming kernel's ring takes
+  int32_t long_maxpid;             // largest parent offset the streaming kernel's ring takes
+  const uint32_t* sched;           // packed groups: schedule words
+  const int32_t* num_items_dev;    // item count on the device (subset solves), or null
+  int32_t* group_flag;             // subset solves: groups listed in this call
+  int32_t group_max;               // largest net of a packed group (G)
 };
 
-// Nets of at most this many nodes are solved as packed groups held on chip.
-constexpr int kShortMax = 256;
 
 template <int W, bool COMP>
 struct RctLds {
@@ -755,7 +758,7 @@ __global__ __launch_bounds__(64) void rct_global_kernel(RctParams P) {
       it.net_end = net + 1;
       const int nn = it.pos_end - it.pos_begin;
       const int mp = gld(P.net_maxpid + net);
-      it.kind = (nn <= kShortMax || mp <= P.long_maxpid) ? ITEM_PACKED : ITEM_GLOBAL;
+      it.kind = (nn <= P.group_max || mp <= P.long_maxpid) ? ITEM_PACKED : ITEM_GLOBAL;
       it.k = 1;
       it.scratch = gld(P.net_scratch + net);
     } else {
@@ -772,7 +775,7 @@ __global__ __launch_bounds__(64) void rct_global_kernel(RctParams P) {
 }
 
 // ===========================================================================
-// Streaming kernel: packed groups of small nets and streamed large nets.
+// Streaming kernel: nets larger than a packed group.
 //
 // Persistent, one wavefront per workgroup, items from a largest-first queue.
 // Positions are visited in windows of 64, one lane per position.  Inside a
@@ -788,19 +791,14 @@ __global__ __launch_bounds__(64) void rct_global_kernel(RctParams P) {
 // Global inputs arrive in chunks of four windows (256 positions) that are
 // requested one chunk ahead.  A wavefront that both loads and stores can
 // only wait for all of its memory operations at once, so the prefetch
-// distance is one chunk of work rather than one window.
-//
-//  * packed group (<= 256 positions, <= 128 whole nets): one chunk; load and
-//    L stay in the ring between the sweeps.
-//  * streamed net: bottom-up over chunks last to first, spilling the
-//    (rounded) load into the load output and L into scratch; then top-down
-//    over chunks first to last, reading them back.
+// distance is one chunk of work rather than one window.  The bottom-up sweep
+// runs over chunks last to first, spilling the (rounded) load into the load
+// output and L into scratch; the top-down sweep runs first to last and reads
+// them back.
 // ===========================================================================
 constexpr int kChunkWin = 4;
-constexpr int kChunkPos = kChunkWin * kWave;  // == kShortMax
-constexpr int kGroupNets = 2 * kWave;
-constexpr int kClaim = 256;  // claim table entries (parent slot mod 256)
-static_assert(kChunkPos == kShortMax, "a packed group is one chunk");
+constexpr int kChunkPos = kChunkWin * kWave;  // 256
+constexpr int kClaim = 256;                   // claim table entries (parent slot mod 256)
 
 template <int W, bool COMP>
 struct StreamLds {
@@ -808,78 +806,71 @@ struct StreamLds {
   float4 b[W];   // bottom-up: L accumulator (hi)    | top-down: impulse (hi)
   float4 al[COMP ? W : 1];
   float4 bl[COMP ? W : 1];
-  float4 si[kGroupNets];               // driver slew of a packed group's nets
-  uint32_t roots[kChunkPos / 32];      // driver positions of a packed group
-  int32_t claim[kClaim];               // one writer per parent slot per round
+  int32_t claim[kClaim];  // one writer per parent slot per round
 };
 
 // Inputs of one chunk: lane l holds positions q0 + 64 s + l, s = 0..3.
 struct Chunk {
   int32_t pid[kChunkWin];
   float r[kChunkWin];
-  float4 x[kChunkWin];  // cap (bottom-up) | load read back (top-down, streamed)
-  float4 y[kChunkWin];  // L read back (top-down, streamed)
-  float4 si[2];         // packed: driver slew of nets l and 64+l | streamed: the net's
-  int32_t ns[2];        // packed: first position of nets l and 64+l
+  float4 x[kChunkWin];  // cap (bottom-up) | load read back (top-down)
+  float4 y[kChunkWin];  // L read back (top-down)
+  float4 si;            // the net's driver slew (top-down)
 };
 
 struct Item {
   int pb, pe, nb, ne, kind;
-  int64_t scratch;
+  int k;            // packed group: its index in the plan's group list
+  int64_t scratch;  // streamed net: float offset of its L scratch;
+                    // packed group: offset of its schedule words
 };
 
-enum : int { PH_GROUP = 0, PH_BU = 1, PH_TD = 2 };
+enum : int { PH_BU = 1, PH_TD = 2 };
 
 __device__ __forceinline__ int num_chunks(const Item& it) {
   return (it.pe - it.pb + kChunkPos - 1) / kChunkPos;
 }
 
-// Loads that must not hit a stale L1 line: the streamed top-down sweep reads
-// back what the same lanes stored during the bottom-up sweep (glc: read from
-// L2).
+// Loads that must not hit a stale L1 line: the top-down sweep reads back
+// what the same lanes stored during the bottom-up sweep (glc: read from L2).
 __device__ __forceinline__ float4 bload4_l2(Rsrc r, int byte_off) {
   const otg_u4v v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 1);
   const otg_f4v f = __builtin_bit_cast(otg_f4v, v);
   return make_float4(f.x, f.y, f.z, f.w);
 }
 
-__device__ __forceinline__ void load_chunk(const RctParams& P, const Item& it, int ph, int c,
-                                           Chunk& ch, int lane) {
+// Every field of the chunk is assigned on every path (a struct filled in
+// part on one path and in part on another is kept in scratch memory).
+template <int PH>
+__device__ __forceinline__ void load_chunk(const RctParams& P, const Item& it, int c, Chunk& ch,
+                                           int lane) {
   const int n = it.pe - it.pb;
-  const int q0 = ph == PH_GROUP ? 0 : c * kChunkPos;
+  const int q0 = c * kChunkPos;
   const Rsrc rp = make_rsrc(P.pid, it.pb, 4, n);
   const Rsrc rr = make_rsrc(P.pres, it.pb, 4, n);
-  if (ph == PH_TD) {
-    const Rsrc rl = make_rsrc(P.load, it.pb, 16, n);
-    const Rsrc rg = make_rsrc(P.scratch + it.scratch, 0, 16, n);
+  const Rsrc rx = PH == PH_TD ? make_rsrc(P.load, it.pb, 16, n) : make_rsrc(P.cap, it.pb, 16, n);
+  const Rsrc ry = make_rsrc(PH == PH_TD ? P.scratch + it.scratch : nullptr, 0, 16, n);
 #pragma unroll
-    for (int s = 0; s < kChunkWin; ++s) {
-      const int q = q0 + s * kWave + lane;
-      ch.pid[s] = bload_i(rp, q * 4);
-      ch.r[s] = bload_f(rr, q * 4);
-      ch.x[s] = bload4_l2(rl, q * 16);
-      ch.y[s] = bload4_l2(rg, q * 16);
-    }
-    ch.si[0] = bload4(make_rsrc(P.slew_in, it.nb, 16, 1), 0);
-  } else {
-    const Rsrc rc = make_rsrc(P.cap, it.pb, 16, n);
-#pragma unroll
-    for (int s = 0; s < kChunkWin; ++s) {
-      const int q = q0 + s * kWave + lane;
-      ch.pid[s] = bload_i(rp, q * 4);
-      ch.r[s] = bload_f(rr, q * 4);
-      ch.x[s] = bload4(rc, q * 16);
-    }
-    if (ph == PH_GROUP) {
-      const int nn = it.ne - it.nb;
-      const Rsrc rs = make_rsrc(P.slew_in, it.nb, 16, nn);
-      const Rsrc rn = make_rsrc(P.nodes_start, it.nb, 4, nn);
-      ch.si[0] = bload4(rs, lane * 16);
-      ch.si[1] = bload4(rs, (lane + kWave) * 16);
-      ch.ns[0] = bload_i(rn, lane * 4);
-      ch.ns[1] = bload_i(rn, (lane + kWave) * 4);
+  for (int s = 0; s < kChunkWin; ++s) {
+    const int q = q0 + s * kWave + lane;
+    ch.pid[s] = bload_i(rp, q * 4);
+    ch.r[s] = bload_f(rr, q * 4);
+    if constexpr (PH == PH_TD) {
+      ch.x[s] = bload4_l2(rx, q * 16);
+      ch.y[s] = bload4_l2(ry, q * 16);
+    } else {
+      ch.x[s] = bload4(rx, q * 16);
+      ch.y[s] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
+  ch.si = PH == PH_TD ? bload4(make_rsrc(P.slew_in, it.nb, 16, 1), 0)
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__device__ __forceinline__ void load_chunk(const RctParams& P, const Item& it, int ph, int c,
+                                           Chunk& ch, int lane) {
+  if (ph == PH_TD) load_chunk<PH_TD>(P, it, c, ch, lane);
+  else load_chunk<PH_BU>(P, it, c, ch, lane);
 }
 
 // In-window depth of this lane's position (-1: no position).  Returns the
@@ -937,40 +928,13 @@ struct SRing {
       S.bl[k] = z;
     }
   }
-  // a[k] += (x, xl), b[k] += (y, yl); compensated: the rounding error of each
-  // high-part add is recovered from the returned old value and added to the
-  // low part.
-  __device__ __forceinline__ void add(int k, const float (&x)[4], const float (&y)[4],
-                                      const float (&xl)[4], const float (&yl)[4]) const {
-    float* pa = reinterpret_cast<float*>(&S.a[k]);
-    float* pb = reinterpret_cast<float*>(&S.b[k]);
-    if constexpr (COMP) {
-      float* pal = reinterpret_cast<float*>(&S.al[k]);
-      float* pbl = reinterpret_cast<float*>(&S.bl[k]);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const float oa = atomicAdd(pa + c, x[c]);
-        const float ob = atomicAdd(pb + c, y[c]);
-        atomicAdd(pal + c, two_sum_err(oa, x[c], oa + x[c]) + xl[c]);
-        atomicAdd(pbl + c, two_sum_err(ob, y[c], ob + y[c]) + yl[c]);
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        atomicAdd(pa + c, x[c]);
-        atomicAdd(pb + c, y[c]);
-      }
-    }
-  }
 };
 
 // Bottom-up window: load_i = C_i + sum load_c, L_i = sum (L_c + R_c load_c^2).
-// Returns the lane's final load and L rounded to FP32 (ldo, Lo).  KEEP
-// (packed group): the final values stay in the lane's slot for the top-down
-// sweep; otherwise the slot is cleared for the position W further back.
-// Both happen once for the whole window at its end: a slot is read only at
-// its own step, and the next user of a cleared slot (position i - W) gets
-// its first child only in a later window.
+// Returns the lane's final load and L rounded to FP32 (ldo, Lo).  The slot of
+// every position of the window is cleared once at the window's end (for the
+// position W further back): a slot is read only at its own step, and its
+// next user gets its first child only in a later window.
 //
 // A child adds into its parent's slot with a plain read-modify-write (LDS
 // float atomics cost several cycles per lane).  Siblings of one step would
@@ -978,7 +942,7 @@ struct SRing {
 // to claim[parent slot mod 256], and the lanes that read their own id back
 // own their parent for the round.  Each round retires at least one lane, and
 // two winners never share a parent.
-template <int W, bool COMP, bool KEEP>
+template <int W, bool COMP>
 __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int pe, int lane,
                                           int pidv, float r, float4 capv, bool root,
                                           float (&ldo)[4], float (&Lo)[4]) {
@@ -1049,17 +1013,75 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
       wave_sync();
     }
   }
-  if constexpr (KEEP) {
-    float z[4] = {0.f, 0.f, 0.f, 0.f};
-    R.put(sl, ldo, Lo, z, z);
-  } else {
-    R.zero(sl);
-  }
+  R.zero(sl);
   wave_sync();
 }
 
+// Final per-node outputs from delay/impulse (d, m, with low parts in the
+// compensated mode), the rounded load and L, and the driver slew.
+template <bool COMP>
+__device__ __forceinline__ void node_outputs(const float (&d)[4], const float (&dl)[4],
+                                             const float (&m)[4], const float (&ml)[4],
+                                             const float (&ld)[4], const float (&L)[4],
+                                             const float4 siv, float (&o_d)[4],
+                                             float (&o_ldel)[4], float (&o_b)[4],
+                                             float (&o_m)[4], float (&o_s)[4]) {
+  float sv[4];
+  unpack(siv, sv);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if constexpr (COMP) {
+      // final per-node combination in FP64
+      const double dd = (double)d[c] + (double)dl[c];
+      const double md = (double)m[c] + (double)ml[c];
+      const double ldd = (double)ld[c];
+      o_d[c] = (float)dd;
+      o_m[c] = (float)md;
+      o_ldel[c] = (float)(dd * ldd + (double)L[c]);
+      o_b[c] = (float)(0.5 * (dd * dd + md));
+      const double q = sqrt((double)sv[c] * (double)sv[c] + md);
+      o_s[c] = (float)(sv[c] < 0.f ? -q : q);
+    } else {
+      o_d[c] = d[c];
+      o_m[c] = m[c];
+      o_ldel[c] = d[c] * ld[c] + L[c];
+      o_b[c] = 0.5f * (d[c] * d[c] + m[c]);
+      const float q = sqrtf(sv[c] * sv[c] + m[c]);
+      o_s[c] = sv[c] < 0.f ? -q : q;
+    }
+  }
+}
+
+// Top-down increments of one node: delay += R load,
+// impulse += (R load)^2 + 2 R L (double-float in the compensated mode).
+template <bool COMP>
+__device__ __forceinline__ void td_node(float rr, const float (&ld)[4], const float (&L)[4],
+                                        const float (&pd)[4], const float (&pdl)[4],
+                                        const float (&pm)[4], const float (&pml)[4],
+                                        float (&d)[4], float (&dl)[4], float (&m)[4],
+                                        float (&ml)[4]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if constexpr (COMP) {
+      const ff uu = ff_mul_f(ff{ld[c], 0.f}, rr);
+      const ff zz = ff_add(ff_mul(uu, uu), ff_mul_f(ff{L[c], 0.f}, 2.0f * rr));
+      const ff dd = ff_add(ff{pd[c], pdl[c]}, uu);
+      const ff mm = ff_add(ff{pm[c], pml[c]}, zz);
+      d[c] = dd.hi;
+      dl[c] = dd.lo;
+      m[c] = mm.hi;
+      ml[c] = mm.lo;
+    } else {
+      const float u = rr * ld[c];
+      d[c] = pd[c] + u;
+      m[c] = pm[c] + (u * u + 2.0f * rr * L[c]);
+      dl[c] = ml[c] = 0.f;
+    }
+  }
+}
+
 // Top-down window: delay_i = delay_p + R_i load_i,
-// impulse_i = impulse_p + (R_i load_i)^2 + 2 R_i L_i; then the lane's six
+// impulse_i = impulse_p + (R_i load_i)^2 + 2 R_i L_i; then the lane's
 // outputs.  ld/L are the lane's rounded bottom-up results.
 template <int W, bool COMP>
 __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int pe, int lane,
@@ -1087,51 +1109,12 @@ __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int p
 #pragma unroll
         for (int c = 0; c < 4; ++c) pd[c] = pm[c] = pdl[c] = pml[c] = 0.f;
       }
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if constexpr (COMP) {
-          const ff uu = ff_mul_f(ff{ld[c], 0.f}, rr);
-          const ff zz = ff_add(ff_mul(uu, uu), ff_mul_f(ff{L[c], 0.f}, 2.0f * rr));
-          const ff dd = ff_add(ff{pd[c], pdl[c]}, uu);
-          const ff mm = ff_add(ff{pm[c], pml[c]}, zz);
-          d[c] = dd.hi;
-          dl[c] = dd.lo;
-          m[c] = mm.hi;
-          ml[c] = mm.lo;
-        } else {
-          const float u = rr * ld[c];
-          d[c] = pd[c] + u;
-          m[c] = pm[c] + (u * u + 2.0f * rr * L[c]);
-        }
-      }
+      td_node<COMP>(rr, ld, L, pd, pdl, pm, pml, d, dl, m, ml);
       R.put(sl, d, m, dl, ml);
     }
     wave_sync();
   }
-  float sv[4];
-  unpack(siv, sv);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    if constexpr (COMP) {
-      // final per-node combination in FP64
-      const double dd = (double)d[c] + (double)dl[c];
-      const double md = (double)m[c] + (double)ml[c];
-      const double ldd = (double)ld[c];
-      o_d[c] = (float)dd;
-      o_m[c] = (float)md;
-      o_ldel[c] = (float)(dd * ldd + (double)L[c]);
-      o_b[c] = (float)(0.5 * (dd * dd + md));
-      const double q = sqrt((double)sv[c] * (double)sv[c] + md);
-      o_s[c] = (float)(sv[c] < 0.f ? -q : q);
-    } else {
-      o_d[c] = d[c];
-      o_m[c] = m[c];
-      o_ldel[c] = d[c] * ld[c] + L[c];
-      o_b[c] = 0.5f * (d[c] * d[c] + m[c]);
-      const float q = sqrtf(sv[c] * sv[c] + m[c]);
-      o_s[c] = sv[c] < 0.f ? -q : q;
-    }
-  }
+  node_outputs<COMP>(d, dl, m, ml, ld, L, siv, o_d, o_ldel, o_b, o_m, o_s);
 }
 
 struct OutBufs {
@@ -1147,66 +1130,6 @@ __device__ __forceinline__ OutBufs out_bufs(const RctParams& P, const Item& it) 
                                         : make_rsrc(nullptr, 0, 16, 0)};
 }
 
-// A packed group: the whole item is one chunk held on chip.
-template <int W, bool COMP>
-__device__ __forceinline__ void run_group(StreamLds<W, COMP>& S, const RctParams& P,
-                                          const Item& it, const Chunk& ch, int lane) {
-  const SRing<W, COMP> R{S, it.pb};
-  const int n = it.pe - it.pb;
-  const int nn = it.ne - it.nb;
-  const int nw = (n + kWave - 1) / kWave;
-  for (int k = lane; k < nw * kWave; k += kWave) R.zero(k);
-  if (lane < kChunkPos / 32) S.roots[lane] = 0u;
-  wave_sync();
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int j = h * kWave + lane;
-    if (j < nn) {
-      const int q = ch.ns[h] - it.pb;
-      atomicOr(&S.roots[q >> 5], 1u << (q & 31));
-      S.si[j] = ch.si[h];
-    }
-  }
-  wave_sync();
-  uint64_t rmask[kChunkWin];
-#pragma unroll
-  for (int s = 0; s < kChunkWin; ++s)
-    rmask[s] = (uint64_t)S.roots[2 * s] | ((uint64_t)S.roots[2 * s + 1] << 32);
-  const OutBufs B = out_bufs(P, it);
-  // bottom-up, windows last to first
-#pragma unroll
-  for (int s = kChunkWin - 1; s >= 0; --s) {
-    if (s < nw) {
-      float ldo[4], Lo[4];
-      bu_window<W, COMP, true>(R, it.pb + s * kWave, it.pe, lane, ch.pid[s], ch.r[s], ch.x[s],
-                               (rmask[s] >> lane) & 1ull, ldo, Lo);
-    }
-  }
-  // top-down, windows first to last
-  int carry = 0;
-#pragma unroll
-  for (int s = 0; s < kChunkWin; ++s) {
-    if (s < nw) {
-      const int q = s * kWave + lane;
-      const int net = carry + __popcll(rmask[s] & ((2ull << lane) - 1ull)) - 1;
-      carry += __popcll(rmask[s]);
-      const float4 siv = S.si[net < 0 ? 0 : (net < kGroupNets ? net : kGroupNets - 1)];
-      float ld[4], L[4], xl[4], yl[4];
-      R.get(R.slot(it.pb + q), ld, L, xl, yl);
-      float o_d[4], o_ldel[4], o_b[4], o_m[4], o_s[4];
-      td_window<W, COMP>(R, it.pb + s * kWave, it.pe, lane, ch.pid[s], ch.r[s],
-                         (rmask[s] >> lane) & 1ull, ld, L, o_d, o_ldel, o_b, o_m, o_s, siv);
-      const int off = q * 16;
-      bstore4(B.load, off, ld);
-      bstore4(B.delay, off, o_d);
-      bstore4(B.ldelay, off, o_ldel);
-      bstore4(B.beta, off, o_b);
-      bstore4(B.imp, off, o_m);
-      bstore4(B.slew, off, o_s);
-    }
-  }
-}
-
 // One chunk of a streamed net's bottom-up sweep (windows last to first).
 template <int W, bool COMP>
 __device__ __forceinline__ void run_bu_chunk(StreamLds<W, COMP>& S, const RctParams& P,
@@ -1219,8 +1142,8 @@ __device__ __forceinline__ void run_bu_chunk(StreamLds<W, COMP>& S, const RctPar
     const int q0 = c * kChunkPos + s * kWave;
     if (q0 < n) {
       float ldo[4], Lo[4];
-      bu_window<W, COMP, false>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], ch.x[s],
-                                q0 + lane == 0, ldo, Lo);
+      bu_window<W, COMP>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], ch.x[s], q0 + lane == 0,
+                         ldo, Lo);
       bstore4(B.load, (q0 + lane) * 16, ldo);
       bstore4(B.gl, (q0 + lane) * 16, Lo);
     }
@@ -1243,7 +1166,7 @@ __device__ __forceinline__ void run_td_chunk(StreamLds<W, COMP>& S, const RctPar
       unpack(ch.y[s], L);
       float o_d[4], o_ldel[4], o_b[4], o_m[4], o_s[4];
       td_window<W, COMP>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], q0 + lane == 0, ld, L,
-                         o_d, o_ldel, o_b, o_m, o_s, ch.si[0]);
+                         o_d, o_ldel, o_b, o_m, o_s, ch.si);
       const int off = (q0 + lane) * 16;
       bstore4(B.delay, off, o_d);
       bstore4(B.ldelay, off, o_ldel);
@@ -1261,55 +1184,48 @@ __device__ __forceinline__ Item item_from(int d) {
   it.nb = __builtin_amdgcn_readlane(d, 2);
   it.ne = __builtin_amdgcn_readlane(d, 3);
   it.kind = __builtin_amdgcn_readlane(d, 4);
+  it.k = __builtin_amdgcn_readlane(d, 5);
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(d, 6);
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(d, 7);
   it.scratch = (int64_t)(((uint64_t)hi << 32) | lo);
   return it;
 }
 
-// The first chunk of an item: the group, or the last chunk of a streamed
-// net's bottom-up sweep.  Empty items and global-window nets have none.
-__device__ __forceinline__ bool first_chunk(const Item& it, int& ph, int& c) {
-  if (it.pe <= it.pb || (it.kind != ITEM_PACKED && it.kind != ITEM_STREAM)) return false;
-  ph = it.kind == ITEM_PACKED ? PH_GROUP : PH_BU;
-  c = ph == PH_GROUP ? 0 : num_chunks(it) - 1;
-  return true;
-}
-
-template <int W, bool COMP>
-__global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
-  __shared__ StreamLds<W, COMP> S;
-  const int lane = threadIdx.x;
-  const int total = P.num_items;
-  const Rsrc ritems = make_rsrc(P.items, 0, (int)sizeof(RctItem), total);
-  auto pop = [&]() {
+// Work items come from a queue; the current item, the next two descriptors
+// and the index after them are always known or in flight, so that the chunk
+// after the current one can be requested whichever item it belongs to.
+struct ItemPipe {
+  int32_t* queue;
+  Rsrc ritems;
+  int total;
+  int lane;
+  int d1, d2, q3;
+  bool v1, v2;
+  __device__ __forceinline__ int pop() const {
     int v = 0;
-    if (lane == 0) v = atomicAdd(&P.queue[0], 1);
+    if (lane == 0) v = atomicAdd(&queue[0], 1);
     return v;  // valid in lane 0
-  };
+  }
   // lanes 0..7 hold the item's eight 32-bit fields; past the end: zeros
-  auto load_desc = [&](int idx) { return bload_i(ritems, idx * (int)sizeof(RctItem) + (lane & 7) * 4); };
-
-  // Item pipeline: the current item, the next two descriptors and the index
-  // after them are always known or in flight, so that the chunk after the
-  // current one can be requested whichever item it belongs to.
-  int q0 = __builtin_amdgcn_readfirstlane(pop());
-  int q1 = pop(), q2 = pop();
-  q1 = __builtin_amdgcn_readfirstlane(q1);
-  q2 = __builtin_amdgcn_readfirstlane(q2);
-  int d1 = load_desc(q1 < total ? q1 : total);
-  int d2 = load_desc(q2 < total ? q2 : total);
-  int q3 = pop();
-  Item it = item_from(load_desc(q0 < total ? q0 : total));
-  bool v1 = q1 < total, v2 = q2 < total;
-  bool have = q0 < total;
-  int ph = 0, c = 0;
-  // skip leading items without chunks
-  while (have && !first_chunk(it, ph, c)) {
-    if (!v1) {
-      have = false;
-      break;
-    }
+  __device__ __forceinline__ int load_desc(int idx) const {
+    return bload_i(ritems, idx * (int)sizeof(RctItem) + (lane & 7) * 4);
+  }
+  // Starts the pipeline; returns the first item (valid if *have).
+  __device__ __forceinline__ Item start(bool* have) {
+    const int q0 = __builtin_amdgcn_readfirstlane(pop());
+    const int q1 = __builtin_amdgcn_readfirstlane(pop());
+    const int q2 = __builtin_amdgcn_readfirstlane(pop());
+    d1 = load_desc(q1 < total ? q1 : total);
+    d2 = load_desc(q2 < total ? q2 : total);
+    q3 = pop();
+    v1 = q1 < total;
+    v2 = q2 < total;
+    *have = q0 < total;
+    return item_from(load_desc(q0 < total ? q0 : total));
+  }
+  // Moves to the next item; false at the end of the queue.
+  __device__ __forceinline__ bool next(Item& it) {
+    if (!v1) return false;
     it = item_from(d1);
     d1 = d2;
     v1 = v2;
@@ -1317,7 +1233,27 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
     v2 = qq < total;
     d2 = load_desc(v2 ? qq : total);
     q3 = pop();
+    return true;
   }
+};
+
+__device__ __forceinline__ bool stream_first_chunk(const Item& it, int& ph, int& c) {
+  if (it.pe <= it.pb || it.kind != ITEM_STREAM) return false;
+  ph = PH_BU;
+  c = num_chunks(it) - 1;
+  return true;
+}
+
+template <int W, bool COMP>
+__global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
+  __shared__ StreamLds<W, COMP> S;
+  const int lane = threadIdx.x;
+  const int total = P.num_items_dev ? gld(P.num_items_dev) : P.num_items;
+  ItemPipe pipe{P.queue, make_rsrc(P.items, 0, (int)sizeof(RctItem), total), total, lane};
+  bool have = false;
+  Item it = pipe.start(&have);
+  int ph = 0, c = 0;
+  while (have && !stream_first_chunk(it, ph, c)) have = pipe.next(it);
   // Chunk data: `cur` is being processed, `nxt` is in flight.  A wavefront
   // that has both loads and stores outstanding can only wait for all of
   // them, so the wait for the next chunk happens once, at the top of the
@@ -1333,9 +1269,9 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
       cur = nxt;
       nxt_pb = -1;
     } else {
-      // read now (first chunk, or a streamed net's top-down chunk 0, which
-      // reads back the bottom-up results stored just before: the wait above
-      // has completed those stores)
+      // read now (first chunk, or top-down chunk 0, which reads back the
+      // bottom-up results stored just before: the wait above has completed
+      // those stores)
       load_chunk(P, it, ph, c, cur, lane);
       __builtin_amdgcn_s_waitcnt(0);
     }
@@ -1358,9 +1294,9 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
         pph = PH_TD;
         pc = c + 1;
         pv = true;
-      } else if (v1) {
-        pit = item_from(d1);
-        pv = first_chunk(pit, pph, pc);
+      } else if (pipe.v1) {
+        pit = item_from(pipe.d1);
+        pv = stream_first_chunk(pit, pph, pc);
       }
       if (pv && !(pit.pb == nxt_pb && pph == nxt_ph && pc == nxt_c)) {
         load_chunk(P, pit, pph, pc, nxt, lane);
@@ -1370,9 +1306,7 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
       }
     }
     // ---- process ----------------------------------------------------------
-    if (ph == PH_GROUP) {
-      run_group<W, COMP>(S, P, it, cur, lane);
-    } else if (ph == PH_BU) {
+    if (ph == PH_BU) {
       if (c == num_chunks(it) - 1) {
         const SRing<W, COMP> R{S, it.pb};
         for (int k = lane; k < W; k += kWave) R.zero(k);
@@ -1391,23 +1325,330 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
     } else if (ph == PH_TD && c + 1 < num_chunks(it)) {
       ++c;
     } else {
-      // next item
       bool found = false;
-      while (v1) {
-        it = item_from(d1);
-        d1 = d2;
-        v1 = v2;
-        const int qq = __builtin_amdgcn_readfirstlane(q3);
-        v2 = qq < total;
-        d2 = load_desc(v2 ? qq : total);
-        q3 = pop();
-        if (first_chunk(it, ph, c)) {
+      while (pipe.next(it)) {
+        if (stream_first_chunk(it, ph, c)) {
           found = true;
           break;
         }
       }
       if (!found) break;
     }
+  }
+  queue_release(P.queue, lane);
+}
+
+// ===========================================================================
+// Packed-group kernel: groups of whole small nets (at most G positions).
+//
+// A group is processed entirely on chip in a static schedule built with the
+// plan: its nodes sorted by depth, then by parent, then by position.  One
+// 32-bit word per schedule index j holds
+//     bits  0..7  the node (group-relative position) at j
+//     bits  8..15 its depth
+//     bits 16..23 schedule index of its first child
+//     bits 24..31 its number of children
+// (children of a node are consecutive in the schedule: same depth, same
+// parent).  A step takes up to 64 consecutive schedule indices of one depth,
+// so every lane of a step has work:
+//  * bottom-up, deepest first: a node pulls the (load, w) of its children
+//    from their schedule slots, w_c = L_c + R_c load_c^2, and writes its own
+//    (no read-modify-write, no conflicts);
+//  * top-down, depth 0 first: a node reads its parent's delay and impulse
+//    (stored by position) and adds its own increments;
+//  * a last pass with lanes on consecutive positions forms the six outputs
+//    and stores them coalesced.
+// Steps per group: about (group size / 64) + depth per sweep, against one
+// step per depth level per 64-position window otherwise.
+// ===========================================================================
+template <int G, bool COMP>
+struct GroupLds {
+  static constexpr int K = G / kWave;        // positions per lane
+  static constexpr int NS = G / (2 * kWave);  // driver slews per lane (<= G/2 nets)
+  float4 cap[G];   // bottom-up: cap by position   | top-down: delay (hi) by position
+  float4 w[G];     // bottom-up: w by schedule     | top-down: impulse (hi) by position
+  float4 ld[G];    // load (hi) by schedule index
+  float4 L[G];     // L rounded to FP32 by schedule index
+  float4 ldl[COMP ? G : 1];  // load (lo) by schedule index
+  float4 wl[COMP ? G : 1];   // bottom-up: w (lo) | top-down: delay (lo) by position
+  float4 ml[COMP ? G : 1];   // top-down: impulse (lo) by position
+  float r[G];
+  int32_t pid[G];
+  uint32_t pw[G];
+  int32_t net[G];           // group-relative net of each position (top-down)
+  int32_t pos2j[G];         // schedule index of each position
+  float4 si[G / 2];
+};
+
+template <int G>
+struct GroupChunk {
+  static constexpr int K = G / kWave;
+  static constexpr int NS = G / (2 * kWave);
+  int32_t pid[K];
+  float r[K];
+  float4 cap[K];
+  uint32_t pw[K];
+  float4 si[NS];
+};
+
+template <int G>
+__device__ __forceinline__ void load_group(const RctParams& P, const Item& it, GroupChunk<G>& ch,
+                                           int lane) {
+  constexpr int K = GroupChunk<G>::K;
+  const int n = it.pe - it.pb;
+  const int nn = it.ne - it.nb;
+  const Rsrc rp = make_rsrc(P.pid, it.pb, 4, n);
+  const Rsrc rr = make_rsrc(P.pres, it.pb, 4, n);
+  const Rsrc rc = make_rsrc(P.cap, it.pb, 16, n);
+  const Rsrc rw = make_rsrc(P.sched, it.scratch, 4, n);
+  const Rsrc rs = make_rsrc(P.slew_in, it.nb, 16, nn);
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int q = s * kWave + lane;
+    ch.pid[s] = bload_i(rp, q * 4);
+    ch.r[s] = bload_f(rr, q * 4);
+    ch.cap[s] = bload4(rc, q * 16);
+    ch.pw[s] = (uint32_t)bload_i(rw, q * 4);
+  }
+#pragma unroll
+  for (int s = 0; s < GroupChunk<G>::NS; ++s) ch.si[s] = bload4(rs, (s * kWave + lane) * 16);
+}
+
+template <int G, bool COMP>
+__device__ __forceinline__ void run_group(GroupLds<G, COMP>& S, const RctParams& P,
+                                          const Item& it, const GroupChunk<G>& ch, int lane) {
+  constexpr int K = GroupChunk<G>::K;
+  constexpr int kPull = 4;  // children fetched per batch
+  const int n = it.pe - it.pb;
+  // ---- stage the group ------------------------------------------------------
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int q = s * kWave + lane;
+    S.cap[q] = ch.cap[s];
+    S.r[q] = ch.r[s];
+    S.pid[q] = ch.pid[s];
+    S.pw[q] = ch.pw[s];
+    if (q < n) S.pos2j[ch.pw[s] & 0xffu] = q;
+  }
+#pragma unroll
+  for (int s = 0; s < GroupChunk<G>::NS; ++s) S.si[s * kWave + lane] = ch.si[s];
+  wave_sync();
+  // ---- bottom-up, deepest schedule entries first ------------------------------
+  uint32_t pwj = n - kWave + lane >= 0 ? S.pw[n - kWave + lane] : 0u;
+  for (int e = n; e > 0;) {
+    const int j = e - kWave + lane;
+    const int dj = (int)((pwj >> 8) & 0xffu);
+    const int dlast = __builtin_amdgcn_readlane(dj, kWave - 1);  // depth of schedule entry e-1
+    const uint64_t m = __ballot(j >= 0 && dj == dlast);
+    const int b = e - __popcll(m);
+    // schedule words of the next step, requested before this step's work
+    const uint32_t pwn = b - kWave + lane >= 0 ? S.pw[b - kWave + lane] : 0u;
+    if (j >= b) {
+      const int q = (int)(pwj & 0xffu);
+      const int cf = (int)((pwj >> 16) & 0xffu);
+      const int cn = (int)(pwj >> 24);
+      float cap4[4];
+      unpack(S.cap[q], cap4);
+      const float r = S.r[q];
+      ff ld[4], sw[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        ld[c] = ff{cap4[c], 0.f};
+        sw[c] = ff{0.f, 0.f};
+      }
+      for (int k0 = 0; k0 < cn; k0 += kPull) {
+        float4 cl[kPull], cw[kPull], cll[kPull], cwl[kPull];
+#pragma unroll
+        for (int u = 0; u < kPull; ++u) {
+          const int jc = cf + k0 + u;
+          const bool ok = k0 + u < cn;
+          cl[u] = ok ? S.ld[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
+          cw[u] = ok ? S.w[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
+          if constexpr (COMP) {
+            cll[u] = ok ? S.ldl[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
+            cwl[u] = ok ? S.wl[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kPull; ++u) {
+          float a1[4], a2[4], a3[4], a4[4];
+          unpack(cl[u], a1);
+          unpack(cw[u], a2);
+          if constexpr (COMP) {
+            unpack(cll[u], a3);
+            unpack(cwl[u], a4);
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            if constexpr (COMP) {
+              ld[c] = ff_add(ld[c], ff{a1[c], a3[c]});
+              sw[c] = ff_add(sw[c], ff{a2[c], a4[c]});
+            } else {
+              ld[c].hi += a1[c];
+              sw[c].hi += a2[c];
+            }
+          }
+        }
+      }
+      float o_ld[4], o_w[4], o_ldl[4], o_wl[4], o_L[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if constexpr (COMP) {
+          const ff q2 = ff_add(sw[c], ff_mul_f(ff_mul(ld[c], ld[c]), r));
+          o_ld[c] = ld[c].hi;
+          o_ldl[c] = ld[c].lo;
+          o_w[c] = q2.hi;
+          o_wl[c] = q2.lo;
+          o_L[c] = sw[c].hi + sw[c].lo;
+        } else {
+          o_ld[c] = ld[c].hi;
+          o_w[c] = sw[c].hi + r * ld[c].hi * ld[c].hi;
+          o_L[c] = sw[c].hi;
+        }
+      }
+      S.ld[j] = pack(o_ld);
+      S.w[j] = pack(o_w);
+      S.L[j] = pack(o_L);
+      if constexpr (COMP) {
+        S.ldl[j] = pack(o_ldl);
+        S.wl[j] = pack(o_wl);
+      }
+    }
+    wave_sync();
+    e = b;
+    pwj = pwn;
+  }
+  // ---- top-down, depth 0 first: delay, impulse and net by position -----------
+  pwj = lane < n ? S.pw[lane] : 0xffffffffu;
+  for (int b = 0; b < n;) {
+    const int j = b + lane;
+    const int dj = (int)((pwj >> 8) & 0xffu);
+    const int dfirst = __builtin_amdgcn_readfirstlane(dj);  // depth of schedule entry b
+    const uint64_t m = __ballot(j < n && dj == dfirst);
+    const int e = b + __popcll(m);
+    const uint32_t pwn = e + lane < n ? S.pw[e + lane] : 0xffffffffu;
+    if (j < e) {
+      const int q = (int)(pwj & 0xffu);
+      const bool root = dfirst == 0;
+      const float rr = root ? 0.f : S.r[q];
+      const int p = root ? 0 : q - S.pid[q];
+      // drivers are the depth-0 schedule entries, in position (= net) order
+      const int net = root ? j : S.net[p];
+      float ldv[4], Lv[4], ldlv[4];
+      unpack(S.ld[j], ldv);
+      unpack(S.L[j], Lv);
+      if constexpr (COMP) {
+        unpack(S.ldl[j], ldlv);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ldv[c] = ldv[c] + ldlv[c];
+      }
+      float pd[4], pm[4], pdl[4], pml[4];
+      if (!root) {
+        unpack(S.cap[p], pd);
+        unpack(S.w[p], pm);
+        if constexpr (COMP) {
+          unpack(S.wl[p], pdl);
+          unpack(S.ml[p], pml);
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) pdl[c] = pml[c] = 0.f;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pd[c] = pm[c] = pdl[c] = pml[c] = 0.f;
+      }
+      float d[4], dl[4], mm[4], ml[4];
+      td_node<COMP>(rr, ldv, Lv, pd, pdl, pm, pml, d, dl, mm, ml);
+      S.cap[q] = pack(d);
+      S.w[q] = pack(mm);
+      if constexpr (COMP) {
+        S.wl[q] = pack(dl);
+        S.ml[q] = pack(ml);
+      }
+      S.net[q] = net;
+    }
+    wave_sync();
+    b = e;
+    pwj = pwn;
+  }
+  // ---- outputs, lanes on consecutive positions (coalesced stores) ------------
+  const Rsrc rout[6] = {make_rsrc(P.load, it.pb, 16, n),   make_rsrc(P.delay, it.pb, 16, n),
+                        make_rsrc(P.ldelay, it.pb, 16, n), make_rsrc(P.beta, it.pb, 16, n),
+                        make_rsrc(P.impulse, it.pb, 16, n), make_rsrc(P.slew, it.pb, 16, n)};
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int q = s * kWave + lane;
+    if (s * kWave < n) {
+      const int j = q < n ? S.pos2j[q] : 0;
+      const int net = q < n ? S.net[q] : 0;
+      float ldv[4], Lv[4], ldlv[4], d[4], dl[4], mm[4], ml[4];
+      unpack(S.ld[j], ldv);
+      unpack(S.L[j], Lv);
+      unpack(S.cap[q], d);
+      unpack(S.w[q], mm);
+      if constexpr (COMP) {
+        unpack(S.ldl[j], ldlv);
+        unpack(S.wl[q], dl);
+        unpack(S.ml[q], ml);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ldv[c] = ldv[c] + ldlv[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dl[c] = ml[c] = 0.f;
+      }
+      float o_d[4], o_ldel[4], o_b[4], o_m[4], o_s[4];
+      node_outputs<COMP>(d, dl, mm, ml, ldv, Lv, S.si[net < G / 2 ? net : G / 2 - 1], o_d,
+                         o_ldel, o_b, o_m, o_s);
+      const int off = q * 16;
+      bstore4(rout[0], off, ldv);
+      bstore4(rout[1], off, o_d);
+      bstore4(rout[2], off, o_ldel);
+      bstore4(rout[3], off, o_b);
+      bstore4(rout[4], off, o_m);
+      bstore4(rout[5], off, o_s);
+    }
+  }
+  wave_sync();
+}
+
+template <int G, bool COMP>
+__global__ __launch_bounds__(64) void rct_group_kernel(RctParams P) {
+  __shared__ GroupLds<G, COMP> S;
+  const int lane = threadIdx.x;
+  const int total = P.num_items_dev ? gld(P.num_items_dev) : P.num_items;
+  ItemPipe pipe{P.queue, make_rsrc(P.items, 0, (int)sizeof(RctItem), total), total, lane};
+  bool have = false;
+  Item it = pipe.start(&have);
+  while (have && it.pe <= it.pb) have = pipe.next(it);
+  GroupChunk<G> cur, nxt;
+  int nxt_pb = -1;
+  while (have) {
+    __builtin_amdgcn_s_waitcnt(0);
+    if (it.pb == nxt_pb) {
+      cur = nxt;
+      nxt_pb = -1;
+    } else {
+      load_group<G>(P, it, cur, lane);
+      __builtin_amdgcn_s_waitcnt(0);
+    }
+    if (pipe.v1) {
+      const Item pit = item_from(pipe.d1);
+      if (pit.pe > pit.pb) {
+        load_group<G>(P, pit, nxt, lane);
+        nxt_pb = pit.pb;
+      }
+    }
+    run_group<G, COMP>(S, P, it, cur, lane);
+    // a group listed by a subset solve is released for the next call
+    if (P.group_flag && lane == 0) P.group_flag[it.k] = 0;
+    bool found = false;
+    while (pipe.next(it)) {
+      if (it.pe > it.pb) {
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
   }
   queue_release(P.queue, lane);
 }
@@ -1430,35 +1671,54 @@ __global__ void rct_netinfo_kernel(int32_t num_nets, const int32_t* __restrict__
   if (!ok) atomicMin(bad, net);
 }
 
-// Subset solves: one item per listed net, classified as the plan would.
-// Nets outside [0, num_nets) and global-window nets become empty items
-// (the latter are taken by rct_global_kernel).
+// Group capacity (positions) of a plan: 128 or 256, from small_target.
+inline int group_cap(int small_target) { return small_target <= 128 ? 128 : 256; }
+
+// Subset solves, streamed nets: one item per listed net.  Nets outside
+// [0, num_nets), packed nets and global-window nets become empty or
+// non-streamed items that the streaming kernel skips.
 __global__ void rct_subset_items_kernel(int32_t count, const int32_t* __restrict__ subset,
                                         int32_t num_nets, const int32_t* __restrict__ nodes_start,
                                         const int32_t* __restrict__ net_maxpid,
                                         const int64_t* __restrict__ net_scratch,
-                                        int32_t long_maxpid, RctItem* __restrict__ items) {
+                                        int32_t group_max, int32_t long_maxpid,
+                                        RctItem* __restrict__ items) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   const int net = subset[j];
   RctItem it{0, 0, 0, 0, ITEM_PACKED, 1, 0};
   if (net >= 0 && net < num_nets) {
-    it.pos_begin = nodes_start[net];
-    it.pos_end = nodes_start[net + 1];
-    it.net_begin = net;
-    it.net_end = net + 1;
-    const int n = it.pos_end - it.pos_begin;
-    if (n > kShortMax) {
-      if (net_maxpid[net] <= long_maxpid) {
-        it.kind = ITEM_STREAM;
-        it.scratch = net_scratch[net];
-      } else {
-        it.kind = ITEM_GLOBAL;
-        it.pos_end = it.pos_begin;
-      }
+    const int pb = nodes_start[net], pe = nodes_start[net + 1];
+    if (pe - pb > group_max && net_maxpid[net] <= long_maxpid) {
+      it = RctItem{pb, pe, net, net + 1, ITEM_STREAM, 1, net_scratch[net]};
     }
   }
   items[j] = it;
+}
+
+// Subset solves, packed nets: the groups holding the listed nets, each once.
+// A group is claimed through its flag (0 -> 1); the group kernel clears the
+// flag after solving it, so every call starts from all-clear flags.
+__global__ void rct_subset_groups_kernel(int32_t count, const int32_t* __restrict__ subset,
+                                         int32_t num_nets, const int32_t* __restrict__ net_group,
+                                         int32_t* __restrict__ group_flag,
+                                         const RctItem* __restrict__ groups,
+                                         RctItem* __restrict__ out, int32_t* __restrict__ out_count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int net = subset[j];
+  if (net < 0 || net >= num_nets) return;
+  const int g = net_group[net];
+  if (g < 0) return;
+  if (atomicCAS(&group_flag[g], 0, 1) != 0) return;
+  out[atomicAdd(out_count, 1)] = groups[g];
+}
+
+// Clears a small device counter (a kernel rather than a memset node, so that
+// captured graphs replay it correctly).
+__global__ void rct_zero_kernel(int32_t* p, int32_t n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) p[j] = 0;
 }
 
 template <int W, bool COMP>
@@ -1477,9 +1737,61 @@ void* pick_stream(int W, bool comp) {
   return stream_kernel_ptr<512, false>();
 }
 
+void* pick_group(int G, bool comp) {
+  if (comp)
+    return G <= 128 ? reinterpret_cast<void*>(&rct_group_kernel<128, true>)
+                    : reinterpret_cast<void*>(&rct_group_kernel<256, true>);
+  return G <= 128 ? reinterpret_cast<void*>(&rct_group_kernel<128, false>)
+                  : reinterpret_cast<void*>(&rct_group_kernel<256, false>);
+}
+
 void* pick_global(bool comp) {
   return comp ? reinterpret_cast<void*>(&rct_global_kernel<true>)
               : reinterpret_cast<void*>(&rct_global_kernel<false>);
+}
+
+int blocks_per_cu(void* fn) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess || nb <= 0) nb = 1;
+  return nb;
+}
+
+// Static schedule of one packed group (positions [pb, pe) of the host
+// arrays): nodes sorted by (depth, parent, position); one word per schedule
+// index (node | depth << 8 | first child << 16 | children << 24).
+void build_group_schedule(const int32_t* hpid, const int32_t* hstart, const RctItem& g,
+                          uint32_t* out) {
+  const int n = g.pos_end - g.pos_begin;
+  int depth[256], par[256], order[256], cf[256], cn[256];
+  int next_root = g.net_begin;
+  for (int q = 0; q < n; ++q) {
+    const int pos = g.pos_begin + q;
+    if (next_root < g.net_end && hstart[next_root] == pos) {
+      par[q] = -1;
+      depth[q] = 0;
+      ++next_root;
+    } else {
+      par[q] = q - hpid[pos];
+      depth[q] = depth[par[q]] + 1;
+    }
+    order[q] = q;
+    cf[q] = 0;
+    cn[q] = 0;
+  }
+  std::sort(order, order + n, [&](int x, int y) {
+    if (depth[x] != depth[y]) return depth[x] < depth[y];
+    if (par[x] != par[y]) return par[x] < par[y];
+    return x < y;
+  });
+  for (int j = 0; j < n; ++j) {
+    const int p = par[order[j]];
+    if (p >= 0 && cn[p]++ == 0) cf[p] = j;
+  }
+  for (int j = 0; j < n; ++j) {
+    const int q = order[j];
+    out[j] = (uint32_t)q | ((uint32_t)depth[q] << 8) | ((uint32_t)cf[q] << 16) |
+             ((uint32_t)cn[q] << 24);
+  }
 }
 
 }  // namespace
@@ -1494,9 +1806,9 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   const otg_rct_tuning& d = ctx->tuning;
   if (!r.ring_slots) r.ring_slots = d.ring_slots ? d.ring_slots : 512;
   if (r.ring_slots != 256 && r.ring_slots != 512 && r.ring_slots != 1024) r.ring_slots = 512;
-  if (!r.small_target) r.small_target = d.small_target ? d.small_target : kShortMax;
-  // a packed group is one chunk: at most 256 positions
-  r.small_target = std::max(64, std::min(r.small_target, kShortMax));
+  if (!r.small_target) r.small_target = d.small_target ? d.small_target : 256;
+  // a packed group holds at most 256 positions (8-bit schedule fields)
+  r.small_target = std::max(64, std::min(r.small_target, 256));
   if (!r.grid_per_cu) r.grid_per_cu = d.grid_per_cu;
   r.reserved = 0;
   (void)m;
@@ -1504,13 +1816,8 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
 }
 
 int otg_rct_blocks_per_cu(otg_context* ctx, const otg_rct_tuning& t, bool comp) {
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_stream(t.ring_slots, comp), 64, 0) !=
-          hipSuccess ||
-      nb <= 0)
-    nb = 1;
   (void)ctx;
-  return nb;
+  return blocks_per_cu(pick_stream(t.ring_slots, comp));
 }
 
 otg_status otg_rct_launch_plain(otg_context* ctx, const otg_rct_tree* tree,
@@ -1533,13 +1840,14 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
                                   int32_t count, hipStream_t s) {
   const bool comp = plan->method == OTG_RCT_BATCHED_COMPENSATED;
   if (plan->stats[2] > 0 && !out->load) {
-    OTG_SET_ERR(ctx, "load output is required: the tree holds nets larger than the window");
+    OTG_SET_ERR(ctx, "load output is required: the tree holds nets larger than a group");
     return OTG_ERR_INVALID_ARGUMENT;
   }
   if (subset && count > plan->num_nets) {
     OTG_SET_ERR(ctx, "subset of %d nets exceeds the plan's %d nets", count, plan->num_nets);
     return OTG_ERR_INVALID_ARGUMENT;
   }
+  if (subset && count <= 0) return OTG_OK;
   RctParams P{};
   P.nodes_start = tree->nodes_start;
   P.pid = tree->pid;
@@ -1556,43 +1864,73 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   P.net_maxpid = plan->d_net_maxpid;
   P.net_scratch = plan->d_net_scratch;
   P.scratch = plan->d_scratch;
+  P.sched = plan->d_sched;
   P.long_maxpid = plan->tuning.ring_slots - kWave;
-  // ---- packed groups and streamed nets ------------------------------------
-  int total = plan->num_items;
-  P.items = plan->d_items;
-  if (subset) {
-    if (count <= 0) return OTG_OK;
-    hipLaunchKernelGGL(rct_subset_items_kernel, dim3((count + 255) / 256), dim3(256), 0, s, count,
-                       subset, plan->num_nets, tree->nodes_start, plan->d_net_maxpid,
-                       plan->d_net_scratch, P.long_maxpid, plan->d_sub_items);
-    OTG_HIP(ctx, hipGetLastError());
-    P.items = plan->d_sub_items;
-    total = count;
+  P.group_max = plan->group_cap;
+  const int G = plan->group_cap;
+  // ---- packed groups ------------------------------------------------------------
+  if (plan->num_groups > 0) {
+    RctParams Q = P;
+    Q.queue = plan->d_queue + 4;
+    if (subset) {
+      hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, plan->d_sub_count, 1);
+      hipLaunchKernelGGL(rct_subset_groups_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+                         count, subset, plan->num_nets, plan->d_net_group, plan->d_group_flag,
+                         plan->d_groups, plan->d_sub_groups, plan->d_sub_count);
+      OTG_HIP(ctx, hipGetLastError());
+      Q.items = plan->d_sub_groups;
+      Q.num_items = plan->num_groups;  // bound; the count is read on the device
+      Q.num_items_dev = plan->d_sub_count;
+      Q.group_flag = plan->d_group_flag;
+    } else {
+      Q.items = plan->d_groups;
+      Q.num_items = plan->num_groups;
+    }
+    const int grid = std::min(plan->grid_group, subset ? std::min(count, plan->num_groups)
+                                                       : plan->num_groups);
+    void* args[] = {&Q};
+    OTG_HIP(ctx, hipLaunchKernel(pick_group(G, comp), dim3(std::max(1, grid)), dim3(64), args, 0, s));
   }
-  P.num_items = total;
-  P.queue = plan->d_queue;
-  if (total > 0) {
-    const int grid = std::min(plan->grid, total);
-    void* args[] = {&P};
-    OTG_HIP(ctx, hipLaunchKernel(pick_stream(plan->tuning.ring_slots, comp), dim3(grid), dim3(64),
-                                 args, 0, s));
+  // ---- streamed nets -----------------------------------------------------------
+  {
+    RctParams Q = P;
+    Q.queue = plan->d_queue;
+    int total = plan->num_items;
+    Q.items = plan->d_items;
+    if (subset && plan->stats[2] > 0) {
+      hipLaunchKernelGGL(rct_subset_items_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+                         count, subset, plan->num_nets, tree->nodes_start, plan->d_net_maxpid,
+                         plan->d_net_scratch, G, P.long_maxpid, plan->d_sub_items);
+      OTG_HIP(ctx, hipGetLastError());
+      Q.items = plan->d_sub_items;
+      total = count;
+    } else if (subset) {
+      total = 0;
+    }
+    Q.num_items = total;
+    if (total > 0) {
+      const int grid = std::min(plan->grid, total);
+      void* args[] = {&Q};
+      OTG_HIP(ctx, hipLaunchKernel(pick_stream(plan->tuning.ring_slots, comp), dim3(grid), dim3(64),
+                                   args, 0, s));
+    }
   }
   // ---- nets whose parent offsets exceed the ring ------------------------------
   if (plan->stats[3] > 0) {
-    RctParams G = P;
-    G.queue = plan->d_queue + 2;
+    RctParams Q = P;
+    Q.queue = plan->d_queue + 2;
     if (subset) {
-      G.subset = subset;
-      G.subset_count = count;
-      G.items = nullptr;
-      G.num_items = 0;
+      Q.subset = subset;
+      Q.subset_count = count;
+      Q.items = nullptr;
+      Q.num_items = 0;
     } else {
-      G.items = plan->d_gitems;
-      G.num_items = plan->num_gitems;
+      Q.items = plan->d_gitems;
+      Q.num_items = plan->num_gitems;
     }
     const int gt = subset ? count : plan->num_gitems;
     const int grid = std::min(plan->grid, std::max(1, gt));
-    void* args[] = {&G};
+    void* args[] = {&Q};
     OTG_HIP(ctx, hipLaunchKernel(pick_global(comp), dim3(grid), dim3(64), args, 0, s));
   }
   return OTG_OK;
@@ -1615,7 +1953,10 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   p->num_nets = topo->num_nets;
   p->total_nodes = topo->total_num_nodes;
   const int N = topo->num_nets;
+  const int64_t V = topo->total_num_nodes;
   const int W = p->tuning.ring_slots;
+  const int G = group_cap(p->tuning.small_target);
+  p->group_cap = G;
   const bool comp = method == OTG_RCT_BATCHED_COMPENSATED;
   auto fail = [&](otg_status st) {
     otg_rct_plan_destroy(p);
@@ -1658,21 +1999,28 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
     }
   }
   // ---- binning --------------------------------------------------------------
-  // Nets of up to 256 nodes go into packed groups (consecutive nets, at most
-  // small_target positions and 128 nets per group; a larger net alone).
-  // Larger nets are streamed when their parent offsets fit the ring
-  // (maxpid <= W - 64) and take the global-window path otherwise.  Streamed
-  // nets get 4 floats of scratch per node (their L), global-window nets 8 (16
-  // compensated).  Streamed nets are queued largest first, ahead of the groups.
-  std::vector<RctItem> big_items, packed, gitems;
+  // Nets of up to G nodes (G = 128 or 256) go into packed groups: consecutive
+  // nets, at most small_target positions and G/2 nets per group (a larger
+  // net alone).  Larger nets are streamed when their parent offsets fit the
+  // ring (maxpid <= W - 64) and take the global-window path otherwise.
+  // Streamed nets get 4 floats of scratch per node (their L), global-window
+  // nets 8 (16 compensated).  Streamed nets are queued largest first.
+  std::vector<RctItem> big_items, groups, gitems;
   std::vector<int64_t> net_scratch(N, -1);
-  int64_t scratch = 0;
+  std::vector<int32_t> net_group(N, -1);
+  int64_t scratch = 0, sched_words = 0;
   const int64_t slot_floats = comp ? 16 : 8;
   int64_t n_packed_nets = 0, n_stream = 0, n_global = 0, maxn = 0;
   RctItem cur{};
   bool open = false;
   auto flush = [&]() {
-    if (open) packed.push_back(cur);
+    if (open) {
+      cur.k = (int32_t)groups.size();
+      cur.scratch = sched_words;
+      sched_words += cur.pos_end - cur.pos_begin;
+      for (int k = cur.net_begin; k < cur.net_end; ++k) net_group[k] = cur.k;
+      groups.push_back(cur);
+    }
     open = false;
   };
   for (int k = 0; k < N; ++k) {
@@ -1682,15 +2030,15 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
       flush();
       continue;
     }
-    if (n <= kShortMax) {
+    if (n <= G) {
       ++n_packed_nets;
       if (open && (cur.pos_end - cur.pos_begin) + n <= p->tuning.small_target &&
-          cur.net_end - cur.net_begin < 2 * kWave) {
+          cur.net_end - cur.net_begin < G / 2) {
         cur.pos_end += n;
         cur.net_end = k + 1;
       } else {
         flush();
-        cur = RctItem{hstart[k], hstart[k + 1], k, k + 1, ITEM_PACKED, 1, 0};
+        cur = RctItem{hstart[k], hstart[k + 1], k, k + 1, ITEM_PACKED, 0, 0};
         open = true;
       }
       continue;
@@ -1715,23 +2063,48 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   };
   std::stable_sort(big_items.begin(), big_items.end(), larger);
   std::stable_sort(gitems.begin(), gitems.end(), larger);
-  std::vector<RctItem> items;
-  items.reserve(big_items.size() + packed.size());
-  items.insert(items.end(), big_items.begin(), big_items.end());
-  items.insert(items.end(), packed.begin(), packed.end());
-  p->num_items = (int32_t)items.size();
+  p->num_items = (int32_t)big_items.size();
   p->num_gitems = (int32_t)gitems.size();
-  p->stats[0] = (int64_t)packed.size();
+  p->num_groups = (int32_t)groups.size();
+  p->stats[0] = p->num_groups;
   p->stats[1] = n_packed_nets;
   p->stats[2] = n_stream;
   p->stats[3] = n_global;
   p->stats[4] = maxn;
-  p->stats[5] = p->num_items + p->num_gitems;
+  p->stats[5] = p->num_items + p->num_gitems + p->num_groups;
   p->stats[6] = scratch;
   p->stats[7] = W;
-  if (!items.empty()) {
-    PLAN_HIP(hipMalloc(&p->d_items, sizeof(RctItem) * items.size()));
-    PLAN_HIP(hipMemcpyAsync(p->d_items, items.data(), sizeof(RctItem) * items.size(),
+  // ---- group schedules (host, from the parent offsets) ---------------------------
+  if (!groups.empty()) {
+    std::vector<int32_t> hpid(V);
+    PLAN_HIP(hipMemcpy(hpid.data(), topo->pid, sizeof(int32_t) * V, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> sched(sched_words);
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t ng = groups.size();
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < hw; ++t)
+      pool.emplace_back([&, t]() {
+        for (size_t g = t; g < ng; g += hw)
+          build_group_schedule(hpid.data(), hstart.data(), groups[g],
+                               sched.data() + groups[g].scratch);
+      });
+    for (auto& th : pool) th.join();
+    PLAN_HIP(hipMalloc(&p->d_sched, sizeof(uint32_t) * sched_words));
+    PLAN_HIP(hipMemcpy(p->d_sched, sched.data(), sizeof(uint32_t) * sched_words,
+                       hipMemcpyHostToDevice));
+    PLAN_HIP(hipMalloc(&p->d_groups, sizeof(RctItem) * ng));
+    PLAN_HIP(hipMemcpy(p->d_groups, groups.data(), sizeof(RctItem) * ng, hipMemcpyHostToDevice));
+    PLAN_HIP(hipMalloc(&p->d_sub_groups, sizeof(RctItem) * ng));
+    PLAN_HIP(hipMalloc(&p->d_group_flag, sizeof(int32_t) * ng));
+    PLAN_HIP(hipMemset(p->d_group_flag, 0, sizeof(int32_t) * ng));
+    PLAN_HIP(hipMalloc(&p->d_sub_count, sizeof(int32_t) * 2));
+    PLAN_HIP(hipMalloc(&p->d_net_group, sizeof(int32_t) * N));
+    PLAN_HIP(hipMemcpy(p->d_net_group, net_group.data(), sizeof(int32_t) * N,
+                       hipMemcpyHostToDevice));
+  }
+  if (!big_items.empty()) {
+    PLAN_HIP(hipMalloc(&p->d_items, sizeof(RctItem) * big_items.size()));
+    PLAN_HIP(hipMemcpyAsync(p->d_items, big_items.data(), sizeof(RctItem) * big_items.size(),
                             hipMemcpyHostToDevice, ctx->stream));
   }
   if (!gitems.empty()) {
@@ -1742,16 +2115,17 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   if (N > 0) {
     PLAN_HIP(hipMemcpyAsync(p->d_net_scratch, net_scratch.data(), sizeof(int64_t) * N,
                             hipMemcpyHostToDevice, ctx->stream));
-    PLAN_HIP(hipMalloc(&p->d_sub_items, sizeof(RctItem) * N));
+    if (n_stream > 0) PLAN_HIP(hipMalloc(&p->d_sub_items, sizeof(RctItem) * N));
   }
   p->scratch_floats = scratch;
   if (scratch > 0) PLAN_HIP(hipMalloc(&p->d_scratch, sizeof(float) * scratch));
-  PLAN_HIP(hipMalloc(&p->d_queue, sizeof(int32_t) * 4));
-  PLAN_HIP(hipMemsetAsync(p->d_queue, 0, sizeof(int32_t) * 4, ctx->stream));
+  PLAN_HIP(hipMalloc(&p->d_queue, sizeof(int32_t) * 8));
+  PLAN_HIP(hipMemsetAsync(p->d_queue, 0, sizeof(int32_t) * 8, ctx->stream));
   PLAN_HIP(hipStreamSynchronize(ctx->stream));
-  const int per_cu =
-      p->tuning.grid_per_cu > 0 ? p->tuning.grid_per_cu : otg_rct_blocks_per_cu(ctx, p->tuning, comp);
-  p->grid = std::max(1, per_cu * std::max(1, ctx->num_cus));
+  const int ncu = std::max(1, ctx->num_cus);
+  const int g1 = p->tuning.grid_per_cu;
+  p->grid = std::max(1, (g1 > 0 ? g1 : blocks_per_cu(pick_stream(W, comp))) * ncu);
+  p->grid_group = std::max(1, (g1 > 0 ? g1 : blocks_per_cu(pick_group(G, comp))) * ncu);
 #undef PLAN_HIP
   *out = p;
   return OTG_OK;
@@ -1762,6 +2136,12 @@ extern "C" otg_status otg_rct_plan_destroy(otg_rct_plan* p) {
   if (p->d_items) hipFree(p->d_items);
   if (p->d_gitems) hipFree(p->d_gitems);
   if (p->d_sub_items) hipFree(p->d_sub_items);
+  if (p->d_groups) hipFree(p->d_groups);
+  if (p->d_sub_groups) hipFree(p->d_sub_groups);
+  if (p->d_sched) hipFree(p->d_sched);
+  if (p->d_group_flag) hipFree(p->d_group_flag);
+  if (p->d_sub_count) hipFree(p->d_sub_count);
+  if (p->d_net_group) hipFree(p->d_net_group);
   if (p->d_queue) hipFree(p->d_queue);
   if (p->d_scratch) hipFree(p->d_scratch);
   if (p->d_net_maxpid) hipFree(p->d_net_maxpid);
