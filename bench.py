@@ -2,10 +2,12 @@
 """Benchmark: RC nets/s (Elmore + slew, 4 corners) and levelize edges/s.
 
 One step = one solve of every net of the batching workload (4M heavy-tailed
-RC trees, ~208M nodes) with the size-binned persistent kernel, inputs and
-outputs resident in HBM.  Also measured on the same data: the plain
-one-net-per-thread kernel, the per-net BFS ordering, and sink-first
-levelization of a 10M-pin / 25M-arc DAG.  Prints ONE JSON line.
+RC trees, ~208M nodes) with the size-binned persistent kernels (packed groups,
+streamed nets), inputs and outputs resident in HBM, timed with HIP events on
+the launch stream.  The headline method is the FP32-compensated one (--method);
+the FP32 method, the plain one-net-per-thread kernel, the per-net BFS
+ordering and sink-first levelization of a 10M-pin / 25M-arc DAG are measured
+beside it.  Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -34,7 +36,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nets", type=int, default=4_000_000)
-    ap.add_argument("--method", default="batched", choices=["batched", "compensated"])
+    ap.add_argument("--method", default="compensated", choices=["batched", "compensated"],
+                    help="headline method (the other batched method is timed beside it)")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-levelize", action="store_true")
@@ -47,9 +50,10 @@ def parse():
 
 
 def cpu_baseline(forest_host, topo, budget_s):
-    """Oracle (CPU ot::Timer arithmetic, FP32) on a bounded prefix of the same
-    nets, over the host cores with a thread pool of 32-net chunks
-    (Timer::_build_rc_timing_tasks, timer.cpp:973-975)."""
+    """Oracle (CPU ot::Timer arithmetic, FP32 DFS + slew) over the host cores,
+    a thread pool over chunks of 32*256 nets (Timer::_build_rc_timing_tasks
+    runs 32-net chunks, timer.cpp:973-975).  The whole workload is solved
+    repeatedly until `budget_s` has passed; the median pass is reported."""
     import concurrent.futures as cf
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -58,25 +62,28 @@ def cpu_baseline(forest_host, topo, budget_s):
     ns = topo["nodes_start"]
     pid, pres, cap, si = topo["pid"], topo["pres"], topo["cap"], topo["slew_in"]
     cores = min(16, len(os.sched_getaffinity(0)))
+    nnets = len(ns) - 1
 
     def run(a, b):
         s, e = int(ns[a]), int(ns[b])
         sub_ns = (ns[a: b + 1] - s).astype(np.int32)
         O.rct_all(sub_ns, pid[s:e], pres[s:e], cap[4 * s: 4 * e], si[4 * a: 4 * b], "dfs_f32")
 
-    # calibrate on a small prefix, then size the sample to the budget
-    t0 = time.perf_counter()
-    run(0, 20000)
-    per_net = (time.perf_counter() - t0) / 20000
-    sample = int(min(len(ns) - 1, max(20000, budget_s * cores / max(per_net, 1e-9))))
     chunk = 32 * 256
-    t0 = time.perf_counter()
+    times = []
+    t_all = time.perf_counter()
     with cf.ThreadPoolExecutor(cores) as ex:
-        list(ex.map(lambda a: run(a, min(sample, a + chunk)), range(0, sample, chunk)))
-    dt = time.perf_counter() - t0
-    return {"value": sample / dt, "unit": "nets/s", "cores": cores, "kind": "port",
-            "sample": f"first {sample} nets of the same workload "
-                      f"({int(ns[sample])} nodes), oracle dfs_f32 + slew, {dt:.2f}s"}
+        while True:
+            t0 = time.perf_counter()
+            list(ex.map(lambda a: run(a, min(nnets, a + chunk)), range(0, nnets, chunk)))
+            times.append(time.perf_counter() - t0)
+            if time.perf_counter() - t_all >= budget_s or len(times) >= 50:
+                break
+    med = float(np.median(times))
+    return {"value": nnets / med, "unit": "nets/s", "cores": cores, "kind": "port",
+            "sample": f"the whole workload ({nnets} nets, {int(ns[-1])} nodes), "
+                      f"{len(times)} passes in {sum(times):.1f}s, median pass {med:.3f}s; "
+                      f"oracle FP32 DFS + slew, {cores} threads"}
 
 
 def run_levelize(ctx, stream, sh, reps=5):
@@ -180,15 +187,17 @@ def main():
                                          N.BATCHED if args.method == "batched" else
                                          N.BATCHED_COMPENSATED, args.tuning.encode(),
                                          C.byref(best)))
-        log(f"autotuned: ring {best.ring_slots} target {best.small_target} "
+        log(f"autotuned: ring {best.ring_slots} group target {best.small_target} "
             f"grid/cu {best.grid_per_cu}")
 
-    t0 = time.perf_counter()
-    s.plan(args.method)
-    torch.cuda.synchronize()
-    plan_s = time.perf_counter() - t0
-    stats = s.plan_stats(args.method)
-    log(f"plan {plan_s * 1e3:.1f} ms: {stats}")
+    other = "batched" if args.method == "compensated" else "compensated"
+    plans = {}
+    for m in (args.method, other):
+        t0 = time.perf_counter()
+        s.plan(m)
+        torch.cuda.synchronize()
+        plans[m] = {"build_ms": (time.perf_counter() - t0) * 1e3, "stats": s.plan_stats(m)}
+        log(f"plan {m}: {plans[m]['build_ms']:.1f} ms {plans[m]['stats']}")
     s.alloc_outputs()
 
     def timed(method, steps, warmup):
@@ -203,14 +212,14 @@ def main():
         torch.cuda.synchronize()
         return a.elapsed_time(b) / max(steps, 1)
 
-    result = {}
-    if args.only in ("", "rct"):
-        ms = timed(args.method, args.steps, args.warmup)
-        log(f"{args.method}: {ms:.3f} ms/step")
-        result["ms"] = ms
+    alg_bytes = 120.0 * V + 20.0 * Nn
+    ms = timed(args.method, args.steps, args.warmup)
+    log(f"{args.method}: {ms:.3f} ms/step")
     if args.only == "rct":
-        print(json.dumps({"only": "rct", "ms_per_step": result["ms"]}))
+        print(json.dumps({"only": "rct", "method": args.method, "ms_per_step": ms}))
         return
+    ms_other = timed(other, max(3, args.steps // 2), 1)
+    log(f"{other}: {ms_other:.3f} ms/step")
     plain_ms = None
     if not args.no_plain and args.only in ("", "plain"):
         plain_ms = timed("plain", args.plain_steps, 1)
@@ -219,48 +228,52 @@ def main():
         print(json.dumps({"only": "plain", "ms_per_step": plain_ms}))
         return
 
-    # ---- sample parity check of the timed outputs ----------------------------
+    # ---- sample parity of the timed outputs ------------------------------------
+    # FP32 batched vs the CPU timer arithmetic (FP32 DFS, 1e-5 on delay and
+    # slew); compensated vs FP64 (1e-6 on every output).
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    parity = None
-    topo = None
-    if args.only == "":
-        s.solve(args.method, stream=sh)
+    import oracle as O
+    topo = s.host_topology()
+    ns = topo["nodes_start"]
+    rng = np.random.default_rng(0)
+    pick = np.sort(rng.choice(Nn, min(Nn, 3000), replace=False))
+    sizes = np.diff(ns)
+    pick = np.unique(np.concatenate([pick, np.argsort(sizes)[-20:]]))  # the largest nets too
+    idx = np.concatenate([np.arange(4 * ns[a], 4 * ns[a + 1]) for a in pick])
+    sub_ns = np.concatenate([[0], np.cumsum(sizes[pick])]).astype(np.int32)
+    nodes = np.concatenate([np.arange(ns[a], ns[a + 1]) for a in pick])
+    nidx4 = np.concatenate([np.arange(4 * a, 4 * a + 4) for a in pick])
+    args_o = (sub_ns, topo["pid"][nodes], topo["pres"][nodes], topo["cap"][idx], topo["slew_in"][nidx4])
+    ref32 = O.rct_all(*args_o, "dfs_f32")
+    ref64 = O.rct_all(*args_o, "f64")
+    idx_t = torch.from_numpy(idx).to(dev)
+    parity = {"nets_checked": int(len(pick))}
+    for m in ("batched", "compensated"):
+        s.solve(m, stream=sh)
         torch.cuda.synchronize()
-        topo = s.host_topology()
-        import oracle as O
-        ns = topo["nodes_start"]
-        rng = np.random.default_rng(0)
-        pick = np.sort(rng.choice(Nn, min(Nn, 3000), replace=False))
-        # always include the largest nets
-        sizes = np.diff(ns)
-        pick = np.unique(np.concatenate([pick, np.argsort(sizes)[-20:]]))
         errs = {}
-        outs = {k: v for k, v in s.out.items()}
-        for k in ("delay", "slew"):
-            errs[k] = 0.0
-        for a in pick:
-            st, ed = int(ns[a]), int(ns[a + 1])
-            sub = np.array([0, ed - st], np.int32)
-            r = O.rct_all(sub, topo["pid"][st:ed], topo["pres"][st:ed], topo["cap"][4 * st: 4 * ed],
-                          topo["slew_in"][4 * a: 4 * a + 4], "dfs_f32")
-            for k in errs:
-                g = outs[k][4 * st: 4 * ed].cpu().numpy().astype(np.float64)
-                ref = r[k].astype(np.float64)
-                fl = max(np.max(np.abs(ref)) * 1e-7, 1e-30)
-                errs[k] = max(errs[k], float(np.max(np.abs(g - ref) / np.maximum(np.abs(ref), fl))))
-        parity = {"nets_checked": int(len(pick)), "max_rel_err_vs_cpu_fp32": errs}
-        log(f"parity sample: {parity}")
+        for k in ("load", "delay", "ldelay", "beta", "impulse", "slew"):
+            g = s.out[k][idx_t].cpu().numpy().astype(np.float64)
+            r64 = ref64[k].astype(np.float64)
+            scale = float(np.max(np.abs(r64))) if r64.size else 1.0
+            fl = max(scale * 1e-7, 1e-30)
+            if k == "impulse":
+                fl = max(float(np.max(np.abs(ref64["beta"]))) * 1e-6, 1e-30)
+            errs[k + "_vs_fp64"] = float(np.max(np.abs(g - r64) / np.maximum(np.abs(r64), fl)))
+            if k in ("delay", "slew"):
+                r32 = ref32[k].astype(np.float64)
+                errs[k + "_vs_cpu_fp32"] = float(np.max(np.abs(g - r32) / np.maximum(np.abs(r32), fl)))
+        parity[m] = errs
+    log(f"parity sample: {parity}")
 
     # ---- levelization ---------------------------------------------------------
     lev = None
     if not args.no_levelize:
         lev = run_levelize(ctx, stream, sh)
 
-    ms = result["ms"]
-    alg_bytes = 120.0 * V + 20.0 * Nn
     achieved = alg_bytes / (ms / 1e3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "rct_batched_cfg1_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"rct_{args.method}_cfg1_traffic.json")
     if os.path.exists(tfile):
         try:
             traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
@@ -268,7 +281,7 @@ def main():
             traffic = None
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(forest, topo if topo is not None else s.host_topology(), args.cpu_seconds)
+        cpu = cpu_baseline(forest, topo, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
     out = {
         "metric": METRIC,
@@ -283,18 +296,24 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{cfg.name} (heavy-tailed RC trees, BFS order, 4 corners)",
+        "config": {"workload": f"{cfg.name} (4M heavy-tailed RC trees, BFS order, 4 corners)",
                    "nets": Nn, "nodes": V, "method": args.method,
-                   "plan": {k: int(v) for k, v in stats.items()}},
+                   "precision": ("fp32 with two-sum compensated subtree/path sums, fp64 final "
+                                 "per-node combination") if args.method == "compensated" else "fp32",
+                   "plan": {k: int(v) for k, v in plans[args.method]["stats"].items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
+        other: {"ms_per_step": ms_other, "nets_per_s": Nn / (ms_other / 1e3),
+                "achieved_GBps": alg_bytes / (ms_other / 1e3) / 1e9,
+                "frac": alg_bytes / (ms_other / 1e3) / 1e9 / PEAK_HBM_GBS},
         "plain": None if plain_ms is None else {"ms_per_step": plain_ms,
                                                 "nets_per_s": Nn / (plain_ms / 1e3),
-                                                "batched_speedup": plain_ms / ms},
+                                                "speedup_of_" + args.method: plain_ms / ms,
+                                                "speedup_of_" + other: plain_ms / ms_other},
         "bfs_ordering_ms": bfs_ms,
-        "plan_build_ms": plan_s * 1e3,
+        "plan_build_ms": {m: plans[m]["build_ms"] for m in plans},
         "levelize": lev,
         "parity": parity,
     }

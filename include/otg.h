@@ -140,11 +140,15 @@ otg_status otg_rct_solve(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tre
                          const otg_rct_result* out, otg_rct_method method,
                          otg_stream stream);
 
-/* Re-solve only the nets listed in net_ids (device array of `count` net
- * indices, any order, no duplicates); other nets' outputs are untouched.
- * Used by incremental re-timing.  Capturable in a hipGraph (no allocation,
- * no memset, no host synchronisation).  No reference counterpart (the
- * reference CUDA path re-flattens all nets, ot/timer/timer.cpp:931). */
+/* Re-solve the nets listed in net_ids (device array of `count` net indices,
+ * any order; duplicates and out-of-range ids are ignored).  Small nets are
+ * solved in the packed groups of the plan, so the other nets of a listed
+ * net's group are re-solved too, from their current inputs (unchanged
+ * inputs give bit-identical outputs); all other nets' outputs are
+ * untouched.  Used by incremental re-timing.  Capturable in a hipGraph (no
+ * allocation, no memset, no host synchronisation); calls on one plan must
+ * not overlap.  No reference counterpart (the reference CUDA path
+ * re-flattens all nets, ot/timer/timer.cpp:931). */
 otg_status otg_rct_solve_subset(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tree* tree,
                                 const otg_rct_result* out, otg_rct_method method,
                                 const int32_t* net_ids, int32_t count, otg_stream stream);
@@ -202,6 +206,29 @@ typedef struct otg_rct_edge_array {
 
 otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_array* data,
                                    otg_rct_method method);
+
+/* Host-pointer drop-in for rct_compute_cuda(RctCUDA) (reference
+ * ot/cuda/rct.cuh:7-13,44, rct.cu:102-118): nets already in BFS order
+ * (nodes_start, pid, pres, cap as in RctCUDA), outputs load, delay, ldelay,
+ * impulse; beta, slew and slew_in are extensions (NULL skips them).
+ * Synchronous; allocates and frees device memory per call. */
+typedef struct otg_rct_host {
+  int32_t num_nets;
+  int32_t total_num_nodes;
+  const int32_t* rct_nodes_start; /* in  [N+1] */
+  const int32_t* rct_pid;         /* in  [V]   */
+  const float* pres;              /* in  [V]   */
+  const float* cap;               /* in  [4V]  */
+  float* load;                    /* out [4V]  */
+  float* delay;                   /* out [4V]  */
+  float* ldelay;                  /* out [4V]  */
+  float* impulse;                 /* out [4V]  */
+  float* beta;                    /* out [4V], optional */
+  float* slew;                    /* out [4V], optional */
+  const float* slew_in;           /* in  [4N], optional */
+} otg_rct_host;
+
+otg_status otg_rct_compute(otg_context* ctx, const otg_rct_host* data, otg_rct_method method);
 
 /* Autotuner: times candidate tunings of the batched launch on `tree`,
  * keeps the fastest in the context (used by later plans) and, when

@@ -269,6 +269,50 @@ extern "C" otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_arr
   return OTG_OK;
 }
 
+extern "C" otg_status otg_rct_compute(otg_context* ctx, const otg_rct_host* d,
+                                      otg_rct_method method) {
+  if (!ctx || !d || d->num_nets < 0 || d->total_num_nodes < 0) return OTG_ERR_INVALID_ARGUMENT;
+  if (d->num_nets == 0) return OTG_OK;
+  if (!d->rct_nodes_start || !d->rct_pid || !d->pres || !d->cap || !d->load || !d->delay ||
+      !d->ldelay || !d->impulse)
+    return OTG_ERR_INVALID_ARGUMENT;
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t N = d->num_nets, V = d->total_num_nodes;
+  DevBufs B;
+  int32_t *ns, *pid;
+  float *pres, *cap, *si = nullptr;
+  float *load, *delay, *ldelay, *beta, *imp, *slew = nullptr;
+  OTG_HIP(ctx, B.alloc(&ns, N + 1));
+  OTG_HIP(ctx, B.alloc(&pid, V));
+  OTG_HIP(ctx, B.alloc(&pres, V));
+  OTG_HIP(ctx, B.alloc(&cap, 4 * V));
+  OTG_HIP(ctx, B.alloc(&load, 4 * V));
+  OTG_HIP(ctx, B.alloc(&delay, 4 * V));
+  OTG_HIP(ctx, B.alloc(&ldelay, 4 * V));
+  OTG_HIP(ctx, B.alloc(&beta, 4 * V));
+  OTG_HIP(ctx, B.alloc(&imp, 4 * V));
+  if (d->slew) OTG_HIP(ctx, B.alloc(&slew, 4 * V));
+  if (d->slew_in) OTG_HIP(ctx, B.alloc(&si, 4 * N));
+  OTG_HIP(ctx, hipMemcpyAsync(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, s));
+  OTG_HIP(ctx, hipMemcpyAsync(pid, d->rct_pid, sizeof(int32_t) * V, hipMemcpyHostToDevice, s));
+  OTG_HIP(ctx, hipMemcpyAsync(pres, d->pres, sizeof(float) * V, hipMemcpyHostToDevice, s));
+  OTG_HIP(ctx, hipMemcpyAsync(cap, d->cap, sizeof(float) * 4 * V, hipMemcpyHostToDevice, s));
+  if (si) OTG_HIP(ctx, hipMemcpyAsync(si, d->slew_in, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s));
+  otg_rct_tree tree{d->num_nets, d->total_num_nodes, ns, pid, pres, cap, si};
+  otg_rct_result out{load, delay, ldelay, beta, imp, slew};
+  otg_status st = otg_rct_solve(ctx, nullptr, &tree, &out, method, s);
+  if (st != OTG_OK) return st;
+  OTG_HIP(ctx, hipMemcpyAsync(d->load, load, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(d->delay, delay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(d->ldelay, ldelay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipMemcpyAsync(d->impulse, imp, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  if (d->beta) OTG_HIP(ctx, hipMemcpyAsync(d->beta, beta, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  if (d->slew) OTG_HIP(ctx, hipMemcpyAsync(d->slew, slew, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipStreamSynchronize(s));
+  return OTG_OK;
+}
+
 // ---------------------------------------------------------------------------
 // P2
 // ---------------------------------------------------------------------------

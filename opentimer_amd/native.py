@@ -82,6 +82,13 @@ class RctEdgeArray(C.Structure):
                 ("beta", vp), ("slew", vp), ("slew_in", vp)]
 
 
+class RctHost(C.Structure):
+    _fields_ = [("num_nets", C.c_int32), ("total_num_nodes", C.c_int32),
+                ("rct_nodes_start", vp), ("rct_pid", vp), ("pres", vp), ("cap", vp),
+                ("load", vp), ("delay", vp), ("ldelay", vp), ("impulse", vp),
+                ("beta", vp), ("slew", vp), ("slew_in", vp)]
+
+
 class GenSizeClass(C.Structure):
     _fields_ = [("weight", C.c_double), ("min_nodes", C.c_int32), ("max_nodes", C.c_int32),
                 ("shape", C.c_int32), ("pad", C.c_int32), ("branch_prob", C.c_double)]
@@ -129,6 +136,7 @@ _sig("otg_rct_solve_subset", C.c_int, ctx_p, plan_p, C.POINTER(RctTree), C.POINT
      C.c_int, vp, C.c_int32, vp)
 _sig("otg_rct_bfs", C.c_int, ctx_p, C.POINTER(RctEdges), vp, vp, vp, vp, vp, vp)
 _sig("otg_rct_bfs_and_compute", C.c_int, ctx_p, C.POINTER(RctEdgeArray), C.c_int)
+_sig("otg_rct_compute", C.c_int, ctx_p, C.POINTER(RctHost), C.c_int)
 _sig("otg_rct_autotune", C.c_int, ctx_p, C.POINTER(RctTree), C.POINTER(RctResult), C.c_int,
      C.c_char_p, C.POINTER(RctTuning))
 _sig("otg_rct_tuning_load", C.c_int, ctx_p, C.c_char_p, C.POINTER(RctTuning))
@@ -147,6 +155,7 @@ EXPORTED = [
     "otg_create", "otg_destroy", "otg_synchronize", "otg_status_string", "otg_last_error",
     "otg_version", "otg_rct_plan_create", "otg_rct_plan_destroy", "otg_rct_plan_stats",
     "otg_rct_solve", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
+    "otg_rct_compute",
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize",
     "otg_gen_rc_sizes", "otg_gen_rc_fill", "otg_gen_dag", "otg_build_fanin_csr",
 ]

@@ -234,6 +234,27 @@ def test_bfs_and_compute_host_api(ctx):
         assert rel_err(outs[k], r64[k], fl[k]) < 1e-5, k
 
 
+@pytest.mark.parametrize("method", ["plain", "batched", "compensated"])
+def test_rct_compute_host_api(ctx, method):
+    """otg_rct_compute: host-pointer drop-in for rct_compute_cuda(RctCUDA)."""
+    import ctypes as C
+    f = W.generate_rc(W.config1(num_nets=400, seed=13))
+    e, n2b, pid, pres, cap = oracle_inputs(f)
+    V, Nn = f.num_nodes, f.num_nets
+    outs = {k: np.zeros(4 * V, np.float32) for k in OUT}
+    h = N.RctHost(num_nets=Nn, total_num_nodes=V, rct_nodes_start=N.addr(f.nodes_start),
+                  rct_pid=N.addr(pid), pres=N.addr(pres), cap=N.addr(cap),
+                  load=N.addr(outs["load"]), delay=N.addr(outs["delay"]),
+                  ldelay=N.addr(outs["ldelay"]), impulse=N.addr(outs["impulse"]),
+                  beta=N.addr(outs["beta"]), slew=N.addr(outs["slew"]), slew_in=N.addr(f.slew_in))
+    m = {"plain": N.PLAIN, "batched": N.BATCHED, "compensated": N.BATCHED_COMPENSATED}[method]
+    ctx.check(N.lib.otg_rct_compute(ctx.ptr, C.byref(h), m))
+    r64 = O.rct_all(f.nodes_start, pid, pres, cap, f.slew_in, "f64")
+    fl = floors(r64)
+    for k in OUT:
+        assert rel_err(outs[k], r64[k], fl[k]) < 1e-5, (method, k)
+
+
 def test_bad_topology_rejected(ctx):
     ns = np.array([0, 3], np.int32)
     pid = np.array([0, 1, 3], np.int32)  # node 2 points before the net start
