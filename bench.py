@@ -45,7 +45,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--tuning", default=os.path.join(ROOT, "opentimer_amd", "tuning.json"))
-    ap.add_argument("--only", default="", help="rct | plain | levelize (profiling runs)")
+    ap.add_argument("--only", default="",
+                    help="rct | plain | levelize | copy (profiling runs) | precision (cfg3 report)")
+    ap.add_argument("--precision-nets", type=int, default=20_000,
+                    help="nets of the cfg3 distribution for --only precision")
     return ap.parse_args()
 
 
@@ -118,6 +121,71 @@ def run_levelize(ctx, stream, sh, reps=5):
     return lev
 
 
+def run_precision(ctx, nets):
+    """cfg3 precision report: every output of the plain kernel (reference
+    arithmetic), the FP32 batched method and the compensated method against
+    the FP64 oracle over every node of `nets` nets of the cfg3 distribution
+    (50% chains of 1,000-20,000 nodes, 50% balanced trees of 64-1,024 nodes,
+    R 1e-3..1e4 ohm, C 1e-3..1e2 fF).  Relative errors use a floor of 1e-7 x
+    the output's largest magnitude (exact zeros at drivers); impulse is
+    compared against 1e-6 x the largest beta (it is a difference that can
+    vanish)."""
+    import concurrent.futures as cf
+
+    import torch
+
+    from opentimer_amd import workloads as W
+    from opentimer_amd.rct import RcSolver
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    cfg = W.config3(num_nets=nets)
+    t0 = time.perf_counter()
+    forest = W.generate_rc(cfg, nthreads=16)
+    log(f"generated {cfg.name} x{nets}: {forest.num_nodes} nodes in {time.perf_counter() - t0:.1f}s")
+    s = RcSolver.from_edges(forest, ctx)
+    del forest
+    topo = s.host_topology()
+    ns = topo["nodes_start"]
+    V, Nn = int(ns[-1]), len(ns) - 1
+    outs = ("load", "delay", "ldelay", "beta", "impulse", "slew")
+    ref = {k: np.empty(4 * V, np.float64) for k in outs}
+
+    def run(a, b):
+        st, ed = int(ns[a]), int(ns[b])
+        sub = (ns[a: b + 1] - st).astype(np.int32)
+        r = O.rct_all(sub, topo["pid"][st:ed], topo["pres"][st:ed], topo["cap"][4 * st: 4 * ed],
+                      topo["slew_in"][4 * a: 4 * b], "f64")
+        for k in outs:
+            ref[k][4 * st: 4 * ed] = r[k]
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(min(16, len(os.sched_getaffinity(0)))) as ex:
+        list(ex.map(lambda a: run(a, min(Nn, a + 64)), range(0, Nn, 64)))
+    log(f"fp64 oracle: {time.perf_counter() - t0:.1f}s")
+    floors = {k: max(float(np.max(np.abs(ref[k]))) * 1e-7, 1e-300) for k in outs}
+    floors["impulse"] = max(float(np.max(np.abs(ref["beta"]))) * 1e-6, 1e-300)
+    res = {"workload": f"{cfg.name}: {Nn} nets, {V} nodes (cfg3 distribution)", "nets": Nn,
+           "nodes": V, "reference": "FP64 oracle (CPU restatement of net.cpp:101-193), every node"}
+    for method in ("plain", "batched", "compensated"):
+        s.solve(method)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        s.solve(method)
+        b.record()
+        torch.cuda.synchronize()
+        errs = {}
+        for k in outs:
+            g = s.out[k].cpu().numpy().astype(np.float64)
+            e = np.abs(g - ref[k]) / np.maximum(np.abs(ref[k]), floors[k])
+            errs[k] = {"max": float(e.max()), "mean": float(e.mean())}
+            del g, e
+        res[method] = {"ms": a.elapsed_time(b), "rel_err_vs_fp64": errs}
+        log(f"precision {method}: {res[method]}")
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -141,6 +209,9 @@ def main():
 
     if args.only == "levelize":
         print(json.dumps(run_levelize(ctx, stream, sh)))
+        return
+    if args.only == "precision":
+        print(json.dumps(run_precision(ctx, args.precision_nets)))
         return
     if args.only == "copy":
         # a plain device copy of known size: the yardstick that converts the
