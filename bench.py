@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--tuning", default=os.path.join(ROOT, "opentimer_amd", "tuning.json"))
     ap.add_argument("--only", default="",
-                    help="rct | plain | levelize | copy (profiling runs) | precision (cfg3 report)")
+                    help="rct | plain | levelize | copy (profiling runs) | precision (cfg3 report) "
+                         "| incremental (cfg4 report)")
     ap.add_argument("--precision-nets", type=int, default=20_000,
                     help="nets of the cfg3 distribution for --only precision")
     return ap.parse_args()
@@ -186,6 +187,119 @@ def run_precision(ctx, nets):
     return res
 
 
+def _closure(fo_start, fo, seeds, n):
+    """Forward closure over a CSR (vectorised frontier expansion)."""
+    mark = np.zeros(n, bool)
+    fr = np.unique(seeds)
+    mark[fr] = True
+    while len(fr):
+        st, ln = fo_start[fr], fo_start[fr + 1] - fo_start[fr]
+        tot = int(ln.sum())
+        if tot == 0:
+            break
+        off = np.repeat(st - (np.cumsum(ln) - ln), ln) + np.arange(tot)
+        nxt = np.unique(fo[off])
+        nxt = nxt[~mark[nxt]]
+        mark[nxt] = True
+        fr = nxt
+    return np.flatnonzero(mark)
+
+
+def run_incremental(ctx, method, pins=5_000_000, nets=1_000_000):
+    """cfg4: a 5M-pin timing graph (cfg2's generator scaled: 50k pins per
+    layer, 2.5 arcs per pin) and 1M RC nets (cfg1 size distribution) resident
+    on the device.  For batches of 1, 100, 10k and 100k changed nets: R and C
+    of the changed nets are perturbed by U(0.8, 1.2) (before the timed
+    region), then one hipGraph launch re-solves them and re-levelizes the
+    fan-out cone of their driver pins.  Reports p50/p99 per batch size."""
+    import torch
+
+    from opentimer_amd import workloads as W
+    from opentimer_amd.incremental import Incremental
+    from opentimer_amd.levelize import Levelizer
+    from opentimer_amd.rct import RcSolver
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    dcfg = W.DagConfig("cfg4-dag-5M", pins, int(2.5 * pins), max(2, pins // 50_000), seed=9)
+    dag = W.generate_dag(dcfg, nthreads=16)
+    fin = W.fanin_csr(dag)
+    fout = W.fanin_csr(W.Dag(dag.n, dag.arc_to, dag.arc_from))
+    L = Levelizer(fin, ctx)
+    L.run()
+    _, ends, lv = L.host()
+    nlev = len(ends) - 1
+    forest = W.generate_rc(W.config1(num_nets=nets), nthreads=16)
+    s = RcSolver.from_edges(forest, ctx)
+    ns = forest.nodes_start
+    del forest
+    rng = np.random.default_rng(11)
+    drivers = rng.choice(np.flatnonzero(fin.out_degree > 0), nets).astype(np.int32)
+    inc = Incremental(s, fin, fout, drivers, lv, nlev)
+    s.plan(method)
+    s.alloc_outputs()
+    dev = s.device
+    side = torch.cuda.Stream(dev)
+    # full re-timing for comparison: one full solve + one full levelization
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.solve(method)
+    L.run()
+    torch.cuda.synchronize()
+    full_ms = (time.perf_counter() - t0) * 1e3
+    res = {"workload": f"{dcfg.name}: {dag.n} pins, {len(dag.arc_from)} arcs, {nlev} levels; "
+                       f"{nets} RC nets (cfg1 sizes), {int(ns[-1])} nodes",
+           "method": method, "full_retime_ms": full_ms, "batches": {}}
+
+    def perturb(ids):
+        lens = ns[ids + 1] - ns[ids]
+        idx = np.repeat(ns[ids] - (np.cumsum(lens) - lens), lens) + np.arange(int(lens.sum()))
+        f = torch.from_numpy(rng.uniform(0.8, 1.2, len(idx)).astype(np.float32)).to(dev)
+        it = torch.from_numpy(idx.astype(np.int64)).to(dev)
+        s.pres[it] *= f
+        s.cap.view(-1, 4)[it] *= f[:, None]
+
+    for B in (1, 100, 10_000, 100_000):
+        buf = torch.zeros(B, dtype=torch.int32, device=dev)
+        g = inc.capture(buf, method, side)
+        rounds = 50 if B <= 100 else (30 if B <= 10_000 else 20)
+        times, cones, levels = [], [], []
+        verified = None
+        for r in range(rounds):
+            ids = np.sort(rng.choice(nets, B, replace=False)).astype(np.int32)
+            buf.copy_(torch.from_numpy(ids))
+            perturb(ids)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(side)
+            g.launch(side)
+            e1.record(side)
+            side.synchronize()
+            times.append(e0.elapsed_time(e1))
+            cone, fr, kends = inc.query(side.cuda_stream)
+            cones.append(len(cone))
+            levels.append(len(kends) - 1)
+            if r == 0 and B <= 100:
+                ref = _closure(fout.start, fout.fanin, drivers[ids], dag.n)
+                cand = np.zeros(dag.n, np.uint8)
+                cand[ref] = 1
+                csr = W.fanin_csr(dag, cand=cand)
+                _, oends, olv = O.levelize(csr.n, csr.start, csr.fanin, csr.out_degree, csr.frontier)
+                lvg = inc.level.cpu().numpy()
+                verified = bool(np.array_equal(np.sort(cone), ref) and np.array_equal(kends, oends)
+                                and np.array_equal(lvg[ref], olv[ref]))
+        g.close()
+        t = np.array(times)
+        res["batches"][str(B)] = {
+            "rounds": rounds, "p50_ms": float(np.percentile(t, 50)), "p99_ms": float(np.percentile(t, 99)),
+            "mean_ms": float(t.mean()), "nets_per_s": float(B / (t.mean() / 1e3)),
+            "mean_cone_pins": float(np.mean(cones)), "cone_pins_per_s": float(np.mean(cones) / (t.mean() / 1e3)),
+            "mean_cone_levels": float(np.mean(levels)), "cone_and_levels_match_oracle": verified}
+        log(f"incremental B={B}: {res['batches'][str(B)]}")
+    inc.close()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -212,6 +326,9 @@ def main():
         return
     if args.only == "precision":
         print(json.dumps(run_precision(ctx, args.precision_nets)))
+        return
+    if args.only == "incremental":
+        print(json.dumps(run_incremental(ctx, args.method)))
         return
     if args.only == "copy":
         # a plain device copy of known size: the yardstick that converts the

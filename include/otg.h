@@ -270,6 +270,61 @@ otg_status otg_levelize(otg_context* ctx, int32_t n, int32_t num_edges,
                         int32_t first_size, int32_t* frontiers, int32_t* frontiers_ends,
                         int32_t* level_of_pin, int32_t* num_ends, otg_stream stream);
 
+/* ------------------------------------------------------------------------ */
+/* Incremental re-levelization and hipGraph capture                          */
+/* ------------------------------------------------------------------------ */
+
+/* The timing graph of an incremental session, device pointers (kept by the
+ * caller for the session's lifetime).  level[num_pins] holds each pin's
+ * level; an update rewrites the levels of the pins of its cone. */
+typedef struct otg_incr_graph {
+  int32_t num_pins;
+  int32_t num_nets;
+  const int32_t* fanout_start; /* [num_pins+1] fanout CSR                      */
+  const int32_t* fanout;
+  const int32_t* fanin_start;  /* [num_pins+1] fanin CSR                       */
+  const int32_t* fanin;
+  const int32_t* out_degree;   /* [num_pins] fanout count                      */
+  const int32_t* net_driver;   /* [num_nets] driver pin of each RC net         */
+  int32_t* level;              /* [num_pins] in/out                            */
+} otg_incr_graph;
+
+typedef struct otg_incr otg_incr;
+typedef struct otg_graph otg_graph;
+
+/* Allocates the session's work buffers (no allocation happens later).
+ * max_levels bounds the depth of any cone (the graph's number of levels).
+ * No reference counterpart: the reference rebuilds the candidate set on the
+ * host for every update (Timer::_build_prop_cands, ot/timer/timer.cpp:845-923). */
+otg_status otg_incr_create(otg_context* ctx, const otg_incr_graph* g, int32_t max_levels,
+                           otg_incr** incr);
+otg_status otg_incr_destroy(otg_incr* incr);
+
+/* Re-levelizes the fan-out cone of the driver pins of the nets in net_ids
+ * (device, `count` entries): the cone is the forward closure over fanout
+ * arcs (the FPROP_CAND set, timer.cpp:845-923) and its pins are levelized
+ * sink-first among themselves (toposort semantics of toposort.cu:4-72 on the
+ * candidate subgraph, timer.cpp:1011-1071).  A fixed sequence of
+ * 2*max_levels+7 kernels with no host round trip: capturable in a hipGraph.
+ * Asynchronous on `stream`. */
+otg_status otg_incr_relevelize(otg_incr* incr, const int32_t* net_ids, int32_t count,
+                               otg_stream stream);
+
+/* Reads the result of the last update (synchronises `stream`): cone size,
+ * number of levels, and device pointers to the cone pins, the cone pins by
+ * level and the level ends (num_levels+1 entries, starting 0). */
+otg_status otg_incr_query(otg_incr* incr, otg_stream stream, int32_t* cone_size,
+                          int32_t* num_levels, const int32_t** cone, const int32_t** frontiers,
+                          const int32_t** frontier_ends);
+
+/* hipGraph capture of any sequence of asynchronous calls on a non-default
+ * stream (for example otg_rct_solve_subset + otg_incr_relevelize), then
+ * replay.  No reference counterpart. */
+otg_status otg_graph_begin(otg_context* ctx, otg_stream stream);
+otg_status otg_graph_end(otg_context* ctx, otg_stream stream, otg_graph** graph);
+otg_status otg_graph_launch(otg_context* ctx, otg_graph* graph, otg_stream stream);
+otg_status otg_graph_destroy(otg_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,15 +31,22 @@ constexpr int kBatch = 32;  // level kernels per host check
 struct LevelizeParams {
   int32_t n;
   int32_t first_size;
-  const int32_t* fanin_start;
+  const int32_t* fanin_start;  // CSR walked from each frontier pin (fanin; fanout for reach)
   const int32_t* fanin;
-  int32_t* out;        // consumed fanout counts (device copy)
+  int32_t* out;        // Kahn: consumed fanout counts (device copy) | reach: visited flags
   int32_t* frontiers;  // [n], frontier 0 given
-  int32_t* ends;       // [n+2]
+  int32_t* ends;       // [levels+2]
   int32_t* level;      // [n] or null
-  int32_t* cnt;        // [n+2] appends made while processing level l
+  int32_t* cnt;        // [levels+2] appends made while processing level l
   int32_t* num_ends;   // [1], INT_MAX until the first empty frontier
+  const int32_t* first_dev;  // size of frontier 0 on the device (overrides first_size), or null
+  const int32_t* cand;       // Kahn: arcs from pins with cand == 0 are skipped; null = all
 };
+
+// Kahn: a fanin source joins the next frontier when its last fanout is
+// consumed.  Reach: a fanout joins the next frontier the first time it is
+// visited (forward closure of frontier 0, the fan-out cone).
+enum : int { LV_KAHN = 0, LV_REACH = 1 };
 
 __global__ __launch_bounds__(kBlock) void levelize_init_kernel(LevelizeParams P) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -79,10 +86,11 @@ __device__ __forceinline__ int owner_of(int t, int off_mine, int lane_count) {
   return lo;
 }
 
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void levelize_level_kernel(LevelizeParams P, int lv) {
   if (lv > P.n) return;
   const int b = P.ends[lv];
-  const int e = lv == 0 ? P.first_size : b + P.cnt[lv - 1];
+  const int e = lv == 0 ? (P.first_dev ? *P.first_dev : P.first_size) : b + P.cnt[lv - 1];
   if (e == b) {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicMin(P.num_ends, lv < 1 ? 2 : lv + 1);
     return;
@@ -122,8 +130,16 @@ __global__ __launch_bounds__(kBlock) void levelize_level_kernel(LevelizeParams P
         const int oj = __shfl(js, o), oo = __shfl(off, o);
         v[k] = ok ? P.fanin[oj + (tt - oo)] : -1;
       }
+      if (MODE == LV_KAHN && P.cand) {
 #pragma unroll
-      for (int k = 0; k < kArcs; ++k) old[k] = v[k] >= 0 ? atomicSub(&P.out[v[k]], 1) : 0;
+        for (int k = 0; k < kArcs; ++k)
+          if (v[k] >= 0 && !P.cand[v[k]]) v[k] = -1;
+      }
+#pragma unroll
+      for (int k = 0; k < kArcs; ++k) {
+        if (MODE == LV_KAHN) old[k] = v[k] >= 0 ? atomicSub(&P.out[v[k]], 1) : 0;
+        else old[k] = v[k] >= 0 ? (atomicExch(&P.out[v[k]], 1) == 0 ? 1 : 0) : 0;
+      }
       uint64_t m[kArcs];
       int tot = 0;
 #pragma unroll
@@ -148,6 +164,91 @@ __global__ __launch_bounds__(kBlock) void levelize_level_kernel(LevelizeParams P
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Incremental re-levelization of the fan-out cone of changed nets.
+//
+// The cone of a set of frontier pins is their forward closure over fanout
+// arcs (the FPROP_CAND set of Timer::_build_fprop_cands, reference
+// ot/timer/timer.cpp:845-923); only it is re-levelized, with the sink-first
+// Kahn sweep restricted to cone pins (the candidate filter of
+// Timer::_build_prop_tasks_cuda, timer.cpp:1011-1047).  A cone is forward
+// closed, so a cone pin's fanout count within the cone is its full fanout
+// count.  Every launch has a fixed grid and the number of level launches is
+// fixed by the caller's bound on the graph depth, so one update is a fixed
+// sequence of kernels with no host round trip (capturable in a hipGraph).
+// ---------------------------------------------------------------------------
+struct IncrState {
+  int32_t n;
+  int32_t num_nets;
+  int32_t max_levels;
+  const int32_t* net_driver;  // [num_nets] driver pin of each net
+  int32_t* flag;              // [n] cone membership, all zero between updates
+  int32_t* cone;              // [n] cone pins, frontier by frontier
+  int32_t* cone_ends;         // [max_levels+3]
+  int32_t* cone_cnt;          // [max_levels+3]
+  int32_t* kout;              // [n] fanout counts consumed by the Kahn sweep
+  int32_t* kfront;            // [n] cone pins by level
+  int32_t* kends;             // [max_levels+3]
+  int32_t* kcnt;              // [max_levels+3]
+  int32_t* counters;          // [0] seeds [1] sinks [2] cone num_ends [3] kahn num_ends
+};
+
+__global__ __launch_bounds__(kBlock) void incr_reset_kernel(IncrState S) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = t; k < S.max_levels + 3; k += gridDim.x * blockDim.x) {
+    S.cone_cnt[k] = 0;
+    S.kcnt[k] = 0;
+    S.cone_ends[k] = 0;
+    S.kends[k] = 0;
+  }
+  if (t == 0) {
+    S.counters[0] = 0;
+    S.counters[1] = 0;
+    S.counters[2] = INT32_MAX;
+    S.counters[3] = INT32_MAX;
+  }
+}
+
+// Frontier 0 of the cone: the driver pins of the changed nets, each once.
+__global__ __launch_bounds__(kBlock) void incr_seed_kernel(IncrState S, const int32_t* net_ids,
+                                                           int32_t count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int net = net_ids[j];
+  if (net < 0 || net >= S.num_nets) return;
+  const int pin = S.net_driver[net];
+  if (pin < 0 || pin >= S.n) return;
+  if (atomicExch(&S.flag[pin], 1) == 0) S.cone[atomicAdd(&S.counters[0], 1)] = pin;
+}
+
+__device__ __forceinline__ int incr_cone_size(const IncrState& S) {
+  const int ne = S.counters[2];
+  const int last = ne == INT32_MAX ? S.max_levels + 1 : ne - 1;
+  return S.cone_ends[last < S.max_levels + 2 ? last : S.max_levels + 2];
+}
+
+// Cone pins: Kahn fanout counts, and frontier 0 of the Kahn sweep (cone pins
+// without fanout).
+__global__ __launch_bounds__(kBlock) void incr_kahn_init_kernel(IncrState S,
+                                                                const int32_t* out_degree,
+                                                                int32_t* level) {
+  const int size = incr_cone_size(S);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < size; k += gridDim.x * blockDim.x) {
+    const int u = S.cone[k];
+    const int d = out_degree[u];
+    S.kout[u] = d;
+    if (level) level[u] = d == 0 ? 0 : -1;
+    if (d == 0) S.kfront[atomicAdd(&S.counters[1], 1)] = u;
+  }
+}
+
+// Clears the cone flags for the next update.
+__global__ __launch_bounds__(kBlock) void incr_cleanup_kernel(IncrState S) {
+  const int size = incr_cone_size(S);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < size; k += gridDim.x * blockDim.x)
+    S.flag[S.cone[k]] = 0;
+}
 }  // namespace
 
 // Launches the whole sweep on `s` and returns the number of ends entries in
@@ -159,7 +260,7 @@ otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_si
                                    hipStream_t s) {
   // scratch: [n+2] per-level append counters followed by num_ends
   LevelizeParams P{n, first_size, fanin_start, fanin, out_copy, frontiers, ends, level,
-                   scratch, scratch + n + 2};
+                   scratch, scratch + n + 2, nullptr, nullptr};
   const int init_grid = (int)std::min<int64_t>(4096, ((int64_t)n + 2 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(levelize_init_kernel, dim3(std::max(1, init_grid)), dim3(kBlock), 0, s, P);
   OTG_HIP(ctx, hipGetLastError());
@@ -174,7 +275,7 @@ otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_si
   int32_t ne = INT32_MAX;
   for (int lv0 = 0; lv0 <= n; lv0 += kBatch) {
     for (int lv = lv0; lv < lv0 + kBatch && lv <= n; ++lv)
-      hipLaunchKernelGGL(levelize_level_kernel, dim3(grid), dim3(kBlock), 0, s, P, lv);
+      hipLaunchKernelGGL(levelize_level_kernel<LV_KAHN>, dim3(grid), dim3(kBlock), 0, s, P, lv);
     OTG_HIP(ctx, hipGetLastError());
     OTG_HIP(ctx, hipMemcpyAsync(&ne, P.num_ends, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     OTG_HIP(ctx, hipStreamSynchronize(s));
@@ -185,5 +286,171 @@ otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_si
     return OTG_ERR_CYCLE;
   }
   *num_ends_host = ne;
+  return OTG_OK;
+}
+
+struct otg_incr {
+  otg_context* ctx = nullptr;
+  IncrState S{};
+  const int32_t* fo_start = nullptr;
+  const int32_t* fo = nullptr;
+  const int32_t* fi_start = nullptr;
+  const int32_t* fi = nullptr;
+  const int32_t* out_degree = nullptr;
+  int32_t* level = nullptr;
+  int grid = 1;
+};
+
+extern "C" otg_status otg_incr_create(otg_context* ctx, const otg_incr_graph* g, int32_t max_levels,
+                                      otg_incr** out) {
+  if (!ctx || !g || !out || g->num_pins < 0 || g->num_nets < 0 || max_levels < 0)
+    return OTG_ERR_INVALID_ARGUMENT;
+  if (g->num_pins > 0 && (!g->fanout_start || !g->fanout || !g->fanin_start || !g->fanin ||
+                          !g->out_degree || !g->level))
+    return OTG_ERR_INVALID_ARGUMENT;
+  if (g->num_nets > 0 && !g->net_driver) return OTG_ERR_INVALID_ARGUMENT;
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  otg_incr* I = new otg_incr();
+  I->ctx = ctx;
+  I->fo_start = g->fanout_start;
+  I->fo = g->fanout;
+  I->fi_start = g->fanin_start;
+  I->fi = g->fanin;
+  I->out_degree = g->out_degree;
+  I->level = g->level;
+  IncrState& S = I->S;
+  S.n = g->num_pins;
+  S.num_nets = g->num_nets;
+  S.max_levels = max_levels;
+  S.net_driver = g->net_driver;
+  const int64_t n = std::max<int64_t>(1, g->num_pins), L = (int64_t)max_levels + 3;
+  auto fail = [&](hipError_t e) {
+    otg_incr_destroy(I);
+    otg_set_error(ctx, "otg_incr_create: %s", hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? OTG_ERR_OUT_OF_MEMORY : OTG_ERR_DEVICE;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&S.flag, sizeof(int32_t) * n)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&S.cone, sizeof(int32_t) * n)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&S.kout, sizeof(int32_t) * n)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&S.kfront, sizeof(int32_t) * n)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&S.cone_ends, sizeof(int32_t) * L)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&S.cone_cnt, sizeof(int32_t) * L)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&S.kends, sizeof(int32_t) * L)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&S.kcnt, sizeof(int32_t) * L)) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&S.counters, sizeof(int32_t) * 8)) != hipSuccess) return fail(e);
+  if ((e = hipMemset(S.flag, 0, sizeof(int32_t) * n)) != hipSuccess) return fail(e);
+  if ((e = hipMemset(S.counters, 0, sizeof(int32_t) * 8)) != hipSuccess) return fail(e);
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e);
+  I->grid = std::max(1, std::min(4 * std::max(1, ctx->num_cus),
+                                 (int)((g->num_pins + kBlock - 1) / kBlock)));
+  *out = I;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_incr_destroy(otg_incr* I) {
+  if (!I) return OTG_OK;
+  IncrState& S = I->S;
+  for (int32_t* p : {S.flag, S.cone, S.kout, S.kfront, S.cone_ends, S.cone_cnt, S.kends, S.kcnt,
+                     S.counters})
+    if (p) hipFree(p);
+  delete I;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_incr_relevelize(otg_incr* I, const int32_t* net_ids, int32_t count,
+                                          otg_stream stream) {
+  if (!I || count < 0 || (count > 0 && !net_ids)) return OTG_ERR_INVALID_ARGUMENT;
+  otg_context* ctx = I->ctx;
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  const IncrState& S = I->S;
+  if (S.n == 0) return OTG_OK;
+  const int L = S.max_levels + 1;  // frontiers 0..max_levels, then one empty
+  hipLaunchKernelGGL(incr_reset_kernel, dim3(1), dim3(kBlock), 0, s, S);
+  if (count > 0)
+    hipLaunchKernelGGL(incr_seed_kernel, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S,
+                       net_ids, count);
+  // forward closure: frontiers of the cone over the fanout CSR
+  LevelizeParams R{S.n, 0, I->fo_start, I->fo, S.flag, S.cone, S.cone_ends, nullptr,
+                   S.cone_cnt, S.counters + 2, S.counters + 0, nullptr};
+  for (int lv = 0; lv <= L; ++lv)
+    hipLaunchKernelGGL(levelize_level_kernel<LV_REACH>, dim3(I->grid), dim3(kBlock), 0, s, R, lv);
+  // sink-first Kahn sweep over the cone
+  hipLaunchKernelGGL(incr_kahn_init_kernel, dim3(I->grid), dim3(kBlock), 0, s, S, I->out_degree,
+                     I->level);
+  LevelizeParams K{S.n, 0, I->fi_start, I->fi, S.kout, S.kfront, S.kends, I->level,
+                   S.kcnt, S.counters + 3, S.counters + 1, S.flag};
+  for (int lv = 0; lv <= L; ++lv)
+    hipLaunchKernelGGL(levelize_level_kernel<LV_KAHN>, dim3(I->grid), dim3(kBlock), 0, s, K, lv);
+  hipLaunchKernelGGL(incr_cleanup_kernel, dim3(I->grid), dim3(kBlock), 0, s, S);
+  OTG_HIP(ctx, hipGetLastError());
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_incr_query(otg_incr* I, otg_stream stream, int32_t* cone_size,
+                                     int32_t* num_levels, const int32_t** cone,
+                                     const int32_t** frontiers, const int32_t** frontier_ends) {
+  if (!I) return OTG_ERR_INVALID_ARGUMENT;
+  otg_context* ctx = I->ctx;
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  const IncrState& S = I->S;
+  int32_t c[4] = {0, 0, 0, 0};
+  OTG_HIP(ctx, hipMemcpyAsync(c, S.counters, sizeof(c), hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipStreamSynchronize(s));
+  if (c[2] == INT32_MAX || c[3] == INT32_MAX) {
+    OTG_SET_ERR(ctx, "cone deeper than max_levels (or a cycle in the cone)");
+    return OTG_ERR_CYCLE;
+  }
+  int32_t csize = 0;
+  OTG_HIP(ctx, hipMemcpyAsync(&csize, S.cone_ends + (c[2] - 1), sizeof(int32_t),
+                              hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipStreamSynchronize(s));
+  if (cone_size) *cone_size = csize;
+  if (num_levels) *num_levels = c[3] - 1;  // frontier_ends has num_levels+1 entries
+  if (cone) *cone = S.cone;
+  if (frontiers) *frontiers = S.kfront;
+  if (frontier_ends) *frontier_ends = S.kends;
+  return OTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hipGraph capture helpers
+// ---------------------------------------------------------------------------
+struct otg_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" otg_status otg_graph_begin(otg_context* ctx, otg_stream stream) {
+  if (!ctx || !stream) return OTG_ERR_INVALID_ARGUMENT;  // the default stream cannot be captured
+  OTG_HIP(ctx, hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_graph_end(otg_context* ctx, otg_stream stream, otg_graph** out) {
+  if (!ctx || !stream || !out) return OTG_ERR_INVALID_ARGUMENT;
+  otg_graph* g = new otg_graph();
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    otg_graph_destroy(g);
+    otg_set_error(ctx, "graph capture: %s", hipGetErrorString(e));
+    return OTG_ERR_DEVICE;
+  }
+  *out = g;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_graph_launch(otg_context* ctx, otg_graph* g, otg_stream stream) {
+  if (!ctx || !g) return OTG_ERR_INVALID_ARGUMENT;
+  OTG_HIP(ctx, hipGraphLaunch(g->exec, otg_pick_stream(ctx, stream)));
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_graph_destroy(otg_graph* g) {
+  if (!g) return OTG_OK;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  delete g;
   return OTG_OK;
 }

@@ -89,6 +89,12 @@ class RctHost(C.Structure):
                 ("beta", vp), ("slew", vp), ("slew_in", vp)]
 
 
+class IncrGraph(C.Structure):
+    _fields_ = [("num_pins", C.c_int32), ("num_nets", C.c_int32),
+                ("fanout_start", vp), ("fanout", vp), ("fanin_start", vp), ("fanin", vp),
+                ("out_degree", vp), ("net_driver", vp), ("level", vp)]
+
+
 class GenSizeClass(C.Structure):
     _fields_ = [("weight", C.c_double), ("min_nodes", C.c_int32), ("max_nodes", C.c_int32),
                 ("shape", C.c_int32), ("pad", C.c_int32), ("branch_prob", C.c_double)]
@@ -144,6 +150,15 @@ _sig("otg_toposort_compute", C.c_int, ctx_p, C.c_int32, C.c_int32, C.c_int32, vp
      i32p, vp)
 _sig("otg_levelize", C.c_int, ctx_p, C.c_int32, C.c_int32, vp, vp, vp, vp, C.c_int32, vp, vp, vp,
      i32p, vp)
+_sig("otg_incr_create", C.c_int, ctx_p, C.POINTER(IncrGraph), C.c_int32, C.POINTER(C.c_void_p))
+_sig("otg_incr_destroy", C.c_int, C.c_void_p)
+_sig("otg_incr_relevelize", C.c_int, C.c_void_p, vp, C.c_int32, vp)
+_sig("otg_incr_query", C.c_int, C.c_void_p, vp, i32p, i32p, C.POINTER(C.c_void_p),
+     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p))
+_sig("otg_graph_begin", C.c_int, ctx_p, vp)
+_sig("otg_graph_end", C.c_int, ctx_p, vp, C.POINTER(C.c_void_p))
+_sig("otg_graph_launch", C.c_int, ctx_p, C.c_void_p, vp)
+_sig("otg_graph_destroy", C.c_int, C.c_void_p)
 _sig("otg_gen_rc_sizes", C.c_int, C.POINTER(GenRcParams), vp, i64p)
 _sig("otg_gen_rc_fill", C.c_int, C.POINTER(GenRcParams), vp, vp, vp, vp, vp, vp, vp, C.c_int)
 _sig("otg_gen_dag", C.c_int, C.POINTER(GenDagParams), vp, vp, C.c_int)
@@ -157,6 +172,8 @@ EXPORTED = [
     "otg_rct_solve", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
     "otg_rct_compute",
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize",
+    "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",
+    "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
     "otg_gen_rc_sizes", "otg_gen_rc_fill", "otg_gen_dag", "otg_build_fanin_csr",
 ]
 
