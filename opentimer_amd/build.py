@@ -25,6 +25,10 @@ HIP_FLAGS = [
     # float2/float4 arithmetic)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # keep every atomic as written: the wave-aggregating rewrite consumes the
+    # returned value at once, which forces a wait for all outstanding memory
+    # operations where the kernels issue an atomic ahead of its use
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
     f"-I{INCLUDE}",
 ]

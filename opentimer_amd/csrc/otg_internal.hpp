@@ -54,7 +54,8 @@ struct otg_rct_plan {
   RctItem* d_gitems = nullptr;     // global-window nets
   RctItem* d_sub_items = nullptr;  // [num_nets] items of a subset solve
   int32_t* d_queue = nullptr;      // next item and finished workgroups of the streaming
-                                   // [0,1], global-window [2,3] and group [4,5] kernels
+                                   // [0,1] and global-window [2,3] kernels; [6]: streamed
+                                   // nets of a subset solve
   float* d_scratch = nullptr;      // window storage of stream/global items
   int64_t scratch_floats = 0;
   int32_t* d_net_maxpid = nullptr;  // per net, for subset solves

@@ -860,11 +860,14 @@ __device__ __forceinline__ void load_chunk(const RctParams& P, const Item& it, i
       ch.y[s] = bload4_l2(ry, q * 16);
     } else {
       ch.x[s] = bload4(rx, q * 16);
-      ch.y[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      // y is not used in this phase and left as it is: a value written here
+      // could land in a register that a load of the other phase still has
+      // in flight, and would wait for that load
     }
   }
-  ch.si = PH == PH_TD ? bload4(make_rsrc(P.slew_in, it.nb, 16, 1), 0)
-                      : make_float4(0.f, 0.f, 0.f, 0.f);
+  // (read in both phases: chunk 0 of the bottom-up sweep becomes the
+  // top-down sweep's first chunk in registers)
+  ch.si = bload4(make_rsrc(P.slew_in, it.nb, 16, 1), 0);
 }
 
 __device__ __forceinline__ void load_chunk(const RctParams& P, const Item& it, int ph, int c,
@@ -967,7 +970,7 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if constexpr (COMP) {
-          const ff tt = ff_add(two_sum(cap4[c], xa[c]), ff{xal[c], 0.f});
+          const ff tt = ff_add_f(ff{xa[c], xal[c]}, cap4[c]);
           ld[c] = tt.hi;
           ldl[c] = tt.lo;
           const ff q = ff_add(ff{xb[c], xbl[c]}, ff_mul_f(ff_mul(tt, tt), r));
@@ -1052,29 +1055,54 @@ __device__ __forceinline__ void node_outputs(const float (&d)[4], const float (&
   }
 }
 
-// Top-down increments of one node: delay += R load,
-// impulse += (R load)^2 + 2 R L (double-float in the compensated mode).
+// Top-down increments of one node: u = R load (added to the delay) and
+// z = (R load)^2 + 2 R L (added to the impulse); double-float in the
+// compensated mode.  They depend on the node alone, so a window computes
+// them once for all its lanes before its depth steps.
+struct TdIncr {
+  float u[4], ul[4], z[4], zl[4];
+};
+
 template <bool COMP>
-__device__ __forceinline__ void td_node(float rr, const float (&ld)[4], const float (&L)[4],
-                                        const float (&pd)[4], const float (&pdl)[4],
-                                        const float (&pm)[4], const float (&pml)[4],
-                                        float (&d)[4], float (&dl)[4], float (&m)[4],
-                                        float (&ml)[4]) {
+__device__ __forceinline__ TdIncr td_incr(float rr, const float (&ld)[4], const float (&L)[4]) {
+  TdIncr t;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     if constexpr (COMP) {
       const ff uu = ff_mul_f(ff{ld[c], 0.f}, rr);
       const ff zz = ff_add(ff_mul(uu, uu), ff_mul_f(ff{L[c], 0.f}, 2.0f * rr));
-      const ff dd = ff_add(ff{pd[c], pdl[c]}, uu);
-      const ff mm = ff_add(ff{pm[c], pml[c]}, zz);
+      t.u[c] = uu.hi;
+      t.ul[c] = uu.lo;
+      t.z[c] = zz.hi;
+      t.zl[c] = zz.lo;
+    } else {
+      const float u = rr * ld[c];
+      t.u[c] = u;
+      t.z[c] = u * u + 2.0f * rr * L[c];
+      t.ul[c] = t.zl[c] = 0.f;
+    }
+  }
+  return t;
+}
+
+// delay = delay_parent + u, impulse = impulse_parent + z.
+template <bool COMP>
+__device__ __forceinline__ void td_accum(const TdIncr& t, const float (&pd)[4],
+                                         const float (&pdl)[4], const float (&pm)[4],
+                                         const float (&pml)[4], float (&d)[4], float (&dl)[4],
+                                         float (&m)[4], float (&ml)[4]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if constexpr (COMP) {
+      const ff dd = ff_add(ff{pd[c], pdl[c]}, ff{t.u[c], t.ul[c]});
+      const ff mm = ff_add(ff{pm[c], pml[c]}, ff{t.z[c], t.zl[c]});
       d[c] = dd.hi;
       dl[c] = dd.lo;
       m[c] = mm.hi;
       ml[c] = mm.lo;
     } else {
-      const float u = rr * ld[c];
-      d[c] = pd[c] + u;
-      m[c] = pm[c] + (u * u + 2.0f * rr * L[c]);
+      d[c] = pd[c] + t.u[c];
+      m[c] = pm[c] + t.z[c];
       dl[c] = ml[c] = 0.f;
     }
   }
@@ -1097,6 +1125,7 @@ __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int p
   const int maxd = window_depth(valid, par, ws, dw);
   const int sl = R.slot(i);
   const int ps = R.slot(par);
+  const TdIncr inc = td_incr<COMP>(rr, ld, L);
   float d[4], dl[4], m[4], ml[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) d[c] = dl[c] = m[c] = ml[c] = 0.f;
@@ -1109,7 +1138,7 @@ __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int p
 #pragma unroll
         for (int c = 0; c < 4; ++c) pd[c] = pm[c] = pdl[c] = pml[c] = 0.f;
       }
-      td_node<COMP>(rr, ld, L, pd, pdl, pm, pml, d, dl, m, ml);
+      td_accum<COMP>(inc, pd, pdl, pm, pml, d, dl, m, ml);
       R.put(sl, d, m, dl, ml);
     }
     wave_sync();
@@ -1130,37 +1159,61 @@ __device__ __forceinline__ OutBufs out_bufs(const RctParams& P, const Item& it) 
                                         : make_rsrc(nullptr, 0, 16, 0)};
 }
 
-// One chunk of a streamed net's bottom-up sweep (windows last to first).
+// One chunk of a streamed net's bottom-up sweep (windows last to first).  If
+// `take`, `ch` becomes the next chunk `nx` (requested a chunk earlier) after
+// the chunk's wait.
 template <int W, bool COMP>
 __device__ __forceinline__ void run_bu_chunk(StreamLds<W, COMP>& S, const RctParams& P,
-                                             const Item& it, int c, const Chunk& ch, int lane) {
+                                             const Item& it, int c, Chunk& ch, bool take,
+                                             const Chunk& nx, int lane) {
   const SRing<W, COMP> R{S, it.pb};
   const OutBufs B = out_bufs(P, it);
   const int n = it.pe - it.pb;
 #pragma unroll
   for (int s = kChunkWin - 1; s >= 0; --s) {
     const int q0 = c * kChunkPos + s * kWave;
-    if (q0 < n) {
+    // window 0 exists in every chunk; it is processed last
+    if (s == 0 || q0 < n) {
       float ldo[4], Lo[4];
       bu_window<W, COMP>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], ch.x[s], q0 + lane == 0,
                          ldo, Lo);
+      // chunk 0 stays in registers as the top-down sweep's first chunk
+      ch.x[s] = make_float4(ldo[0], ldo[1], ldo[2], ldo[3]);
+      ch.y[s] = make_float4(Lo[0], Lo[1], Lo[2], Lo[3]);
+      if (s == 0) {
+        // the chunk's one wait, on every path: before its last window's
+        // stores (the next chunk was requested a whole chunk earlier,
+        // earlier windows' stores long issued); the next chunk is taken
+        // over right after it
+        __builtin_amdgcn_s_waitcnt(0);
+        if (take) ch = nx;
+      }
       bstore4(B.load, (q0 + lane) * 16, ldo);
       bstore4(B.gl, (q0 + lane) * 16, Lo);
     }
   }
 }
 
-// One chunk of a streamed net's top-down sweep (windows first to last).
+// One chunk of a streamed net's top-down sweep (windows first to last).  The
+// last window's outputs are stored after the chunk's one wait, which sits on
+// every path; the next chunk is taken over right after that wait.
 template <int W, bool COMP>
 __device__ __forceinline__ void run_td_chunk(StreamLds<W, COMP>& S, const RctParams& P,
-                                             const Item& it, int c, const Chunk& ch, int lane) {
+                                             const Item& it, int c, Chunk& ch, bool take,
+                                             const Chunk& nx, int lane) {
   const SRing<W, COMP> R{S, it.pb};
   const OutBufs B = out_bufs(P, it);
   const int n = it.pe - it.pb;
+  // the last window's outputs (every chunk has one; left undefined until
+  // then, so that no instruction writes these registers early)
+  const float u = __builtin_nondeterministic_value(0.f);
+  float p_d[4] = {u, u, u, u}, p_ldel[4] = {u, u, u, u}, p_b[4] = {u, u, u, u},
+        p_m[4] = {u, u, u, u}, p_s[4] = {u, u, u, u};
+  int p_off = __builtin_nondeterministic_value(0);
 #pragma unroll
   for (int s = 0; s < kChunkWin; ++s) {
     const int q0 = c * kChunkPos + s * kWave;
-    if (q0 < n) {
+    if (s == 0 || q0 < n) {
       float ld[4], L[4];
       unpack(ch.x[s], ld);
       unpack(ch.y[s], L);
@@ -1168,71 +1221,128 @@ __device__ __forceinline__ void run_td_chunk(StreamLds<W, COMP>& S, const RctPar
       td_window<W, COMP>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], q0 + lane == 0, ld, L,
                          o_d, o_ldel, o_b, o_m, o_s, ch.si);
       const int off = (q0 + lane) * 16;
-      bstore4(B.delay, off, o_d);
-      bstore4(B.ldelay, off, o_ldel);
-      bstore4(B.beta, off, o_b);
-      bstore4(B.imp, off, o_m);
-      bstore4(B.slew, off, o_s);
+      if (s < kChunkWin - 1 && q0 + kWave < n) {
+        bstore4(B.delay, off, o_d);
+        bstore4(B.ldelay, off, o_ldel);
+        bstore4(B.beta, off, o_b);
+        bstore4(B.imp, off, o_m);
+        bstore4(B.slew, off, o_s);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          p_d[j] = o_d[j];
+          p_ldel[j] = o_ldel[j];
+          p_b[j] = o_b[j];
+          p_m[j] = o_m[j];
+          p_s[j] = o_s[j];
+        }
+        p_off = off;
+      }
     }
   }
+  __builtin_amdgcn_s_waitcnt(0);
+  if (take) ch = nx;
+  bstore4(B.delay, p_off, p_d);
+  bstore4(B.ldelay, p_off, p_ldel);
+  bstore4(B.beta, p_off, p_b);
+  bstore4(B.imp, p_off, p_m);
+  bstore4(B.slew, p_off, p_s);
 }
 
-__device__ __forceinline__ Item item_from(int d) {
-  Item it;
-  it.pb = __builtin_amdgcn_readlane(d, 0);
-  it.pe = __builtin_amdgcn_readlane(d, 1);
-  it.nb = __builtin_amdgcn_readlane(d, 2);
-  it.ne = __builtin_amdgcn_readlane(d, 3);
-  it.kind = __builtin_amdgcn_readlane(d, 4);
-  it.k = __builtin_amdgcn_readlane(d, 5);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane(d, 6);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane(d, 7);
-  it.scratch = (int64_t)(((uint64_t)hi << 32) | lo);
-  return it;
-}
+// Work items come from the largest-first list in one of two ways:
+//  * dealt out (packed groups, whose costs are alike): in round r workgroup
+//    w takes item r*G + w for even r and r*G + G-1-w for odd r (G
+//    workgroups), which evens out the totals of a sorted list without a
+//    shared counter;
+//  * from a queue (streamed nets, whose costs vary with their shapes): a
+//    counter in global memory, incremented once per item.  The increment
+//    for an item is issued one item before its index is read, so the wait
+//    for it is the chunk wait that the item in between has anyway.
+// Descriptors are read through the scalar cache (the list is read-only
+// during a launch) two items ahead, so the next item is always known when
+// its first loads are requested.  Scalar loads count against lgkmcnt, which
+// every LDS wait drains in passing; a vector load of the descriptor would be
+// waited for with vmcnt, i.e. together with every store still in flight.
+#ifndef GROUP_QUEUE
+#define GROUP_QUEUE false
+#endif
+typedef const __attribute__((address_space(4))) RctItem ConstItem;
 
-// Work items come from a queue; the current item, the next two descriptors
-// and the index after them are always known or in flight, so that the chunk
-// after the current one can be requested whichever item it belongs to.
+template <bool QUEUE>
 struct ItemPipe {
-  int32_t* queue;
-  Rsrc ritems;
+  const RctItem* items;
   int total;
+  int32_t* queue;  // QUEUE: next item index, finished workgroups
   int lane;
-  int d1, d2, q3;
-  bool v1, v2;
+  int round = 0;    // dealt: round of n2
+  int q = 0;        // queue: index after n2 (lane 0; requested, read one item later)
+  Item n1{}, n2{};  // the next two items of this workgroup (past the end: empty)
+  bool v1 = false, v2 = false;
+
+  __device__ __forceinline__ int dealt(int r) const {
+    const int g = gridDim.x, w = blockIdx.x;
+    return r * g + ((r & 1) ? g - 1 - w : w);
+  }
   __device__ __forceinline__ int pop() const {
     int v = 0;
-    if (lane == 0) v = atomicAdd(&queue[0], 1);
-    return v;  // valid in lane 0
+    if (lane == 0) v = atomicAdd(queue, 1);
+    return v;
   }
-  // lanes 0..7 hold the item's eight 32-bit fields; past the end: zeros
-  __device__ __forceinline__ int load_desc(int idx) const {
-    return bload_i(ritems, idx * (int)sizeof(RctItem) + (lane & 7) * 4);
+  __device__ __forceinline__ Item fetch(int i) const {
+    Item it{};
+    if (i < total) {
+      ConstItem* d = (ConstItem*)items + i;
+      it.pb = d->pos_begin;
+      it.pe = d->pos_end;
+      it.nb = d->net_begin;
+      it.ne = d->net_end;
+      it.kind = d->kind;
+      it.k = d->k;
+      it.scratch = d->scratch;
+    }
+    return it;
   }
   // Starts the pipeline; returns the first item (valid if *have).
   __device__ __forceinline__ Item start(bool* have) {
-    const int q0 = __builtin_amdgcn_readfirstlane(pop());
-    const int q1 = __builtin_amdgcn_readfirstlane(pop());
-    const int q2 = __builtin_amdgcn_readfirstlane(pop());
-    d1 = load_desc(q1 < total ? q1 : total);
-    d2 = load_desc(q2 < total ? q2 : total);
-    q3 = pop();
-    v1 = q1 < total;
-    v2 = q2 < total;
-    *have = q0 < total;
-    return item_from(load_desc(q0 < total ? q0 : total));
+    int i0, i1, i2;
+    if constexpr (QUEUE) {
+      int v = 0;
+      if (lane == 0) v = atomicAdd(queue, 3);
+      const int b = __builtin_amdgcn_readfirstlane(v);
+      i0 = b;
+      i1 = b + 1;
+      i2 = b + 2;
+    } else {
+      i0 = dealt(0);
+      i1 = dealt(1);
+      i2 = dealt(2);
+      round = 2;
+    }
+    *have = i0 < total;
+    const Item first = fetch(i0);
+    v1 = i1 < total;
+    n1 = fetch(i1);
+    v2 = i2 < total;
+    n2 = fetch(i2);
+    return first;
   }
-  // Moves to the next item; false at the end of the queue.
+  __device__ __forceinline__ bool has_next() const { return v1; }
+  // Moves to the next item; false at the end of this workgroup's share.
+  // (Dealt indices grow with the round and queue indices with every
+  // request, so the first index past the end ends the share.)
   __device__ __forceinline__ bool next(Item& it) {
     if (!v1) return false;
-    it = item_from(d1);
-    d1 = d2;
+    it = n1;
+    n1 = n2;
     v1 = v2;
-    const int qq = __builtin_amdgcn_readfirstlane(q3);
-    v2 = qq < total;
-    d2 = load_desc(v2 ? qq : total);
-    q3 = pop();
+    int i;
+    if constexpr (QUEUE) {
+      i = __builtin_amdgcn_readfirstlane(pop());
+    } else {
+      i = dealt(++round);
+    }
+    v2 = i < total;
+    n2 = fetch(i);
     return true;
   }
 };
@@ -1249,80 +1359,83 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
   __shared__ StreamLds<W, COMP> S;
   const int lane = threadIdx.x;
   const int total = P.num_items_dev ? gld(P.num_items_dev) : P.num_items;
-  ItemPipe pipe{P.queue, make_rsrc(P.items, 0, (int)sizeof(RctItem), total), total, lane};
+  ItemPipe<true> pipe{P.items, total, P.queue, lane};
   bool have = false;
   Item it = pipe.start(&have);
   int ph = 0, c = 0;
   while (have && !stream_first_chunk(it, ph, c)) have = pipe.next(it);
   // Chunk data: `cur` is being processed, `nxt` is in flight.  A wavefront
   // that has both loads and stores outstanding can only wait for all of
-  // them, so the wait for the next chunk happens once, at the top of the
-  // loop, before the following chunk is requested; the processing itself
-  // then reads only registers with no load in flight and never waits on
-  // global memory.
+  // them, so each chunk waits once, just before the stores of its last
+  // window, on every path through the chunk: by then the next chunk's loads
+  // and the earlier windows' stores were issued long ago, and the last
+  // window's stores stay in flight during the next chunk.  `nxt` is copied
+  // to `cur` right after that wait, where nothing is outstanding, so the
+  // copy needs no wait of its own.  The bottom-up sweep's chunk 0 (the last
+  // one it processes) becomes the top-down sweep's chunk 0 in registers.
   Chunk cur, nxt;
   int nxt_pb = -1, nxt_ph = -1, nxt_c = -1;  // what nxt holds
-  while (have) {
-    // ---- the chunk to process: prefetched, or read now ----------------------
+  if (have) {
+    load_chunk(P, it, ph, c, cur, lane);
     __builtin_amdgcn_s_waitcnt(0);
-    if (it.pb == nxt_pb && ph == nxt_ph && c == nxt_c) {
-      cur = nxt;
-      nxt_pb = -1;
-    } else {
-      // read now (first chunk, or top-down chunk 0, which reads back the
-      // bottom-up results stored just before: the wait above has completed
-      // those stores)
-      load_chunk(P, it, ph, c, cur, lane);
-      __builtin_amdgcn_s_waitcnt(0);
-    }
+  }
+  while (have) {
+    const int nc = num_chunks(it);
     // ---- request the chunk after this one --------------------------------
-    {
-      int pph = -1, pc = -1;
-      Item pit = it;
-      bool pv = false;
-      if (ph == PH_BU && c > 0) {
-        pph = PH_BU;
-        pc = c - 1;
-        pv = true;
-      } else if (ph == PH_BU && c == 0 && num_chunks(it) > 1) {
-        // top-down chunk 0 is read back after this chunk's stores; chunk 1
-        // was stored by the previous chunk, complete since the wait above
-        pph = PH_TD;
-        pc = 1;
-        pv = true;
-      } else if (ph == PH_TD && c + 1 < num_chunks(it)) {
-        pph = PH_TD;
-        pc = c + 1;
-        pv = true;
-      } else if (pipe.v1) {
-        pit = item_from(pipe.d1);
-        pv = stream_first_chunk(pit, pph, pc);
-      }
-      if (pv && !(pit.pb == nxt_pb && pph == nxt_ph && pc == nxt_c)) {
-        load_chunk(P, pit, pph, pc, nxt, lane);
-        nxt_pb = pit.pb;
-        nxt_ph = pph;
-        nxt_c = pc;
-      }
+    Item pit = it;
+    int pph = -1, pc = -1;
+    bool pv = false;
+    if (ph == PH_BU && c > 0) {
+      pph = PH_BU;
+      pc = c - 1;
+      pv = true;
+    } else if (ph == PH_TD && c + 1 < nc) {
+      pph = PH_TD;
+      pc = c + 1;
+      pv = true;
+    } else if ((ph == PH_TD || nc == 1) && pipe.has_next()) {
+      // the current item's last chunk: the next item's first chunk
+      // (top-down chunk 1 of a longer net is requested after the bottom-up
+      // chunk 0, whose wait completes the stores it reads)
+      pit = pipe.n1;
+      pv = stream_first_chunk(pit, pph, pc);
     }
+    if (pv && !(pit.pb == nxt_pb && pph == nxt_ph && pc == nxt_c)) {
+      load_chunk(P, pit, pph, pc, nxt, lane);
+      nxt_pb = pit.pb;
+      nxt_ph = pph;
+      nxt_c = pc;
+    }
+    // the next chunk is in nxt (always, except after the bottom-up chunk 0,
+    // and at an item's end when the next item was not known in time)
+    const bool take = pv && pit.pb == nxt_pb && pph == nxt_ph && pc == nxt_c;
     // ---- process ----------------------------------------------------------
     if (ph == PH_BU) {
-      if (c == num_chunks(it) - 1) {
+      if (c == nc - 1) {
         const SRing<W, COMP> R{S, it.pb};
         for (int k = lane; k < W; k += kWave) R.zero(k);
         wave_sync();
       }
-      run_bu_chunk<W, COMP>(S, P, it, c, cur, lane);
+      run_bu_chunk<W, COMP>(S, P, it, c, cur, take, nxt, lane);
     } else {
-      run_td_chunk<W, COMP>(S, P, it, c, cur, lane);
+      run_td_chunk<W, COMP>(S, P, it, c, cur, take, nxt, lane);
     }
+    if (take) nxt_pb = -1;
     // ---- advance ------------------------------------------------------------
     if (ph == PH_BU && c == 0) {
-      ph = PH_TD;
+      if (nc > 1) {
+        load_chunk<PH_TD>(P, it, 1, nxt, lane);
+        nxt_pb = it.pb;
+        nxt_ph = PH_TD;
+        nxt_c = 1;
+      }
+      ph = PH_TD;  // chunk 0 is in cur already
       c = 0;
-    } else if (ph == PH_BU) {
+      continue;
+    }
+    if (ph == PH_BU) {
       --c;
-    } else if (ph == PH_TD && c + 1 < num_chunks(it)) {
+    } else if (c + 1 < nc) {
       ++c;
     } else {
       bool found = false;
@@ -1333,6 +1446,12 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
         }
       }
       if (!found) break;
+    }
+    if (!take) {
+      // not requested ahead (an item that was not known in time)
+      __builtin_amdgcn_s_waitcnt(0);
+      load_chunk(P, it, ph, c, cur, lane);
+      __builtin_amdgcn_s_waitcnt(0);
     }
   }
   queue_release(P.queue, lane);
@@ -1415,10 +1534,9 @@ __device__ __forceinline__ void load_group(const RctParams& P, const Item& it, G
 }
 
 template <int G, bool COMP>
-__device__ __forceinline__ void run_group(GroupLds<G, COMP>& S, const RctParams& P,
-                                          const Item& it, const GroupChunk<G>& ch, int lane) {
+__device__ __forceinline__ void group_stage(GroupLds<G, COMP>& S, const Item& it,
+                                            const GroupChunk<G>& ch, int lane) {
   constexpr int K = GroupChunk<G>::K;
-  constexpr int kPull = 4;  // children fetched per batch
   const int n = it.pe - it.pb;
   // ---- stage the group ------------------------------------------------------
 #pragma unroll
@@ -1433,6 +1551,13 @@ __device__ __forceinline__ void run_group(GroupLds<G, COMP>& S, const RctParams&
 #pragma unroll
   for (int s = 0; s < GroupChunk<G>::NS; ++s) S.si[s * kWave + lane] = ch.si[s];
   wave_sync();
+}
+
+// Both sweeps of a staged group (LDS only).
+template <int G, bool COMP>
+__device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& it, int lane) {
+  constexpr int kPull = 4;  // children fetched per batch
+  const int n = it.pe - it.pb;
   // ---- bottom-up, deepest schedule entries first ------------------------------
   uint32_t pwj = n - kWave + lane >= 0 ? S.pw[n - kWave + lane] : 0u;
   for (int e = n; e > 0;) {
@@ -1558,7 +1683,7 @@ __device__ __forceinline__ void run_group(GroupLds<G, COMP>& S, const RctParams&
         for (int c = 0; c < 4; ++c) pd[c] = pm[c] = pdl[c] = pml[c] = 0.f;
       }
       float d[4], dl[4], mm[4], ml[4];
-      td_node<COMP>(rr, ldv, Lv, pd, pdl, pm, pml, d, dl, mm, ml);
+      td_accum<COMP>(td_incr<COMP>(rr, ldv, Lv), pd, pdl, pm, pml, d, dl, mm, ml);
       S.cap[q] = pack(d);
       S.w[q] = pack(mm);
       if constexpr (COMP) {
@@ -1571,7 +1696,14 @@ __device__ __forceinline__ void run_group(GroupLds<G, COMP>& S, const RctParams&
     b = e;
     pwj = pwn;
   }
-  // ---- outputs, lanes on consecutive positions (coalesced stores) ------------
+}
+
+// Outputs of a swept group, lanes on consecutive positions (coalesced stores).
+template <int G, bool COMP>
+__device__ __forceinline__ void group_outputs(GroupLds<G, COMP>& S, const RctParams& P,
+                                              const Item& it, int lane) {
+  constexpr int K = GroupChunk<G>::K;
+  const int n = it.pe - it.pb;
   const Rsrc rout[6] = {make_rsrc(P.load, it.pb, 16, n),   make_rsrc(P.delay, it.pb, 16, n),
                         make_rsrc(P.ldelay, it.pb, 16, n), make_rsrc(P.beta, it.pb, 16, n),
                         make_rsrc(P.impulse, it.pb, 16, n), make_rsrc(P.slew, it.pb, 16, n)};
@@ -1616,41 +1748,36 @@ __global__ __launch_bounds__(64) void rct_group_kernel(RctParams P) {
   __shared__ GroupLds<G, COMP> S;
   const int lane = threadIdx.x;
   const int total = P.num_items_dev ? gld(P.num_items_dev) : P.num_items;
-  ItemPipe pipe{P.queue, make_rsrc(P.items, 0, (int)sizeof(RctItem), total), total, lane};
+  ItemPipe<GROUP_QUEUE> pipe{P.items, total, P.queue, lane};
   bool have = false;
   Item it = pipe.start(&have);
-  while (have && it.pe <= it.pb) have = pipe.next(it);
-  GroupChunk<G> cur, nxt;
-  int nxt_pb = -1;
-  while (have) {
+  // One register buffer: a group's inputs are staged into LDS first, after
+  // which the buffer takes the next group's inputs while this group is
+  // swept.  The single wait per group sits between the sweeps and the
+  // output stores, so it covers loads and stores issued a whole group
+  // earlier, and this group's stores stay in flight during the next group.
+  GroupChunk<G> R;
+  if (have) {
+    load_group<G>(P, it, R, lane);
     __builtin_amdgcn_s_waitcnt(0);
-    if (it.pb == nxt_pb) {
-      cur = nxt;
-      nxt_pb = -1;
-    } else {
-      load_group<G>(P, it, cur, lane);
-      __builtin_amdgcn_s_waitcnt(0);
-    }
-    if (pipe.v1) {
-      const Item pit = item_from(pipe.d1);
-      if (pit.pe > pit.pb) {
-        load_group<G>(P, pit, nxt, lane);
-        nxt_pb = pit.pb;
-      }
-    }
-    run_group<G, COMP>(S, P, it, cur, lane);
+  }
+  while (have) {
+    group_stage<G, COMP>(S, it, R, lane);
+    // (every listed group is non-empty, and an empty one would be solved as
+    // a no-op, so the next item is taken without a skip loop: a loop here
+    // would make the compiler wait for the descriptor loads it issues)
+    Item nit = it;
+    const bool nhave = pipe.next(nit);
+    if (nhave) load_group<G>(P, nit, R, lane);
+    group_sweeps<G, COMP>(S, it, lane);
+    __builtin_amdgcn_s_waitcnt(0);
+    group_outputs<G, COMP>(S, P, it, lane);
     // a group listed by a subset solve is released for the next call
     if (P.group_flag && lane == 0) P.group_flag[it.k] = 0;
-    bool found = false;
-    while (pipe.next(it)) {
-      if (it.pe > it.pb) {
-        found = true;
-        break;
-      }
-    }
-    if (!found) break;
+    it = nit;
+    have = nhave;
   }
-  queue_release(P.queue, lane);
+  if constexpr (GROUP_QUEUE) queue_release(P.queue, lane);
 }
 
 // Per-net validation and largest parent offset (plan building).
@@ -1674,26 +1801,21 @@ __global__ void rct_netinfo_kernel(int32_t num_nets, const int32_t* __restrict__
 // Group capacity (positions) of a plan: 128 or 256, from small_target.
 inline int group_cap(int small_target) { return small_target <= 128 ? 128 : 256; }
 
-// Subset solves, streamed nets: one item per listed net.  Nets outside
-// [0, num_nets), packed nets and global-window nets become empty or
-// non-streamed items that the streaming kernel skips.
+// Subset solves, streamed nets: the listed nets that the streaming kernel
+// solves, appended to a compact list (count in *out_count).
 __global__ void rct_subset_items_kernel(int32_t count, const int32_t* __restrict__ subset,
                                         int32_t num_nets, const int32_t* __restrict__ nodes_start,
                                         const int32_t* __restrict__ net_maxpid,
                                         const int64_t* __restrict__ net_scratch,
                                         int32_t group_max, int32_t long_maxpid,
-                                        RctItem* __restrict__ items) {
+                                        RctItem* __restrict__ items, int32_t* __restrict__ out_count) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   const int net = subset[j];
-  RctItem it{0, 0, 0, 0, ITEM_PACKED, 1, 0};
-  if (net >= 0 && net < num_nets) {
-    const int pb = nodes_start[net], pe = nodes_start[net + 1];
-    if (pe - pb > group_max && net_maxpid[net] <= long_maxpid) {
-      it = RctItem{pb, pe, net, net + 1, ITEM_STREAM, 1, net_scratch[net]};
-    }
-  }
-  items[j] = it;
+  if (net < 0 || net >= num_nets) return;
+  const int pb = nodes_start[net], pe = nodes_start[net + 1];
+  if (pe - pb > group_max && net_maxpid[net] <= long_maxpid)
+    items[atomicAdd(out_count, 1)] = RctItem{pb, pe, net, net + 1, ITEM_STREAM, 1, net_scratch[net]};
 }
 
 // Subset solves, packed nets: the groups holding the listed nets, each once.
@@ -1759,8 +1881,9 @@ int blocks_per_cu(void* fn) {
 // Static schedule of one packed group (positions [pb, pe) of the host
 // arrays): nodes sorted by (depth, parent, position); one word per schedule
 // index (node | depth << 8 | first child << 16 | children << 24).
-void build_group_schedule(const int32_t* hpid, const int32_t* hstart, const RctItem& g,
-                          uint32_t* out) {
+// Returns the group's number of depth steps per sweep (its cost estimate).
+int build_group_schedule(const int32_t* hpid, const int32_t* hstart, const RctItem& g,
+                         uint32_t* out) {
   const int n = g.pos_end - g.pos_begin;
   int depth[256], par[256], order[256], cf[256], cn[256];
   int next_root = g.net_begin;
@@ -1787,11 +1910,15 @@ void build_group_schedule(const int32_t* hpid, const int32_t* hstart, const RctI
     const int p = par[order[j]];
     if (p >= 0 && cn[p]++ == 0) cf[p] = j;
   }
+  int steps = 0;
   for (int j = 0; j < n; ++j) {
     const int q = order[j];
     out[j] = (uint32_t)q | ((uint32_t)depth[q] << 8) | ((uint32_t)cf[q] << 16) |
              ((uint32_t)cn[q] << 24);
+    // a step takes up to 64 schedule entries of one depth
+    if (j == 0 || depth[q] != depth[order[j - 1]] || (j & 63) == 0) ++steps;
   }
+  return steps;
 }
 
 }  // namespace
@@ -1898,12 +2025,15 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
     int total = plan->num_items;
     Q.items = plan->d_items;
     if (subset && plan->stats[2] > 0) {
+      int32_t* n_sub = plan->d_queue + 6;
+      hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, n_sub, 1);
       hipLaunchKernelGGL(rct_subset_items_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
                          count, subset, plan->num_nets, tree->nodes_start, plan->d_net_maxpid,
-                         plan->d_net_scratch, G, P.long_maxpid, plan->d_sub_items);
+                         plan->d_net_scratch, G, P.long_maxpid, plan->d_sub_items, n_sub);
       OTG_HIP(ctx, hipGetLastError());
       Q.items = plan->d_sub_items;
-      total = count;
+      Q.num_items_dev = n_sub;
+      total = count;  // bound; the count is read on the device
     } else if (subset) {
       total = 0;
     }
@@ -2004,7 +2134,8 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   // net alone).  Larger nets are streamed when their parent offsets fit the
   // ring (maxpid <= W - 64) and take the global-window path otherwise.
   // Streamed nets get 4 floats of scratch per node (their L), global-window
-  // nets 8 (16 compensated).  Streamed nets are queued largest first.
+  // nets 8 (16 compensated).  Streamed nets are queued largest first; packed
+  // groups are ordered by their number of depth steps, most first.
   std::vector<RctItem> big_items, groups, gitems;
   std::vector<int64_t> net_scratch(N, -1);
   std::vector<int32_t> net_group(N, -1);
@@ -2081,14 +2212,30 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
     std::vector<uint32_t> sched(sched_words);
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     const size_t ng = groups.size();
+    std::vector<int32_t> steps(ng);
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < hw; ++t)
       pool.emplace_back([&, t]() {
         for (size_t g = t; g < ng; g += hw)
-          build_group_schedule(hpid.data(), hstart.data(), groups[g],
-                               sched.data() + groups[g].scratch);
+          steps[g] = build_group_schedule(hpid.data(), hstart.data(), groups[g],
+                                          sched.data() + groups[g].scratch);
       });
     for (auto& th : pool) th.join();
+    // costliest groups first (the kernel deals them out in that order); a
+    // group's index k follows it
+    {
+      std::vector<int32_t> order(ng);
+      for (size_t g = 0; g < ng; ++g) order[g] = (int32_t)g;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int32_t a, int32_t b) { return steps[a] > steps[b]; });
+      std::vector<RctItem> sorted(ng);
+      for (size_t i = 0; i < ng; ++i) {
+        sorted[i] = groups[order[i]];
+        sorted[i].k = (int32_t)i;
+        for (int k = sorted[i].net_begin; k < sorted[i].net_end; ++k) net_group[k] = (int32_t)i;
+      }
+      groups.swap(sorted);
+    }
     PLAN_HIP(hipMalloc(&p->d_sched, sizeof(uint32_t) * sched_words));
     PLAN_HIP(hipMemcpy(p->d_sched, sched.data(), sizeof(uint32_t) * sched_words,
                        hipMemcpyHostToDevice));
