@@ -403,9 +403,9 @@ static bool write_tuning(const char* path, const otg_rct_tuning& t, otg_rct_meth
   FILE* f = fopen(path, "w");
   if (!f) return false;
   fprintf(f,
-          "{\"ring_slots\": %d, \"small_target\": %d, \"grid_per_cu\": %d, "
-          "\"method\": %d, \"ms\": %.6f}\n",
-          t.ring_slots, t.small_target, t.grid_per_cu, (int)m, ms);
+          "{\"ring_slots\": %d, \"min_ring\": %d, \"small_target\": %d, "
+          "\"grid_per_cu\": %d, \"method\": %d, \"ms\": %.6f}\n",
+          t.ring_slots, t.min_ring, t.small_target, t.grid_per_cu, (int)m, ms);
   fclose(f);
   return true;
 }
@@ -433,6 +433,7 @@ extern "C" otg_status otg_rct_tuning_load(otg_context* ctx, const char* path, ot
   fclose(f);
   otg_rct_tuning r{};
   r.ring_slots = json_int(s, "ring_slots", 0);
+  r.min_ring = json_int(s, "min_ring", 0);
   r.small_target = json_int(s, "small_target", 0);
   r.grid_per_cu = json_int(s, "grid_per_cu", 0);
   ctx->tuning = r;
@@ -451,18 +452,23 @@ extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tre
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   std::vector<otg_rct_tuning> cands;
-  // ring size (LDS per wavefront, hence occupancy, against the nets that
-  // fall back to the global-window path), packed-group size, and the number
-  // of persistent workgroups per CU (0: as many as fit)
+  // largest and smallest ring size (LDS per wavefront, hence occupancy,
+  // against the nets that fall back to the global-window path and the
+  // launches per ring size), packed-group size, and the number of persistent
+  // workgroups per CU (0: as many as fit)
   for (int W : {512, 256, 1024})
-    for (int target : {256, 192, 128})
-      for (int g : {0, 4, 6}) {
-        otg_rct_tuning t{};
-        t.ring_slots = W;
-        t.small_target = target;
-        t.grid_per_cu = g;
-        cands.push_back(t);
-      }
+    for (int lo : {256, 512, 1024}) {
+      if (lo > W) continue;
+      for (int target : {256, 128})
+        for (int g : {0, 4}) {
+          otg_rct_tuning t{};
+          t.ring_slots = W;
+          t.min_ring = lo;
+          t.small_target = target;
+          t.grid_per_cu = g;
+          cands.push_back(t);
+        }
+    }
   hipEvent_t e0, e1;
   OTG_HIP(ctx, hipEventCreate(&e0));
   OTG_HIP(ctx, hipEventCreate(&e1));

@@ -40,8 +40,17 @@ struct otg_rct_plan {
   otg_rct_tuning tuning{};
   int32_t num_nets = 0;
   int32_t total_nodes = 0;
-  int32_t num_items = 0;
-  RctItem* d_items = nullptr;      // streamed nets, largest first
+  // Streamed nets in bins by ring size (ascending, at most the tuned ring):
+  // a net goes to the first bin whose ring holds its parent offsets
+  // (maxpid <= ring - 64).  A smaller ring needs less LDS, so more
+  // workgroups fit on a CU.
+  static constexpr int kMaxBins = 3;
+  int num_bins = 0;
+  int bin_ring[kMaxBins] = {0, 0, 0};
+  int32_t bin_items[kMaxBins] = {0, 0, 0};     // nets in the bin
+  RctItem* d_bin_items[kMaxBins] = {};         // largest first
+  RctItem* d_bin_sub[kMaxBins] = {};           // subset solves: the bin's listed nets
+  int bin_grid[kMaxBins] = {0, 0, 0};          // persistent workgroups
   int32_t num_groups = 0;
   int32_t group_cap = 256;         // largest net of a packed group
   RctItem* d_groups = nullptr;     // packed groups (k: group index, scratch: schedule offset)
@@ -52,15 +61,14 @@ struct otg_rct_plan {
   int32_t* d_sub_count = nullptr;  // number of them
   int32_t num_gitems = 0;
   RctItem* d_gitems = nullptr;     // global-window nets
-  RctItem* d_sub_items = nullptr;  // [num_nets] items of a subset solve
-  int32_t* d_queue = nullptr;      // next item and finished workgroups of the streaming
-                                   // [0,1] and global-window [2,3] kernels; [6]: streamed
-                                   // nets of a subset solve
+  int32_t* d_queue = nullptr;      // [16]: next item and finished workgroups of streaming
+                                   // bin b [2b, 2b+1] and of the global-window kernel
+                                   // [6, 7]; [8 + b]: bin b's nets of a subset solve
   float* d_scratch = nullptr;      // window storage of stream/global items
   int64_t scratch_floats = 0;
   int32_t* d_net_maxpid = nullptr;  // per net, for subset solves
   int64_t* d_net_scratch = nullptr; // per net scratch offset (-1: none)
-  int grid = 0;                     // persistent workgroups (streaming, global-window)
+  int grid = 0;                     // persistent workgroups (global-window)
   int grid_group = 0;               // persistent workgroups (packed groups)
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };

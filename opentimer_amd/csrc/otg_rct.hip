@@ -1801,21 +1801,35 @@ __global__ void rct_netinfo_kernel(int32_t num_nets, const int32_t* __restrict__
 // Group capacity (positions) of a plan: 128 or 256, from small_target.
 inline int group_cap(int small_target) { return small_target <= 128 ? 128 : 256; }
 
-// Subset solves, streamed nets: the listed nets that the streaming kernel
-// solves, appended to a compact list (count in *out_count).
+// Subset solves, streamed nets: the listed nets that a streaming kernel
+// solves, appended to the list of the first bin whose ring holds their
+// parent offsets (lim[b] = ring - 64; counts[b] counts bin b's list).
+struct SubBins {
+  RctItem* items[3];
+  int32_t lim[3];
+  int32_t num_bins;
+};
+
 __global__ void rct_subset_items_kernel(int32_t count, const int32_t* __restrict__ subset,
                                         int32_t num_nets, const int32_t* __restrict__ nodes_start,
                                         const int32_t* __restrict__ net_maxpid,
                                         const int64_t* __restrict__ net_scratch,
-                                        int32_t group_max, int32_t long_maxpid,
-                                        RctItem* __restrict__ items, int32_t* __restrict__ out_count) {
+                                        int32_t group_max, SubBins bins,
+                                        int32_t* __restrict__ counts) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   const int net = subset[j];
   if (net < 0 || net >= num_nets) return;
   const int pb = nodes_start[net], pe = nodes_start[net + 1];
-  if (pe - pb > group_max && net_maxpid[net] <= long_maxpid)
-    items[atomicAdd(out_count, 1)] = RctItem{pb, pe, net, net + 1, ITEM_STREAM, 1, net_scratch[net]};
+  if (pe - pb <= group_max) return;
+  const int mp = net_maxpid[net];
+  for (int b = 0; b < bins.num_bins; ++b) {
+    if (mp <= bins.lim[b]) {
+      bins.items[b][atomicAdd(counts + b, 1)] =
+          RctItem{pb, pe, net, net + 1, ITEM_STREAM, 1, net_scratch[net]};
+      return;
+    }
+  }
 }
 
 // Subset solves, packed nets: the groups holding the listed nets, each once.
@@ -1937,8 +1951,13 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   // a packed group holds at most 256 positions (8-bit schedule fields)
   r.small_target = std::max(64, std::min(r.small_target, 256));
   if (!r.grid_per_cu) r.grid_per_cu = d.grid_per_cu;
-  r.reserved = 0;
-  (void)m;
+  // the compensated kernel's ring entries are twice as large, and its
+  // occupancy with the 512-slot ring (4 wavefronts per CU) leaves LDS
+  // latency exposed: its default splits the streamed nets by ring size
+  if (!r.min_ring) r.min_ring = d.min_ring ? d.min_ring
+                                           : (m == OTG_RCT_BATCHED_COMPENSATED ? 256 : r.ring_slots);
+  if (r.min_ring != 256 && r.min_ring != 512 && r.min_ring != 1024) r.min_ring = 256;
+  r.min_ring = std::min(r.min_ring, r.ring_slots);
   return r;
 }
 
@@ -1998,7 +2017,6 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   // ---- packed groups ------------------------------------------------------------
   if (plan->num_groups > 0) {
     RctParams Q = P;
-    Q.queue = plan->d_queue + 4;
     if (subset) {
       hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, plan->d_sub_count, 1);
       hipLaunchKernelGGL(rct_subset_groups_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
@@ -2018,37 +2036,44 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
     void* args[] = {&Q};
     OTG_HIP(ctx, hipLaunchKernel(pick_group(G, comp), dim3(std::max(1, grid)), dim3(64), args, 0, s));
   }
-  // ---- streamed nets -----------------------------------------------------------
-  {
-    RctParams Q = P;
-    Q.queue = plan->d_queue;
-    int total = plan->num_items;
-    Q.items = plan->d_items;
+  // ---- streamed nets, one launch per ring size (largest ring first) ----------
+  if (plan->num_bins > 0) {
+    int32_t* n_sub = plan->d_queue + 8;
     if (subset && plan->stats[2] > 0) {
-      int32_t* n_sub = plan->d_queue + 6;
-      hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, n_sub, 1);
+      SubBins sb{};
+      sb.num_bins = plan->num_bins;
+      for (int b = 0; b < plan->num_bins; ++b) {
+        sb.items[b] = plan->d_bin_sub[b];
+        sb.lim[b] = plan->bin_ring[b] - kWave;
+      }
+      hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, n_sub, plan->num_bins);
       hipLaunchKernelGGL(rct_subset_items_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
                          count, subset, plan->num_nets, tree->nodes_start, plan->d_net_maxpid,
-                         plan->d_net_scratch, G, P.long_maxpid, plan->d_sub_items, n_sub);
+                         plan->d_net_scratch, G, sb, n_sub);
       OTG_HIP(ctx, hipGetLastError());
-      Q.items = plan->d_sub_items;
-      Q.num_items_dev = n_sub;
-      total = count;  // bound; the count is read on the device
-    } else if (subset) {
-      total = 0;
     }
-    Q.num_items = total;
-    if (total > 0) {
-      const int grid = std::min(plan->grid, total);
+    for (int b = plan->num_bins - 1; b >= 0; --b) {
+      if (plan->bin_items[b] <= 0) continue;
+      RctParams Q = P;
+      Q.queue = plan->d_queue + 2 * b;
+      int total = plan->bin_items[b];
+      Q.items = plan->d_bin_items[b];
+      if (subset) {
+        Q.items = plan->d_bin_sub[b];
+        Q.num_items_dev = n_sub + b;
+        total = std::min(count, plan->bin_items[b]);  // bound; the count is read on the device
+      }
+      Q.num_items = total;
+      const int grid = std::min(plan->bin_grid[b], total);
       void* args[] = {&Q};
-      OTG_HIP(ctx, hipLaunchKernel(pick_stream(plan->tuning.ring_slots, comp), dim3(grid), dim3(64),
+      OTG_HIP(ctx, hipLaunchKernel(pick_stream(plan->bin_ring[b], comp), dim3(grid), dim3(64),
                                    args, 0, s));
     }
   }
   // ---- nets whose parent offsets exceed the ring ------------------------------
   if (plan->stats[3] > 0) {
     RctParams Q = P;
-    Q.queue = plan->d_queue + 2;
+    Q.queue = plan->d_queue + 6;
     if (subset) {
       Q.subset = subset;
       Q.subset_count = count;
@@ -2131,12 +2156,16 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   // ---- binning --------------------------------------------------------------
   // Nets of up to G nodes (G = 128 or 256) go into packed groups: consecutive
   // nets, at most small_target positions and G/2 nets per group (a larger
-  // net alone).  Larger nets are streamed when their parent offsets fit the
-  // ring (maxpid <= W - 64) and take the global-window path otherwise.
+  // net alone).  Larger nets are streamed with the smallest ring (256, 512
+  // or 1024 slots, from min_ring up to W) that holds their parent offsets
+  // (maxpid <= ring - 64), and take the global-window path when none does.
   // Streamed nets get 4 floats of scratch per node (their L), global-window
   // nets 8 (16 compensated).  Streamed nets are queued largest first; packed
   // groups are ordered by their number of depth steps, most first.
-  std::vector<RctItem> big_items, groups, gitems;
+  int nb = 0;
+  for (int ring = p->tuning.min_ring; ring <= W; ring *= 2) p->bin_ring[nb++] = ring;
+  p->num_bins = nb;
+  std::vector<RctItem> bin_list[otg_rct_plan::kMaxBins], groups, gitems;
   std::vector<int64_t> net_scratch(N, -1);
   std::vector<int32_t> net_group(N, -1);
   int64_t scratch = 0, sched_words = 0;
@@ -2176,10 +2205,12 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
     }
     flush();
     RctItem it{hstart[k], hstart[k + 1], k, k + 1, ITEM_STREAM, 1, scratch};
-    if (hmax[k] <= W - kWave) {
+    int bin = 0;
+    while (bin < nb && hmax[k] > p->bin_ring[bin] - kWave) ++bin;
+    if (bin < nb) {
       scratch += 4LL * n;
       ++n_stream;
-      big_items.push_back(it);
+      bin_list[bin].push_back(it);
     } else {
       it.kind = ITEM_GLOBAL;
       scratch += slot_floats * n;
@@ -2192,9 +2223,13 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   auto larger = [](const RctItem& a, const RctItem& b) {
     return (a.pos_end - a.pos_begin) > (b.pos_end - b.pos_begin);
   };
-  std::stable_sort(big_items.begin(), big_items.end(), larger);
+  int64_t n_items = 0;
+  for (int b = 0; b < nb; ++b) {
+    std::stable_sort(bin_list[b].begin(), bin_list[b].end(), larger);
+    p->bin_items[b] = (int32_t)bin_list[b].size();
+    n_items += p->bin_items[b];
+  }
   std::stable_sort(gitems.begin(), gitems.end(), larger);
-  p->num_items = (int32_t)big_items.size();
   p->num_gitems = (int32_t)gitems.size();
   p->num_groups = (int32_t)groups.size();
   p->stats[0] = p->num_groups;
@@ -2202,7 +2237,7 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   p->stats[2] = n_stream;
   p->stats[3] = n_global;
   p->stats[4] = maxn;
-  p->stats[5] = p->num_items + p->num_gitems + p->num_groups;
+  p->stats[5] = n_items + p->num_gitems + p->num_groups;
   p->stats[6] = scratch;
   p->stats[7] = W;
   // ---- group schedules (host, from the parent offsets) ---------------------------
@@ -2249,10 +2284,13 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
     PLAN_HIP(hipMemcpy(p->d_net_group, net_group.data(), sizeof(int32_t) * N,
                        hipMemcpyHostToDevice));
   }
-  if (!big_items.empty()) {
-    PLAN_HIP(hipMalloc(&p->d_items, sizeof(RctItem) * big_items.size()));
-    PLAN_HIP(hipMemcpyAsync(p->d_items, big_items.data(), sizeof(RctItem) * big_items.size(),
+  for (int b = 0; b < nb; ++b) {
+    const size_t m = bin_list[b].size();
+    if (!m) continue;
+    PLAN_HIP(hipMalloc(&p->d_bin_items[b], sizeof(RctItem) * m));
+    PLAN_HIP(hipMemcpyAsync(p->d_bin_items[b], bin_list[b].data(), sizeof(RctItem) * m,
                             hipMemcpyHostToDevice, ctx->stream));
+    PLAN_HIP(hipMalloc(&p->d_bin_sub[b], sizeof(RctItem) * m));
   }
   if (!gitems.empty()) {
     PLAN_HIP(hipMalloc(&p->d_gitems, sizeof(RctItem) * gitems.size()));
@@ -2262,16 +2300,18 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   if (N > 0) {
     PLAN_HIP(hipMemcpyAsync(p->d_net_scratch, net_scratch.data(), sizeof(int64_t) * N,
                             hipMemcpyHostToDevice, ctx->stream));
-    if (n_stream > 0) PLAN_HIP(hipMalloc(&p->d_sub_items, sizeof(RctItem) * N));
   }
   p->scratch_floats = scratch;
   if (scratch > 0) PLAN_HIP(hipMalloc(&p->d_scratch, sizeof(float) * scratch));
-  PLAN_HIP(hipMalloc(&p->d_queue, sizeof(int32_t) * 8));
-  PLAN_HIP(hipMemsetAsync(p->d_queue, 0, sizeof(int32_t) * 8, ctx->stream));
+  PLAN_HIP(hipMalloc(&p->d_queue, sizeof(int32_t) * 16));
+  PLAN_HIP(hipMemsetAsync(p->d_queue, 0, sizeof(int32_t) * 16, ctx->stream));
   PLAN_HIP(hipStreamSynchronize(ctx->stream));
   const int ncu = std::max(1, ctx->num_cus);
   const int g1 = p->tuning.grid_per_cu;
   p->grid = std::max(1, (g1 > 0 ? g1 : blocks_per_cu(pick_stream(W, comp))) * ncu);
+  for (int b = 0; b < nb; ++b)
+    p->bin_grid[b] =
+        std::max(1, (g1 > 0 ? g1 : blocks_per_cu(pick_stream(p->bin_ring[b], comp))) * ncu);
   p->grid_group = std::max(1, (g1 > 0 ? g1 : blocks_per_cu(pick_group(G, comp))) * ncu);
 #undef PLAN_HIP
   *out = p;
@@ -2280,9 +2320,11 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
 
 extern "C" otg_status otg_rct_plan_destroy(otg_rct_plan* p) {
   if (!p) return OTG_OK;
-  if (p->d_items) hipFree(p->d_items);
+  for (int b = 0; b < otg_rct_plan::kMaxBins; ++b) {
+    if (p->d_bin_items[b]) hipFree(p->d_bin_items[b]);
+    if (p->d_bin_sub[b]) hipFree(p->d_bin_sub[b]);
+  }
   if (p->d_gitems) hipFree(p->d_gitems);
-  if (p->d_sub_items) hipFree(p->d_sub_items);
   if (p->d_groups) hipFree(p->d_groups);
   if (p->d_sub_groups) hipFree(p->d_sub_groups);
   if (p->d_sched) hipFree(p->d_sched);

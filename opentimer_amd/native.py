@@ -62,7 +62,7 @@ class RctResult(C.Structure):
 
 
 class RctTuning(C.Structure):
-    _fields_ = [("ring_slots", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("ring_slots", C.c_int32), ("min_ring", C.c_int32),
                 ("small_target", C.c_int32), ("grid_per_cu", C.c_int32)]
 
 
