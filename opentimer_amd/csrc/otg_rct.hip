@@ -85,7 +85,13 @@ __global__ __launch_bounds__(256) void rct_plain_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Double-float helpers (compensated mode).
+// Double-float helpers (compensated mode).  The compensated sweeps keep
+// every sum (subtree sums of load and L, path sums of delay and impulse) as
+// an unevaluated hi + lo pair built with two-sum, and form each product in
+// plain FP32: all terms are non-negative, so a product's rounding stays a
+// relative error of a few units in the last place of the sums it enters,
+// while an uncompensated sum's error would grow with the number of terms
+// (chains of 10^4 nodes).
 // ---------------------------------------------------------------------------
 struct ff {
   float hi, lo;
@@ -111,15 +117,14 @@ __device__ __forceinline__ ff ff_add_f(ff a, float b) {
   ff s = two_sum(a.hi, b);
   return quick_two_sum(s.hi, s.lo + a.lo);
 }
-__device__ __forceinline__ ff ff_mul_f(ff a, float b) {
-  const float p = a.hi * b;
-  const float e = fmaf(a.hi, b, -p);
-  return quick_two_sum(p, fmaf(a.lo, b, e));
-}
-__device__ __forceinline__ ff ff_mul(ff a, ff b) {
-  const float p = a.hi * b.hi;
-  const float e = fmaf(a.hi, b.hi, -p);
-  return quick_two_sum(p, e + (a.hi * b.lo + a.lo * b.hi));
+// a + b with the rounding error of the high parts kept in the low part and
+// no renormalisation: for the k children added into one node's accumulator
+// (the low part stays within k units in the last place of the high part, so
+// its own rounding adds a relative k^2 u^2), renormalised by the next
+// ff_add_f / quick_two_sum.
+__device__ __forceinline__ ff ff_acc(ff a, ff b) {
+  const ff s = two_sum(a.hi, b.hi);
+  return {s.hi, s.lo + (a.lo + b.lo)};
 }
 
 __device__ __forceinline__ void unpack(float4 v, float (&a)[4]) {
@@ -434,12 +439,13 @@ __device__ __forceinline__ void bu_window(const ItemCtx<W, COMP>& X, const Ring<
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if constexpr (COMP) {
-          const ff t = ff_add(two_sum(cap4[c], xa[k][c]), ff{xal[k][c], 0.f});
+          // compensated sums, FP32 products
+          const ff t = ff_add_f(ff{xa[k][c], xal[k][c]}, cap4[c]);
           ld[c] = t.hi;
           ldl[c] = t.lo;
           L[c] = xb[k][c];
           Ll[c] = xbl[k][c];
-          const ff q = ff_add(ff{L[c], Ll[c]}, ff_mul_f(ff_mul(t, t), in.r[k]));
+          const ff q = ff_add_f(quick_two_sum(L[c], Ll[c]), in.r[k] * t.hi * t.hi);
           w[c] = q.hi;
           wl[c] = q.lo;
         } else {
@@ -550,10 +556,11 @@ __device__ __forceinline__ void td_window(const ItemCtx<W, COMP>& X, const Ring<
         const float qd = has_par ? pd[k][c] : 0.f, qdl = has_par ? pdl[k][c] : 0.f;
         const float qm = has_par ? pm[k][c] : 0.f, qml = has_par ? pml[k][c] : 0.f;
         if constexpr (COMP) {
-          const ff uu = ff_mul_f(ff{ld[k][c], ldl[k][c]}, r);
-          const ff zz = ff_add(ff_mul(uu, uu), ff_mul_f(ff{L[k][c], Ll[k][c]}, 2.0f * r));
-          const ff dd = ff_add(ff{qd, qdl}, uu);
-          const ff mm = ff_add(ff{qm, qml}, zz);
+          // FP32 products of the rounded load and L, compensated path sums
+          const float u = r * (ld[k][c] + ldl[k][c]);
+          const float z = u * u + 2.0f * r * (L[k][c] + Ll[k][c]);
+          const ff dd = ff_add_f(ff{qd, qdl}, u);
+          const ff mm = ff_add_f(ff{qm, qml}, z);
           d[c] = dd.hi;
           dl[c] = dd.lo;
           m[c] = mm.hi;
@@ -970,13 +977,17 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if constexpr (COMP) {
+          // sums compensated, products in FP32 (every term is positive, so
+          // a product's rounding stays a relative error of a few units in
+          // the last place of the sum; an uncompensated sum's error would
+          // grow with the number of terms)
           const ff tt = ff_add_f(ff{xa[c], xal[c]}, cap4[c]);
           ld[c] = tt.hi;
           ldl[c] = tt.lo;
-          const ff q = ff_add(ff{xb[c], xbl[c]}, ff_mul_f(ff_mul(tt, tt), r));
+          const ff q = ff_add_f(quick_two_sum(xb[c], xbl[c]), r * tt.hi * tt.hi);
           w[c] = q.hi;
           wl[c] = q.lo;
-          ldo[c] = ld[c] + ldl[c];
+          ldo[c] = tt.hi;
           Lo[c] = xb[c] + xbl[c];
         } else {
           ld[c] = cap4[c] + xa[c];
@@ -999,8 +1010,8 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           if constexpr (COMP) {
-            const ff sa = ff_add(ff{pa[c], pal[c]}, ff{ld[c], ldl[c]});
-            const ff sb = ff_add(ff{pb[c], pbl[c]}, ff{w[c], wl[c]});
+            const ff sa = ff_acc(ff{pa[c], pal[c]}, ff{ld[c], ldl[c]});
+            const ff sb = ff_acc(ff{pb[c], pbl[c]}, ff{w[c], wl[c]});
             pa[c] = sa.hi;
             pal[c] = sa.lo;
             pb[c] = sb.hi;
@@ -1060,27 +1071,19 @@ __device__ __forceinline__ void node_outputs(const float (&d)[4], const float (&
 // compensated mode.  They depend on the node alone, so a window computes
 // them once for all its lanes before its depth steps.
 struct TdIncr {
-  float u[4], ul[4], z[4], zl[4];
+  float u[4], z[4];
 };
 
+// FP32 products in both modes (positive terms: a few units in the last
+// place); the compensated mode's compensation is in the path sums of
+// td_accum.
 template <bool COMP>
 __device__ __forceinline__ TdIncr td_incr(float rr, const float (&ld)[4], const float (&L)[4]) {
   TdIncr t;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    if constexpr (COMP) {
-      const ff uu = ff_mul_f(ff{ld[c], 0.f}, rr);
-      const ff zz = ff_add(ff_mul(uu, uu), ff_mul_f(ff{L[c], 0.f}, 2.0f * rr));
-      t.u[c] = uu.hi;
-      t.ul[c] = uu.lo;
-      t.z[c] = zz.hi;
-      t.zl[c] = zz.lo;
-    } else {
-      const float u = rr * ld[c];
-      t.u[c] = u;
-      t.z[c] = u * u + 2.0f * rr * L[c];
-      t.ul[c] = t.zl[c] = 0.f;
-    }
+    t.u[c] = rr * ld[c];
+    t.z[c] = t.u[c] * t.u[c] + 2.0f * rr * L[c];
   }
   return t;
 }
@@ -1094,8 +1097,8 @@ __device__ __forceinline__ void td_accum(const TdIncr& t, const float (&pd)[4],
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     if constexpr (COMP) {
-      const ff dd = ff_add(ff{pd[c], pdl[c]}, ff{t.u[c], t.ul[c]});
-      const ff mm = ff_add(ff{pm[c], pml[c]}, ff{t.z[c], t.zl[c]});
+      const ff dd = ff_add_f(ff{pd[c], pdl[c]}, t.u[c]);
+      const ff mm = ff_add_f(ff{pm[c], pml[c]}, t.z[c]);
       d[c] = dd.hi;
       dl[c] = dd.lo;
       m[c] = mm.hi;
@@ -1606,8 +1609,8 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             if constexpr (COMP) {
-              ld[c] = ff_add(ld[c], ff{a1[c], a3[c]});
-              sw[c] = ff_add(sw[c], ff{a2[c], a4[c]});
+              ld[c] = ff_acc(ld[c], ff{a1[c], a3[c]});
+              sw[c] = ff_acc(sw[c], ff{a2[c], a4[c]});
             } else {
               ld[c].hi += a1[c];
               sw[c].hi += a2[c];
@@ -1619,12 +1622,16 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if constexpr (COMP) {
-          const ff q2 = ff_add(sw[c], ff_mul_f(ff_mul(ld[c], ld[c]), r));
-          o_ld[c] = ld[c].hi;
-          o_ldl[c] = ld[c].lo;
+          // children accumulated unnormalised: renormalise, then the
+          // node's own term (FP32 product, compensated sum)
+          const ff l2 = quick_two_sum(ld[c].hi, ld[c].lo);
+          const ff s2 = quick_two_sum(sw[c].hi, sw[c].lo);
+          const ff q2 = ff_add_f(s2, r * l2.hi * l2.hi);
+          o_ld[c] = l2.hi;
+          o_ldl[c] = l2.lo;
           o_w[c] = q2.hi;
           o_wl[c] = q2.lo;
-          o_L[c] = sw[c].hi + sw[c].lo;
+          o_L[c] = s2.hi;
         } else {
           o_ld[c] = ld[c].hi;
           o_w[c] = sw[c].hi + r * ld[c].hi * ld[c].hi;
