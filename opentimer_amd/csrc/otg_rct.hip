@@ -1559,7 +1559,7 @@ __device__ __forceinline__ void group_stage(GroupLds<G, COMP>& S, const Item& it
 // Both sweeps of a staged group (LDS only).
 template <int G, bool COMP>
 __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& it, int lane) {
-  constexpr int kPull = 4;  // children fetched per batch
+  constexpr int kPull = 1;  // children fetched per batch (a step loops to its largest child count)
   const int n = it.pe - it.pb;
   // ---- bottom-up, deepest schedule entries first ------------------------------
   uint32_t pwj = n - kWave + lane >= 0 ? S.pw[n - kWave + lane] : 0u;
