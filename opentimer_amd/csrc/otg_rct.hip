@@ -1053,8 +1053,11 @@ __device__ __forceinline__ void node_outputs(const float (&d)[4], const float (&
       o_m[c] = (float)md;
       o_ldel[c] = (float)(dd * ldd + (double)L[c]);
       o_b[c] = (float)(0.5 * (dd * dd + md));
-      const double q = sqrt((double)sv[c] * (double)sv[c] + md);
-      o_s[c] = (float)(sv[c] < 0.f ? -q : q);
+      // the radicand is formed in FP64 from the compensated impulse; its
+      // square root is taken in FP32 (a square root halves the relative
+      // error of the rounded radicand)
+      const float q = sqrtf((float)((double)sv[c] * (double)sv[c] + md));
+      o_s[c] = sv[c] < 0.f ? -q : q;
     } else {
       o_d[c] = d[c];
       o_m[c] = m[c];
