@@ -309,7 +309,7 @@ otg_status otg_incr_destroy(otg_incr* incr);
  * arcs (the FPROP_CAND set, timer.cpp:845-923) and its pins are levelized
  * sink-first among themselves (toposort semantics of toposort.cu:4-72 on the
  * candidate subgraph, timer.cpp:1011-1071).  A fixed sequence of
- * 2*max_levels+7 kernels with no host round trip: capturable in a hipGraph.
+ * 2*max_levels+8 kernels with no host round trip: capturable in a hipGraph.
  * Asynchronous on `stream`. */
 otg_status otg_incr_relevelize(otg_incr* incr, const int32_t* net_ids, int32_t count,
                                otg_stream stream);
