@@ -47,7 +47,9 @@ def parse():
     ap.add_argument("--tuning", default=os.path.join(ROOT, "opentimer_amd", "tuning.json"))
     ap.add_argument("--only", default="",
                     help="rct | plain | levelize | copy (profiling runs) | precision (cfg3 report) "
-                         "| incremental (cfg4 report)")
+                         "| incremental (cfg4 report) | spef (SPEF loader + GPU solve)")
+    ap.add_argument("--spef-nets", type=int, default=400_000,
+                    help="nets of the cfg1 distribution written as SPEF for --only spef")
     ap.add_argument("--precision-nets", type=int, default=20_000,
                     help="nets of the cfg3 distribution for --only precision")
     return ap.parse_args()
@@ -300,6 +302,94 @@ def run_incremental(ctx, method, pins=5_000_000, nets=1_000_000):
     return res
 
 
+def run_spef(ctx, nets, method):
+    """SPEF -> flattened forest (host loader, all host threads) -> GPU BFS
+    ordering -> solve.  The SPEF is written from a cfg1-distribution forest
+    (per-node caps equal on the four corners, as SPEF has them), so the loaded
+    forest must equal the generated one net for net; the GPU results are
+    compared with the oracle on a sample of nets."""
+    import tempfile
+
+    import torch
+
+    from opentimer_amd import workloads as W
+    from opentimer_amd.rct import RcSolver
+    from opentimer_amd.spef import SpefForest, write
+
+    g = W.generate_rc(W.config1(num_nets=nets, seed=4), nthreads=16)
+    c4 = g.node_cap.reshape(-1, 4)
+    c4[:] = c4[:, :1]
+    tmpdir = tempfile.mkdtemp(prefix="otg_spef_")
+    path = os.path.join(tmpdir, "cfg1.spef")
+    try:
+        t0 = time.time()
+        write(path, g)
+        wr_s = time.time() - t0
+        size = os.path.getsize(path)
+        log(f"spef: wrote {size/1e6:.0f} MB ({nets} nets, {g.num_nodes} nodes) in {wr_s:.1f}s")
+        threads = min(16, os.cpu_count() or 1)
+        runs = []
+        f = None
+        for _ in range(3):
+            if f is not None:
+                f.close()
+            t0 = time.time()
+            f = SpefForest.read(path, threads=threads)
+            runs.append(time.time() - t0)
+        parse_s = float(np.median(runs))
+        same = (np.array_equal(f.nodes_start, g.nodes_start) and np.array_equal(f.edge_res, g.edge_res)
+                and np.array_equal(np.sort(f.node_cap[0::4]), np.sort(g.node_cap[0::4])))
+        # GPU: upload + BFS ordering, then one solve
+        torch.cuda.synchronize()
+        t0 = time.time()
+        s = RcSolver.from_edges(f, ctx)
+        torch.cuda.synchronize()
+        bfs_s = time.time() - t0
+        s.solve(method)
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(5):
+            s.solve(method)
+        b.record()
+        torch.cuda.synchronize()
+        solve_ms = a.elapsed_time(b) / 5
+        # parity on a sample: the loaded forest's GPU results vs the oracle
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        topo = s.host_topology()
+        ns = topo["nodes_start"]
+        pick = np.sort(np.random.default_rng(1).choice(f.num_nets, min(2000, f.num_nets),
+                                                       replace=False))
+        idx = np.concatenate([np.arange(4 * ns[k], 4 * ns[k + 1]) for k in pick])
+        nodes = np.concatenate([np.arange(ns[k], ns[k + 1]) for k in pick])
+        sub = np.concatenate([[0], np.cumsum(np.diff(ns)[pick])]).astype(np.int32)
+        n4 = np.concatenate([np.arange(4 * k, 4 * k + 4) for k in pick])
+        r64 = O.rct_all(sub, topo["pid"][nodes], topo["pres"][nodes], topo["cap"][idx],
+                        topo["slew_in"][n4], "f64")
+        out = s.host()
+        err = {}
+        for k in ("load", "delay", "slew"):
+            r = r64[k].astype(np.float64)
+            fl = max(float(np.max(np.abs(r))) * 1e-7, 1e-30)
+            err[k] = float(np.max(np.abs(out[k][idx] - r) / np.maximum(np.abs(r), fl)))
+        res = {"only": "spef", "nets": int(f.num_nets), "nodes": int(f.num_nodes),
+               "file_MB": size / 1e6, "parse_threads": threads, "parse_s": parse_s,
+               "parse_MB_per_s": size / 1e6 / parse_s, "parse_nets_per_s": f.num_nets / parse_s,
+               "loaded_equals_written": bool(same), "gpu_upload_and_bfs_s": bfs_s,
+               "solve_method": method, "solve_ms": solve_ms,
+               "rel_err_vs_fp64_sample": err, "undriven_nets": int(f.num_undriven)}
+        log(f"spef: {res}")
+        return res
+    finally:
+        try:
+            os.remove(path)
+            os.rmdir(tmpdir)
+        except OSError:
+            pass
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -329,6 +419,9 @@ def main():
         return
     if args.only == "incremental":
         print(json.dumps(run_incremental(ctx, args.method)))
+        return
+    if args.only == "spef":
+        print(json.dumps(run_spef(ctx, args.spef_nets, args.method)))
         return
     if args.only == "copy":
         # a plain device copy of known size: the yardstick that converts the

@@ -1,4 +1,5 @@
-"""ctypes binding of libotg.so (include/otg.h, include/otg_gen.h).
+"""ctypes binding of libotg.so (include/otg.h, include/otg_gen.h,
+include/otg_spef.h).
 
 The library is the product: every compute call goes to its HIP kernels.  If
 the library is missing this module raises at import time -- there is no CPU
@@ -118,6 +119,20 @@ class GenDagParams(C.Structure):
                 ("pad", C.c_int32), ("skip_prob", C.c_double), ("shuffle_ids", C.c_int32)]
 
 
+class SpefOptions(C.Structure):
+    _fields_ = [("cap_unit", C.c_double), ("res_unit", C.c_double), ("threads", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class SpefForest(C.Structure):
+    _fields_ = [("num_nets", C.c_int32), ("total_num_nodes", C.c_int32),
+                ("total_num_edges", C.c_int32), ("num_undriven", C.c_int32),
+                ("num_dropped_caps", C.c_int32), ("reserved", C.c_int32),
+                ("nodes_start", vp), ("edges", vp), ("roots", vp), ("edges_res", vp),
+                ("nodes_cap", vp), ("total_cap", vp), ("cap_unit", C.c_double),
+                ("res_unit", C.c_double)]
+
+
 def _sig(name, res, *args):
     f = getattr(lib, name)
     f.restype = res
@@ -164,6 +179,19 @@ _sig("otg_gen_rc_fill", C.c_int, C.POINTER(GenRcParams), vp, vp, vp, vp, vp, vp,
 _sig("otg_gen_dag", C.c_int, C.POINTER(GenDagParams), vp, vp, C.c_int)
 _sig("otg_build_fanin_csr", C.c_int, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, i32p,
      i64p)
+spef_p = C.c_void_p
+_sig("otg_spef_read", C.c_int, C.c_char_p, C.POINTER(SpefOptions), C.POINTER(spef_p))
+_sig("otg_spef_read_buffer", C.c_int, C.c_char_p, C.c_size_t, C.POINTER(SpefOptions),
+     C.POINTER(spef_p))
+_sig("otg_spef_last_error", C.c_char_p)
+_sig("otg_spef_forest_get", C.c_int, spef_p, C.POINTER(SpefForest))
+_sig("otg_spef_net_name", C.c_int32, spef_p, C.c_int32, C.c_char_p, C.c_size_t)
+_sig("otg_spef_node_name", C.c_int32, spef_p, C.c_int32, C.c_int32, C.c_char_p, C.c_size_t)
+_sig("otg_spef_find_net", C.c_int32, spef_p, C.c_char_p)
+_sig("otg_spef_find_node", C.c_int32, spef_p, C.c_int32, C.c_char_p)
+_sig("otg_spef_add_pin_cap", C.c_int, spef_p, C.c_int32, C.c_int32, f32p)
+_sig("otg_spef_write", C.c_int, C.c_char_p, C.c_int32, vp, vp, vp, vp, vp)
+_sig("otg_spef_destroy", C.c_int, spef_p)
 
 # Every symbol the C headers declare; tests check the library exports each.
 EXPORTED = [
@@ -175,6 +203,9 @@ EXPORTED = [
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",
     "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
     "otg_gen_rc_sizes", "otg_gen_rc_fill", "otg_gen_dag", "otg_build_fanin_csr",
+    "otg_spef_read", "otg_spef_read_buffer", "otg_spef_last_error", "otg_spef_forest_get",
+    "otg_spef_net_name", "otg_spef_node_name", "otg_spef_find_net", "otg_spef_find_node",
+    "otg_spef_add_pin_cap", "otg_spef_write", "otg_spef_destroy",
 ]
 
 
