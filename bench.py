@@ -374,12 +374,27 @@ def run_spef(ctx, nets, method):
             r = r64[k].astype(np.float64)
             fl = max(float(np.max(np.abs(r))) * 1e-7, 1e-30)
             err[k] = float(np.max(np.abs(out[k][idx] - r) / np.maximum(np.abs(r), fl)))
+        # pin queries (otg_rct_query): 1M random (net, node) pins
+        rng = np.random.default_rng(2)
+        qn = rng.integers(0, f.num_nets, 1_000_000).astype(np.int32)
+        ql = (rng.random(1_000_000) * np.diff(ns)[qn]).astype(np.int32)
+        qn_t = torch.from_numpy(qn).to(s.device)
+        ql_t = torch.from_numpy(ql).to(s.device)
+        s.query(qn_t, ql_t)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(10):
+            s.query(qn_t, ql_t)
+        b.record()
+        torch.cuda.synchronize()
+        query_ms = a.elapsed_time(b) / 10
         res = {"only": "spef", "nets": int(f.num_nets), "nodes": int(f.num_nodes),
                "file_MB": size / 1e6, "parse_threads": threads, "parse_s": parse_s,
                "parse_MB_per_s": size / 1e6 / parse_s, "parse_nets_per_s": f.num_nets / parse_s,
                "loaded_equals_written": bool(same), "gpu_upload_and_bfs_s": bfs_s,
                "solve_method": method, "solve_ms": solve_ms,
-               "rel_err_vs_fp64_sample": err, "undriven_nets": int(f.num_undriven)}
+               "rel_err_vs_fp64_sample": err, "undriven_nets": int(f.num_undriven),
+               "query_1M_pins_ms": query_ms}
         log(f"spef: {res}")
         return res
     finally:

@@ -183,6 +183,20 @@ otg_status otg_rct_bfs(otg_context* ctx, const otg_rct_edges* in, int32_t* node2
                        int32_t* pid, int32_t* edges_directed, float* pres, float* cap_bfs,
                        otg_stream stream);
 
+/* Timing at pins from a solve's device results: the lookups the reference's
+ * timer makes on the flat RC storage, FlatRct::delay / FlatRct::slew
+ * (ot/timer/net.cpp:280-287, result[(arr_start + node2bfs[arr_start + id])*4
+ * + c]) and Net::_load (net.cpp:626-642, the driver's load), for a batch of
+ * pins.  Pin i is node local[i] (original numbering) of net[i]; outputs
+ * [4*count] per corner (c = el*2 + rf): delay and slew at the pin, load of
+ * its net.  Any output may be NULL.  A pin with a net or node index out of
+ * range gets NaN.  All pointers are device pointers; asynchronous on
+ * `stream`. */
+otg_status otg_rct_query(otg_context* ctx, int32_t num_nets, const int32_t* nodes_start,
+                         const int32_t* node2bfs, const otg_rct_result* res, int32_t count,
+                         const int32_t* net, const int32_t* local, float* delay, float* slew,
+                         float* load, otg_stream stream);
+
 /* Host-pointer drop-in for rct_bfs_and_compute_cuda(RctEdgeArrayCUDA)
  * (reference ot/cuda/rct.cuh:45, rct.cu:361-397).  Same field meaning as
  * RctEdgeArrayCUDA (rct.cuh:20-42); rct_edges is oriented in place;

@@ -191,6 +191,57 @@ extern "C" otg_status otg_rct_bfs(otg_context* ctx, const otg_rct_edges* in, int
 }
 
 namespace {
+// One thread per (pin, corner): four consecutive lanes read the four corners
+// of one node, 16 contiguous bytes.
+__global__ void rct_query_kernel(int32_t num_nets, const int32_t* __restrict__ nodes_start,
+                                 const int32_t* __restrict__ node2bfs, const float* __restrict__ d,
+                                 const float* __restrict__ sl, const float* __restrict__ ld,
+                                 int32_t count, const int32_t* __restrict__ net,
+                                 const int32_t* __restrict__ local, float* delay, float* slew,
+                                 float* load) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4 * (int64_t)count) return;
+  const int64_t q = i >> 2;
+  const int c = (int)(i & 3);
+  const int32_t k = net[q];
+  float vd = __builtin_nanf(""), vs = vd, vl = vd;
+  if (k >= 0 && k < num_nets) {
+    const int32_t b = nodes_start[k], n = nodes_start[k + 1] - b, u = local[q];
+    if (u >= 0 && u < n) {
+      const int64_t pos = (int64_t)b + node2bfs[(int64_t)b + u];
+      if (d) vd = d[4 * pos + c];
+      if (sl) vs = sl[4 * pos + c];
+      if (ld) vl = ld[4 * (int64_t)b + c];  // the driver is BFS position 0
+    }
+  }
+  if (delay) delay[i] = vd;
+  if (slew) slew[i] = vs;
+  if (load) load[i] = vl;
+}
+}  // namespace
+
+extern "C" otg_status otg_rct_query(otg_context* ctx, int32_t num_nets, const int32_t* nodes_start,
+                                    const int32_t* node2bfs, const otg_rct_result* res,
+                                    int32_t count, const int32_t* net, const int32_t* local,
+                                    float* delay, float* slew, float* load, otg_stream stream) {
+  if (!ctx || !res || count < 0 || num_nets < 0) return OTG_ERR_INVALID_ARGUMENT;
+  if (count == 0) return OTG_OK;
+  if (!nodes_start || !node2bfs || !net || !local) return OTG_ERR_INVALID_ARGUMENT;
+  if ((delay && !res->delay) || (slew && !res->slew) || (load && !res->load)) {
+    OTG_SET_ERR(ctx, "query asks for an output the result does not hold");
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t threads = 4 * (int64_t)count;
+  hipLaunchKernelGGL(rct_query_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
+                     num_nets, nodes_start, node2bfs, res->delay, res->slew, res->load, count,
+                     net, local, delay, slew, load);
+  OTG_HIP(ctx, hipGetLastError());
+  return OTG_OK;
+}
+
+namespace {
 // RAII list of device buffers for the host-pointer entry points
 struct DevBufs {
   std::vector<void*> p;

@@ -158,6 +158,8 @@ _sig("otg_rct_solve_subset", C.c_int, ctx_p, plan_p, C.POINTER(RctTree), C.POINT
 _sig("otg_rct_bfs", C.c_int, ctx_p, C.POINTER(RctEdges), vp, vp, vp, vp, vp, vp)
 _sig("otg_rct_bfs_and_compute", C.c_int, ctx_p, C.POINTER(RctEdgeArray), C.c_int)
 _sig("otg_rct_compute", C.c_int, ctx_p, C.POINTER(RctHost), C.c_int)
+_sig("otg_rct_query", C.c_int, ctx_p, C.c_int32, vp, vp, C.POINTER(RctResult), C.c_int32, vp, vp,
+     vp, vp, vp, vp)
 _sig("otg_rct_autotune", C.c_int, ctx_p, C.POINTER(RctTree), C.POINTER(RctResult), C.c_int,
      C.c_char_p, C.POINTER(RctTuning))
 _sig("otg_rct_tuning_load", C.c_int, ctx_p, C.c_char_p, C.POINTER(RctTuning))
@@ -198,7 +200,7 @@ EXPORTED = [
     "otg_create", "otg_destroy", "otg_synchronize", "otg_status_string", "otg_last_error",
     "otg_version", "otg_rct_plan_create", "otg_rct_plan_destroy", "otg_rct_plan_stats",
     "otg_rct_solve", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
-    "otg_rct_compute",
+    "otg_rct_compute", "otg_rct_query",
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize",
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",
     "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",

@@ -154,6 +154,22 @@ class RcSolver:
                     slew_in=self.slew_in.cpu().numpy(),
                     node2bfs=self.node2bfs[:V].cpu().numpy() if hasattr(self, "node2bfs") else None)
 
+    def query(self, net_ids: torch.Tensor, local_ids: torch.Tensor, stream=None):
+        """delay, slew and net load [count, 4] at pins (net, original local id)
+        from the last solve (otg_rct_query; FlatRct::delay/slew and
+        Net::_load, ot/timer/net.cpp:280-287,626-642)."""
+        stream = torch_stream(self.device) if stream is None else stream
+        n = int(net_ids.numel())
+        out = {k: torch.empty(max(4 * n, 1), dtype=torch.float32, device=self.device)
+               for k in ("delay", "slew", "load")}
+        res = self.result()
+        self.ctx.check(N.lib.otg_rct_query(self.ctx.ptr, self.num_nets, self.nodes_start.data_ptr(),
+                                           self.node2bfs.data_ptr(), C.byref(res), n,
+                                           net_ids.data_ptr(), local_ids.data_ptr(),
+                                           out["delay"].data_ptr(), out["slew"].data_ptr(),
+                                           out["load"].data_ptr(), stream))
+        return {k: v[: 4 * n].view(n, 4) for k, v in out.items()}
+
     def close(self):
         for p in self.plans.values():
             N.lib.otg_rct_plan_destroy(p)
