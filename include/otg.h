@@ -109,14 +109,16 @@ typedef struct otg_rct_result {
 /* Tuning knobs of the batched launch; zero fields take the defaults (or the
  * values persisted by the autotuner, see otg_rct_autotune). */
 typedef struct otg_rct_tuning {
-  int32_t ring_slots;     /* largest on-chip ring per wavefront: 256, 512 or */
-                          /* 1024; nets with parent offsets above ring-64  */
-                          /* take the slower global-window path            */
-  int32_t min_ring;       /* smallest ring (256..ring_slots): nets go to the */
-                          /* smallest ring, 256/512/1024, from min_ring up,  */
-                          /* that holds their parent offsets; a smaller ring */
-                          /* fits more wavefronts per CU (default: 256 for   */
-                          /* the compensated method, ring_slots otherwise)   */
+  int32_t ring_slots;     /* largest on-chip ring per wavefront: 256, 512,   */
+                          /* 1024 or 2048 (default); nets with parent      */
+                          /* offsets above ring-64 take the slower         */
+                          /* global-window path                            */
+  int32_t min_ring;       /* smallest ring (same sizes, <= ring_slots): a    */
+                          /* net goes to the smallest ring from min_ring up  */
+                          /* that holds its parent offsets, one launch per   */
+                          /* ring size holding nets; a smaller ring fits     */
+                          /* more wavefronts per CU (default: 256 for the    */
+                          /* compensated method, 512 for FP32)               */
   int32_t small_target;   /* node budget of one packed group (64..256)      */
   int32_t grid_per_cu;    /* persistent workgroups per CU (0 = occupancy)   */
 } otg_rct_tuning;

@@ -503,23 +503,19 @@ extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tre
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   std::vector<otg_rct_tuning> cands;
-  // largest and smallest ring size (LDS per wavefront, hence occupancy,
-  // against the nets that fall back to the global-window path and the
-  // launches per ring size), packed-group size, and the number of persistent
-  // workgroups per CU (0: as many as fit)
-  for (int W : {512, 256, 1024})
-    for (int lo : {256, 512, 1024}) {
-      if (lo > W) continue;
-      for (int target : {256, 128})
-        for (int g : {0, 4}) {
-          otg_rct_tuning t{};
-          t.ring_slots = W;
-          t.min_ring = lo;
-          t.small_target = target;
-          t.grid_per_cu = g;
-          cands.push_back(t);
-        }
-    }
+  // smallest ring size (LDS per wavefront, hence occupancy, against one
+  // more launch), packed-group size, and the number of persistent
+  // workgroups per CU (0: as many as fit); rings up to 2048 slots
+  for (int lo : {256, 512, 1024})
+    for (int target : {256, 128})
+      for (int g : {0, 4}) {
+        otg_rct_tuning t{};
+        t.ring_slots = 2048;
+        t.min_ring = lo;
+        t.small_target = target;
+        t.grid_per_cu = g;
+        cands.push_back(t);
+      }
   hipEvent_t e0, e1;
   OTG_HIP(ctx, hipEventCreate(&e0));
   OTG_HIP(ctx, hipEventCreate(&e1));

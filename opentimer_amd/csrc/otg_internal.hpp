@@ -44,13 +44,14 @@ struct otg_rct_plan {
   // a net goes to the first bin whose ring holds its parent offsets
   // (maxpid <= ring - 64).  A smaller ring needs less LDS, so more
   // workgroups fit on a CU.
-  static constexpr int kMaxBins = 3;
+  static constexpr int kMaxBins = 4;          // rings of 256, 512, 1024, 2048 slots
+  static constexpr int kQueueInts = 4 * kMaxBins + 2;
   int num_bins = 0;
-  int bin_ring[kMaxBins] = {0, 0, 0};
-  int32_t bin_items[kMaxBins] = {0, 0, 0};     // nets in the bin
+  int bin_ring[kMaxBins] = {};
+  int32_t bin_items[kMaxBins] = {};            // nets in the bin
   RctItem* d_bin_items[kMaxBins] = {};         // largest first
   RctItem* d_bin_sub[kMaxBins] = {};           // subset solves: the bin's listed nets
-  int bin_grid[kMaxBins] = {0, 0, 0};          // persistent workgroups
+  int bin_grid[kMaxBins] = {};                 // persistent workgroups
   int32_t num_groups = 0;
   int32_t group_cap = 256;         // largest net of a packed group
   RctItem* d_groups = nullptr;     // packed groups (k: group index, scratch: schedule offset)
@@ -61,9 +62,10 @@ struct otg_rct_plan {
   int32_t* d_sub_count = nullptr;  // number of them
   int32_t num_gitems = 0;
   RctItem* d_gitems = nullptr;     // global-window nets
-  int32_t* d_queue = nullptr;      // [16]: next item and finished workgroups of streaming
-                                   // bin b [2b, 2b+1] and of the global-window kernel
-                                   // [6, 7]; [8 + b]: bin b's nets of a subset solve
+  int32_t* d_queue = nullptr;      // [kQueueInts]: next item and finished workgroups of
+                                   // streaming bin b [2b, 2b+1] and of the global-window
+                                   // kernel [2B, 2B+1] (B = kMaxBins); [2B + 2 + b]: bin
+                                   // b's nets of a subset solve
   float* d_scratch = nullptr;      // window storage of stream/global items
   int64_t scratch_floats = 0;
   int32_t* d_net_maxpid = nullptr;  // per net, for subset solves

@@ -1815,8 +1815,8 @@ inline int group_cap(int small_target) { return small_target <= 128 ? 128 : 256;
 // solves, appended to the list of the first bin whose ring holds their
 // parent offsets (lim[b] = ring - 64; counts[b] counts bin b's list).
 struct SubBins {
-  RctItem* items[3];
-  int32_t lim[3];
+  RctItem* items[otg_rct_plan::kMaxBins];
+  int32_t lim[otg_rct_plan::kMaxBins];
   int32_t num_bins;
 };
 
@@ -1872,15 +1872,18 @@ void* stream_kernel_ptr() {
   return reinterpret_cast<void*>(&rct_stream_kernel<W, COMP>);
 }
 
-void* pick_stream(int W, bool comp) {
-  if (comp) {
-    if (W == 256) return stream_kernel_ptr<256, true>();
-    if (W == 1024) return stream_kernel_ptr<1024, true>();
-    return stream_kernel_ptr<512, true>();
+template <bool COMP>
+void* pick_stream_t(int W) {
+  switch (W) {
+    case 256: return stream_kernel_ptr<256, COMP>();
+    case 1024: return stream_kernel_ptr<1024, COMP>();
+    case 2048: return stream_kernel_ptr<2048, COMP>();
+    default: return stream_kernel_ptr<512, COMP>();
   }
-  if (W == 256) return stream_kernel_ptr<256, false>();
-  if (W == 1024) return stream_kernel_ptr<1024, false>();
-  return stream_kernel_ptr<512, false>();
+}
+
+void* pick_stream(int W, bool comp) {
+  return comp ? pick_stream_t<true>(W) : pick_stream_t<false>(W);
 }
 
 void* pick_group(int G, bool comp) {
@@ -1955,8 +1958,11 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   otg_rct_tuning r{};
   if (t) r = *t;
   const otg_rct_tuning& d = ctx->tuning;
-  if (!r.ring_slots) r.ring_slots = d.ring_slots ? d.ring_slots : 512;
-  if (r.ring_slots != 256 && r.ring_slots != 512 && r.ring_slots != 1024) r.ring_slots = 512;
+  auto ring_ok = [](int w) { return w == 256 || w == 512 || w == 1024 || w == 2048; };
+  // the largest ring by default: a bin holding no nets costs nothing, and
+  // nets with parent offsets up to 1984 (high-fanout nets) stay on chip
+  if (!r.ring_slots) r.ring_slots = d.ring_slots ? d.ring_slots : 2048;
+  if (!ring_ok(r.ring_slots)) r.ring_slots = 2048;
   if (!r.small_target) r.small_target = d.small_target ? d.small_target : 256;
   // a packed group holds at most 256 positions (8-bit schedule fields)
   r.small_target = std::max(64, std::min(r.small_target, 256));
@@ -1965,8 +1971,8 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   // occupancy with the 512-slot ring (4 wavefronts per CU) leaves LDS
   // latency exposed: its default splits the streamed nets by ring size
   if (!r.min_ring) r.min_ring = d.min_ring ? d.min_ring
-                                           : (m == OTG_RCT_BATCHED_COMPENSATED ? 256 : r.ring_slots);
-  if (r.min_ring != 256 && r.min_ring != 512 && r.min_ring != 1024) r.min_ring = 256;
+                                           : (m == OTG_RCT_BATCHED_COMPENSATED ? 256 : 512);
+  if (!ring_ok(r.min_ring)) r.min_ring = 256;
   r.min_ring = std::min(r.min_ring, r.ring_slots);
   return r;
 }
@@ -2048,7 +2054,7 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   }
   // ---- streamed nets, one launch per ring size (largest ring first) ----------
   if (plan->num_bins > 0) {
-    int32_t* n_sub = plan->d_queue + 8;
+    int32_t* n_sub = plan->d_queue + 2 * otg_rct_plan::kMaxBins + 2;
     if (subset && plan->stats[2] > 0) {
       SubBins sb{};
       sb.num_bins = plan->num_bins;
@@ -2083,7 +2089,7 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   // ---- nets whose parent offsets exceed the ring ------------------------------
   if (plan->stats[3] > 0) {
     RctParams Q = P;
-    Q.queue = plan->d_queue + 6;
+    Q.queue = plan->d_queue + 2 * otg_rct_plan::kMaxBins;
     if (subset) {
       Q.subset = subset;
       Q.subset_count = count;
@@ -2313,8 +2319,8 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   }
   p->scratch_floats = scratch;
   if (scratch > 0) PLAN_HIP(hipMalloc(&p->d_scratch, sizeof(float) * scratch));
-  PLAN_HIP(hipMalloc(&p->d_queue, sizeof(int32_t) * 16));
-  PLAN_HIP(hipMemsetAsync(p->d_queue, 0, sizeof(int32_t) * 16, ctx->stream));
+  PLAN_HIP(hipMalloc(&p->d_queue, sizeof(int32_t) * otg_rct_plan::kQueueInts));
+  PLAN_HIP(hipMemsetAsync(p->d_queue, 0, sizeof(int32_t) * otg_rct_plan::kQueueInts, ctx->stream));
   PLAN_HIP(hipStreamSynchronize(ctx->stream));
   const int ncu = std::max(1, ctx->num_cus);
   const int g1 = p->tuning.grid_per_cu;

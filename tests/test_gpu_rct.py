@@ -273,3 +273,28 @@ def test_not_a_tree_rejected(ctx):
     with pytest.raises(N.OtgError) as ei:
         RcSolver.from_edges(f, ctx)
     assert ei.value.status == 2
+
+
+@pytest.mark.parametrize("method,tol64", [("batched", 1e-5), ("compensated", 1e-6)])
+def test_wide_nets_take_large_rings(ctx, method, tol64):
+    # balanced trees of up to 3000 nodes have BFS levels up to ~1500 wide,
+    # so their parent offsets need the 1024- and 2048-slot rings (default
+    # tuning: no net falls back to the global-window path)
+    cfg = W.RcConfig("wide", 400, [W.SizeClass(0.5, 2, 64), W.SizeClass(0.5, 600, 3000, N.SHAPE_BALANCED)],
+                     seed=13)
+    f, s = _forest(cfg, ctx)
+    h = s.host_topology()
+    st = s.plan_stats(method)
+    assert st["global_nets"] == 0 and st["stream_nets"] > 0
+    ns, pid = h["nodes_start"], h["pid"]
+    maxpid = max(int(pid[ns[k]:ns[k + 1]].max()) for k in range(f.num_nets))
+    assert maxpid > 960  # beyond the 1024-slot ring: the 2048 ring is used
+    s.solve(method)
+    g = s.host()
+    r32 = O.rct_all(ns, pid, h["pres"], h["cap"], h["slew_in"], "dfs_f32")
+    r64 = O.rct_all(ns, pid, h["pres"], h["cap"], h["slew_in"], "f64")
+    fl = floors(r64)
+    for k in ("delay", "slew"):
+        assert rel_err(g[k], r32[k], fl[k]) < 1e-5, k
+    for k in OUT:
+        assert rel_err(g[k], r64[k], fl[k]) < tol64, k
