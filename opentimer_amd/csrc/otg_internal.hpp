@@ -44,10 +44,12 @@ struct otg_rct_plan {
   // a net goes to the first bin whose ring holds its parent offsets
   // (maxpid <= ring - 64).  A smaller ring needs less LDS, so more
   // workgroups fit on a CU.
-  static constexpr int kMaxBins = 4;          // rings of 256, 512, 1024, 2048 slots
+  static constexpr int kMaxBins = 8;  // rings of 256, 512, 1024, 2048 slots, each for
+                                      // ordinary and (compensated) chain nets
   static constexpr int kQueueInts = 4 * kMaxBins + 2;
   int num_bins = 0;
   int bin_ring[kMaxBins] = {};
+  int bin_chain[kMaxBins] = {};                // 1: chain nets (chain-window kernel variant)
   int32_t bin_items[kMaxBins] = {};            // nets in the bin
   RctItem* d_bin_items[kMaxBins] = {};         // largest first
   RctItem* d_bin_sub[kMaxBins] = {};           // subset solves: the bin's listed nets
@@ -69,6 +71,7 @@ struct otg_rct_plan {
   float* d_scratch = nullptr;      // window storage of stream/global items
   int64_t scratch_floats = 0;
   int32_t* d_net_maxpid = nullptr;  // per net, for subset solves
+  int32_t* d_net_chain = nullptr;   // per net: 1 if at least half its windows are paths
   int64_t* d_net_scratch = nullptr; // per net scratch offset (-1: none)
   int grid = 0;                     // persistent workgroups (global-window)
   int grid_group = 0;               // persistent workgroups (packed groups)

@@ -940,6 +940,50 @@ struct SRing {
   }
 };
 
+// Chain windows (compensated mode).  When every position of a window but
+// the first has the previous position as its parent, the window is a path:
+// its in-window subtree sums are suffix sums over the lanes and its path
+// sums are prefix sums.  Both are taken as wave-wide scans of double-float
+// values (6 shuffle stages) instead of one depth step per position (64 for
+// a chain).  The compensated sums keep their accuracy in any summation
+// order; the FP32 method keeps the sequential order of the CPU timer.
+__device__ __forceinline__ bool chain_window(bool valid, int par, int i, int lane) {
+  const uint64_t vm = __ballot(valid);
+  const uint64_t cm = __ballot(valid && (lane == 0 || par == i - 1));
+  return vm != 0 && cm == vm && (vm & 1ull) && __popcll(vm) > 8;
+}
+
+__device__ __forceinline__ ff shfl_ff_down(ff v, int off) {
+  return ff{__shfl_down(v.hi, off), __shfl_down(v.lo, off)};
+}
+__device__ __forceinline__ ff shfl_ff_up(ff v, int off) {
+  return ff{__shfl_up(v.hi, off), __shfl_up(v.lo, off)};
+}
+// inclusive suffix sums over lanes (lane l: sum of lanes l..63)
+__device__ __forceinline__ void suffix_scan(ff (&v)[4], int lane) {
+  // (stages not unrolled: the scan is inlined into every window of a chunk,
+  // and unrolled stages would triple the kernel's code)
+#pragma unroll 1
+  for (int off = 1; off < kWave; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const ff o = shfl_ff_down(v[c], off);
+      if (lane + off < kWave) v[c] = ff_add(v[c], o);
+    }
+  }
+}
+// inclusive prefix sums over lanes (lane l: sum of lanes 0..l)
+__device__ __forceinline__ void prefix_scan(ff (&v)[4], int lane) {
+#pragma unroll 1
+  for (int off = 1; off < kWave; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const ff o = shfl_ff_up(v[c], off);
+      if (lane >= off) v[c] = ff_add(v[c], o);
+    }
+  }
+}
+
 // Bottom-up window: load_i = C_i + sum load_c, L_i = sum (L_c + R_c load_c^2).
 // Returns the lane's final load and L rounded to FP32 (ldo, Lo).  The slot of
 // every position of the window is cleared once at the window's end (for the
@@ -952,20 +996,62 @@ struct SRing {
 // to claim[parent slot mod 256], and the lanes that read their own id back
 // own their parent for the round.  Each round retires at least one lane, and
 // two winners never share a parent.
-template <int W, bool COMP>
+template <int W, bool COMP, bool CHAIN>
 __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int pe, int lane,
                                           int pidv, float r, float4 capv, bool root,
                                           float (&ldo)[4], float (&Lo)[4]) {
   const int i = ws + lane;
   const bool valid = i < pe;
   const int par = (valid && !root) ? i - pidv : -1;
-  int dw;
-  const int maxd = window_depth(valid, par, ws, dw);
   const int sl = R.slot(i);
   const int ps = R.slot(par);
-  const int cs = ps & (kClaim - 1);
   float cap4[4];
   unpack(capv, cap4);
+  if (CHAIN && COMP && chain_window(valid, par, i, lane)) {
+    // load_l = sum over lanes j >= l of (C_j + external children of j);
+    // w_l = sum over j >= l of (external L of j + R_j load_j^2); lane 0
+    // pushes (load, w) to its parent in an earlier window
+    float xa[4], xb[4], xal[4], xbl[4];
+    R.get(sl, xa, xb, xal, xbl);
+    ff S[4], V[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      S[c] = valid ? ff_add_f(quick_two_sum(xa[c], xal[c]), cap4[c]) : ff{0.f, 0.f};
+    suffix_scan(S, lane);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      V[c] = valid ? ff_add_f(quick_two_sum(xb[c], xbl[c]), r * S[c].hi * S[c].hi) : ff{0.f, 0.f};
+    suffix_scan(V, lane);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      ff nx = shfl_ff_down(V[c], 1);  // w of the in-window child
+      if (lane == kWave - 1) nx = ff{0.f, 0.f};
+      const ff L = ff_add(nx, ff{xb[c], xbl[c]});
+      ldo[c] = S[c].hi;
+      Lo[c] = L.hi + L.lo;
+    }
+    if (lane == 0 && par >= 0) {
+      float pa[4], pb[4], pal[4], pbl[4];
+      R.get(ps, pa, pb, pal, pbl);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const ff sa = ff_acc(ff{pa[c], pal[c]}, S[c]);
+        const ff sb = ff_acc(ff{pb[c], pbl[c]}, V[c]);
+        pa[c] = sa.hi;
+        pal[c] = sa.lo;
+        pb[c] = sb.hi;
+        pbl[c] = sb.lo;
+      }
+      R.put(ps, pa, pb, pal, pbl);
+    }
+    wave_sync();
+    R.zero(sl);
+    wave_sync();
+    return;
+  }
+  int dw;
+  const int maxd = window_depth(valid, par, ws, dw);
+  const int cs = ps & (kClaim - 1);
 #pragma unroll
   for (int c = 0; c < 4; ++c) ldo[c] = Lo[c] = 0.f;
   for (int t = maxd; t >= 0; --t) {
@@ -1117,7 +1203,7 @@ __device__ __forceinline__ void td_accum(const TdIncr& t, const float (&pd)[4],
 // Top-down window: delay_i = delay_p + R_i load_i,
 // impulse_i = impulse_p + (R_i load_i)^2 + 2 R_i L_i; then the lane's
 // outputs.  ld/L are the lane's rounded bottom-up results.
-template <int W, bool COMP>
+template <int W, bool COMP, bool CHAIN>
 __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int pe, int lane,
                                           int pidv, float r, bool root, const float (&ld)[4],
                                           const float (&L)[4], float (&o_d)[4],
@@ -1127,11 +1213,44 @@ __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int p
   const bool valid = i < pe;
   const int par = (valid && !root) ? i - pidv : -1;
   const float rr = root ? 0.f : r;
-  int dw;
-  const int maxd = window_depth(valid, par, ws, dw);
   const int sl = R.slot(i);
   const int ps = R.slot(par);
   const TdIncr inc = td_incr<COMP>(rr, ld, L);
+  if (CHAIN && COMP && chain_window(valid, par, i, lane)) {
+    // delay_l = delay of lane 0's parent + sum over lanes j <= l of u_j
+    // (impulse likewise with z)
+    ff D[4], M[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      D[c] = valid ? ff{inc.u[c], 0.f} : ff{0.f, 0.f};
+      M[c] = valid ? ff{inc.z[c], 0.f} : ff{0.f, 0.f};
+    }
+    if (lane == 0 && par >= 0) {
+      float pd[4], pm[4], pdl[4], pml[4];
+      R.get(ps, pd, pm, pdl, pml);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        D[c] = ff_add_f(ff{pd[c], pdl[c]}, inc.u[c]);
+        M[c] = ff_add_f(ff{pm[c], pml[c]}, inc.z[c]);
+      }
+    }
+    prefix_scan(D, lane);
+    prefix_scan(M, lane);
+    float d[4], dl[4], m[4], ml[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      d[c] = D[c].hi;
+      dl[c] = D[c].lo;
+      m[c] = M[c].hi;
+      ml[c] = M[c].lo;
+    }
+    if (valid) R.put(sl, d, m, dl, ml);
+    wave_sync();
+    node_outputs<COMP>(d, dl, m, ml, ld, L, siv, o_d, o_ldel, o_b, o_m, o_s);
+    return;
+  }
+  int dw;
+  const int maxd = window_depth(valid, par, ws, dw);
   float d[4], dl[4], m[4], ml[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) d[c] = dl[c] = m[c] = ml[c] = 0.f;
@@ -1168,7 +1287,7 @@ __device__ __forceinline__ OutBufs out_bufs(const RctParams& P, const Item& it) 
 // One chunk of a streamed net's bottom-up sweep (windows last to first).  If
 // `take`, `ch` becomes the next chunk `nx` (requested a chunk earlier) after
 // the chunk's wait.
-template <int W, bool COMP>
+template <int W, bool COMP, bool CHAIN>
 __device__ __forceinline__ void run_bu_chunk(StreamLds<W, COMP>& S, const RctParams& P,
                                              const Item& it, int c, Chunk& ch, bool take,
                                              const Chunk& nx, int lane) {
@@ -1181,7 +1300,7 @@ __device__ __forceinline__ void run_bu_chunk(StreamLds<W, COMP>& S, const RctPar
     // window 0 exists in every chunk; it is processed last
     if (s == 0 || q0 < n) {
       float ldo[4], Lo[4];
-      bu_window<W, COMP>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], ch.x[s], q0 + lane == 0,
+      bu_window<W, COMP, CHAIN>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], ch.x[s], q0 + lane == 0,
                          ldo, Lo);
       // chunk 0 stays in registers as the top-down sweep's first chunk
       ch.x[s] = make_float4(ldo[0], ldo[1], ldo[2], ldo[3]);
@@ -1203,7 +1322,7 @@ __device__ __forceinline__ void run_bu_chunk(StreamLds<W, COMP>& S, const RctPar
 // One chunk of a streamed net's top-down sweep (windows first to last).  The
 // last window's outputs are stored after the chunk's one wait, which sits on
 // every path; the next chunk is taken over right after that wait.
-template <int W, bool COMP>
+template <int W, bool COMP, bool CHAIN>
 __device__ __forceinline__ void run_td_chunk(StreamLds<W, COMP>& S, const RctParams& P,
                                              const Item& it, int c, Chunk& ch, bool take,
                                              const Chunk& nx, int lane) {
@@ -1224,7 +1343,7 @@ __device__ __forceinline__ void run_td_chunk(StreamLds<W, COMP>& S, const RctPar
       unpack(ch.x[s], ld);
       unpack(ch.y[s], L);
       float o_d[4], o_ldel[4], o_b[4], o_m[4], o_s[4];
-      td_window<W, COMP>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], q0 + lane == 0, ld, L,
+      td_window<W, COMP, CHAIN>(R, it.pb + q0, it.pe, lane, ch.pid[s], ch.r[s], q0 + lane == 0, ld, L,
                          o_d, o_ldel, o_b, o_m, o_s, ch.si);
       const int off = (q0 + lane) * 16;
       if (s < kChunkWin - 1 && q0 + kWave < n) {
@@ -1360,7 +1479,7 @@ __device__ __forceinline__ bool stream_first_chunk(const Item& it, int& ph, int&
   return true;
 }
 
-template <int W, bool COMP>
+template <int W, bool COMP, bool CHAIN>
 __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
   __shared__ StreamLds<W, COMP> S;
   const int lane = threadIdx.x;
@@ -1422,9 +1541,9 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
         for (int k = lane; k < W; k += kWave) R.zero(k);
         wave_sync();
       }
-      run_bu_chunk<W, COMP>(S, P, it, c, cur, take, nxt, lane);
+      run_bu_chunk<W, COMP, CHAIN>(S, P, it, c, cur, take, nxt, lane);
     } else {
-      run_td_chunk<W, COMP>(S, P, it, c, cur, take, nxt, lane);
+      run_td_chunk<W, COMP, CHAIN>(S, P, it, c, cur, take, nxt, lane);
     }
     if (take) nxt_pb = -1;
     // ---- advance ------------------------------------------------------------
@@ -1790,21 +1909,37 @@ __global__ __launch_bounds__(64) void rct_group_kernel(RctParams P) {
   if constexpr (GROUP_QUEUE) queue_release(P.queue, lane);
 }
 
-// Per-net validation and largest parent offset (plan building).
+// Per-net validation, largest parent offset and chain flag (plan building).
+// A net is a chain net when at least half of its 64-position windows are
+// paths (every position's parent is the position before it, the first one
+// excepted).
 __global__ void rct_netinfo_kernel(int32_t num_nets, const int32_t* __restrict__ nodes_start,
                                    const int32_t* __restrict__ pid, int32_t* __restrict__ maxpid,
-                                   int32_t* __restrict__ bad) {
+                                   int32_t* __restrict__ chain, int32_t* __restrict__ bad) {
   const int net = blockIdx.x * blockDim.x + threadIdx.x;
   if (net >= num_nets) return;
   const int32_t st = nodes_start[net], ed = nodes_start[net + 1];
-  int32_t m = 0;
-  bool ok = ed >= st;
+  int32_t m = 0, paths = 0, wins = 0;
+  bool ok = ed >= st, path = true;
   for (int32_t i = st + 1; i < ed; ++i) {
     const int32_t q = pid[i];
     if (q < 1 || q > i - st) ok = false;
     m = q > m ? q : m;
+    const int32_t o = (i - st) & (kWave - 1);
+    if (o == 0) {  // a window ends before i
+      paths += path;
+      ++wins;
+      path = true;
+    } else if (q != 1) {
+      path = false;
+    }
+  }
+  if (ed > st) {
+    paths += path;
+    ++wins;
   }
   maxpid[net] = m;
+  chain[net] = 2 * paths >= wins && wins > 1;
   if (!ok) atomicMin(bad, net);
 }
 
@@ -1812,11 +1947,13 @@ __global__ void rct_netinfo_kernel(int32_t num_nets, const int32_t* __restrict__
 inline int group_cap(int small_target) { return small_target <= 128 ? 128 : 256; }
 
 // Subset solves, streamed nets: the listed nets that a streaming kernel
-// solves, appended to the list of the first bin whose ring holds their
-// parent offsets (lim[b] = ring - 64; counts[b] counts bin b's list).
+// solves, appended to the list of the first bin of their kind (chain net or
+// not) whose ring holds their parent offsets (lim[b] = ring - 64; counts[b]
+// counts bin b's list).
 struct SubBins {
   RctItem* items[otg_rct_plan::kMaxBins];
   int32_t lim[otg_rct_plan::kMaxBins];
+  int32_t chain[otg_rct_plan::kMaxBins];
   int32_t num_bins;
 };
 
@@ -1824,6 +1961,7 @@ __global__ void rct_subset_items_kernel(int32_t count, const int32_t* __restrict
                                         int32_t num_nets, const int32_t* __restrict__ nodes_start,
                                         const int32_t* __restrict__ net_maxpid,
                                         const int64_t* __restrict__ net_scratch,
+                                        const int32_t* __restrict__ net_chain,
                                         int32_t group_max, SubBins bins,
                                         int32_t* __restrict__ counts) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1833,8 +1971,9 @@ __global__ void rct_subset_items_kernel(int32_t count, const int32_t* __restrict
   const int pb = nodes_start[net], pe = nodes_start[net + 1];
   if (pe - pb <= group_max) return;
   const int mp = net_maxpid[net];
+  const int ch = net_chain[net];
   for (int b = 0; b < bins.num_bins; ++b) {
-    if (mp <= bins.lim[b]) {
+    if (mp <= bins.lim[b] && bins.chain[b] == ch) {
       bins.items[b][atomicAdd(counts + b, 1)] =
           RctItem{pb, pe, net, net + 1, ITEM_STREAM, 1, net_scratch[net]};
       return;
@@ -1867,23 +2006,27 @@ __global__ void rct_zero_kernel(int32_t* p, int32_t n) {
   if (j < n) p[j] = 0;
 }
 
-template <int W, bool COMP>
+template <int W, bool COMP, bool CHAIN>
 void* stream_kernel_ptr() {
-  return reinterpret_cast<void*>(&rct_stream_kernel<W, COMP>);
+  return reinterpret_cast<void*>(&rct_stream_kernel<W, COMP, CHAIN>);
 }
 
-template <bool COMP>
+template <bool COMP, bool CHAIN>
 void* pick_stream_t(int W) {
   switch (W) {
-    case 256: return stream_kernel_ptr<256, COMP>();
-    case 1024: return stream_kernel_ptr<1024, COMP>();
-    case 2048: return stream_kernel_ptr<2048, COMP>();
-    default: return stream_kernel_ptr<512, COMP>();
+    case 256: return stream_kernel_ptr<256, COMP, CHAIN>();
+    case 1024: return stream_kernel_ptr<1024, COMP, CHAIN>();
+    case 2048: return stream_kernel_ptr<2048, COMP, CHAIN>();
+    default: return stream_kernel_ptr<512, COMP, CHAIN>();
   }
 }
 
-void* pick_stream(int W, bool comp) {
-  return comp ? pick_stream_t<true>(W) : pick_stream_t<false>(W);
+// The chain variant (wave-wide scans for windows that are paths) exists for
+// the compensated method only; it is a separate kernel so that the other
+// nets run the smaller code.
+void* pick_stream(int W, bool comp, bool chain = false) {
+  if (!comp) return pick_stream_t<false, false>(W);
+  return chain ? pick_stream_t<true, true>(W) : pick_stream_t<true, false>(W);
 }
 
 void* pick_group(int G, bool comp) {
@@ -2061,11 +2204,12 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
       for (int b = 0; b < plan->num_bins; ++b) {
         sb.items[b] = plan->d_bin_sub[b];
         sb.lim[b] = plan->bin_ring[b] - kWave;
+        sb.chain[b] = plan->bin_chain[b];
       }
       hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, n_sub, plan->num_bins);
       hipLaunchKernelGGL(rct_subset_items_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
                          count, subset, plan->num_nets, tree->nodes_start, plan->d_net_maxpid,
-                         plan->d_net_scratch, G, sb, n_sub);
+                         plan->d_net_scratch, plan->d_net_chain, G, sb, n_sub);
       OTG_HIP(ctx, hipGetLastError());
     }
     for (int b = plan->num_bins - 1; b >= 0; --b) {
@@ -2082,7 +2226,8 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
       Q.num_items = total;
       const int grid = std::min(plan->bin_grid[b], total);
       void* args[] = {&Q};
-      OTG_HIP(ctx, hipLaunchKernel(pick_stream(plan->bin_ring[b], comp), dim3(grid), dim3(64),
+      OTG_HIP(ctx, hipLaunchKernel(pick_stream(plan->bin_ring[b], comp, plan->bin_chain[b]),
+                                   dim3(grid), dim3(64),
                                    args, 0, s));
     }
   }
@@ -2141,22 +2286,25 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
       return fail(e_ == hipErrorOutOfMemory ? OTG_ERR_OUT_OF_MEMORY : OTG_ERR_DEVICE);        \
     }                                                                                         \
   } while (0)
-  std::vector<int32_t> hstart(N + 1, 0), hmax(N, 0);
+  std::vector<int32_t> hstart(N + 1, 0), hmax(N, 0), hchain(N, 0);
   if (N > 0) {
     PLAN_HIP(hipMalloc(&p->d_net_maxpid, sizeof(int32_t) * N));
+    PLAN_HIP(hipMalloc(&p->d_net_chain, sizeof(int32_t) * N));
     PLAN_HIP(hipMalloc(&p->d_net_scratch, sizeof(int64_t) * N));
     int32_t* d_bad = nullptr;
     PLAN_HIP(hipMalloc(&d_bad, sizeof(int32_t)));
     const int32_t big = INT32_MAX;
     PLAN_HIP(hipMemcpyAsync(d_bad, &big, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(rct_netinfo_kernel, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, N,
-                       topo->nodes_start, topo->pid, p->d_net_maxpid, d_bad);
+                       topo->nodes_start, topo->pid, p->d_net_maxpid, p->d_net_chain, d_bad);
     PLAN_HIP(hipGetLastError());
     int32_t bad = 0;
     PLAN_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     PLAN_HIP(hipMemcpyAsync(hstart.data(), topo->nodes_start, sizeof(int32_t) * (N + 1),
                             hipMemcpyDeviceToHost, ctx->stream));
     PLAN_HIP(hipMemcpyAsync(hmax.data(), p->d_net_maxpid, sizeof(int32_t) * N,
+                            hipMemcpyDeviceToHost, ctx->stream));
+    PLAN_HIP(hipMemcpyAsync(hchain.data(), p->d_net_chain, sizeof(int32_t) * N,
                             hipMemcpyDeviceToHost, ctx->stream));
     PLAN_HIP(hipStreamSynchronize(ctx->stream));
     hipFree(d_bad);
@@ -2179,7 +2327,14 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   // nets 8 (16 compensated).  Streamed nets are queued largest first; packed
   // groups are ordered by their number of depth steps, most first.
   int nb = 0;
-  for (int ring = p->tuning.min_ring; ring <= W; ring *= 2) p->bin_ring[nb++] = ring;
+  // per ring size a bin of ordinary nets and (compensated) one of chain
+  // nets, which take the kernel variant with the chain-window scans
+  for (int ring = p->tuning.min_ring; ring <= W; ring *= 2)
+    for (int ch = 0; ch <= (comp ? 1 : 0); ++ch) {
+      p->bin_ring[nb] = ring;
+      p->bin_chain[nb] = ch;
+      ++nb;
+    }
   p->num_bins = nb;
   std::vector<RctItem> bin_list[otg_rct_plan::kMaxBins], groups, gitems;
   std::vector<int64_t> net_scratch(N, -1);
@@ -2222,7 +2377,9 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
     flush();
     RctItem it{hstart[k], hstart[k + 1], k, k + 1, ITEM_STREAM, 1, scratch};
     int bin = 0;
-    while (bin < nb && hmax[k] > p->bin_ring[bin] - kWave) ++bin;
+    const int want_chain = comp && hchain[k];
+    while (bin < nb && (hmax[k] > p->bin_ring[bin] - kWave || p->bin_chain[bin] != want_chain))
+      ++bin;
     if (bin < nb) {
       scratch += 4LL * n;
       ++n_stream;
@@ -2327,7 +2484,8 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   p->grid = std::max(1, (g1 > 0 ? g1 : blocks_per_cu(pick_stream(W, comp))) * ncu);
   for (int b = 0; b < nb; ++b)
     p->bin_grid[b] =
-        std::max(1, (g1 > 0 ? g1 : blocks_per_cu(pick_stream(p->bin_ring[b], comp))) * ncu);
+        std::max(1, (g1 > 0 ? g1 : blocks_per_cu(pick_stream(p->bin_ring[b], comp,
+                                                                 p->bin_chain[b]))) * ncu);
   p->grid_group = std::max(1, (g1 > 0 ? g1 : blocks_per_cu(pick_group(G, comp))) * ncu);
 #undef PLAN_HIP
   *out = p;
@@ -2350,6 +2508,7 @@ extern "C" otg_status otg_rct_plan_destroy(otg_rct_plan* p) {
   if (p->d_queue) hipFree(p->d_queue);
   if (p->d_scratch) hipFree(p->d_scratch);
   if (p->d_net_maxpid) hipFree(p->d_net_maxpid);
+  if (p->d_net_chain) hipFree(p->d_net_chain);
   if (p->d_net_scratch) hipFree(p->d_net_scratch);
   delete p;
   return OTG_OK;

@@ -298,3 +298,50 @@ def test_wide_nets_take_large_rings(ctx, method, tol64):
         assert rel_err(g[k], r32[k], fl[k]) < 1e-5, k
     for k in OUT:
         assert rel_err(g[k], r64[k], fl[k]) < tol64, k
+
+
+@pytest.mark.parametrize("method,tol64", [("batched", 1e-5), ("compensated", 1e-6)])
+def test_chains(ctx, method, tol64):
+    # pure chains (every 64-position window is a path: the compensated
+    # kernels take it with wave-wide scans) next to branching chains
+    cfg = W.RcConfig("chains", 300, [W.SizeClass(0.6, 100, 5000, N.SHAPE_CHAIN, 0.0),
+                                     W.SizeClass(0.4, 100, 3000, N.SHAPE_CHAIN, 0.05)],
+                     seed=17, r_range=(1e-3, 1e4), c_range=(1e-3, 1e2))
+    f, s = _forest(cfg, ctx)
+    h = s.host_topology()
+    s.solve(method)
+    g = s.host()
+    r64 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "f64")
+    fl = floors(r64)
+    for k in OUT:
+        assert rel_err(g[k], r64[k], fl[k]) < (1e-6 if method == "compensated" else 5e-3), k
+
+
+def test_subset_solve_compensated_all_bins(ctx):
+    # small nets (packed groups), branching chains (ordinary streamed bins),
+    # pure chains (chain bins) and wide balanced trees (large rings)
+    cfg = W.RcConfig("mix", 1500, [W.SizeClass(0.6, 2, 40), W.SizeClass(0.2, 100, 3000, N.SHAPE_CHAIN, 0.1),
+                                   W.SizeClass(0.15, 200, 3000, N.SHAPE_CHAIN, 0.0),
+                                   W.SizeClass(0.05, 600, 2500, N.SHAPE_BALANCED)], seed=41)
+    f, s = _forest(cfg, ctx)
+    s.solve("compensated")
+    full = {k: v.copy() for k, v in s.host().items()}
+    rng = np.random.default_rng(3)
+    ids = np.sort(rng.choice(cfg.num_nets, 300, replace=False)).astype(np.int32)
+    ns = s.nodes_start.cpu().numpy()
+    cap = s.cap.clone()
+    for a in ids:
+        cap[ns[a] * 4: ns[a + 1] * 4] *= 1.25
+    s.cap = cap
+    for _ in range(2):  # a second call starts from the state the first one left
+        s.solve_subset(torch.from_numpy(ids).to(s.device), "compensated")
+    sub = s.host()
+    h = s.host_topology()
+    r64 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "f64")
+    fl = floors(r64)
+    mask = np.zeros(s.num_nodes * 4, bool)
+    for a in ids:
+        mask[ns[a] * 4: ns[a + 1] * 4] = True
+    for k in OUT:
+        assert rel_err(sub[k][mask], r64[k][mask], fl[k]) < 1e-6, k
+        assert np.array_equal(sub[k][~mask], full[k][~mask]), k
