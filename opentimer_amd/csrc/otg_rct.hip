@@ -1613,9 +1613,11 @@ struct GroupLds {
   float4 w[G];     // bottom-up: w by schedule     | top-down: impulse (hi) by position
   float4 ld[G];    // load (hi) by schedule index
   float4 L[G];     // L rounded to FP32 by schedule index
-  float4 ldl[COMP ? G : 1];  // load (lo) by schedule index
+  float4 ldl[COMP ? G : 1];  // bottom-up: load (lo) by schedule index |
+                             // top-down: impulse (lo) by position (the sweep
+                             // and the outputs take load's normalised high
+                             // part, which is its rounding to FP32)
   float4 wl[COMP ? G : 1];   // bottom-up: w (lo) | top-down: delay (lo) by position
-  float4 ml[COMP ? G : 1];   // top-down: impulse (lo) by position
   float r[G];
   int32_t pid[G];
   uint32_t pw[G];
@@ -1788,21 +1790,16 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
       const int p = root ? 0 : q - S.pid[q];
       // drivers are the depth-0 schedule entries, in position (= net) order
       const int net = root ? j : S.net[p];
-      float ldv[4], Lv[4], ldlv[4];
+      float ldv[4], Lv[4];
       unpack(S.ld[j], ldv);
       unpack(S.L[j], Lv);
-      if constexpr (COMP) {
-        unpack(S.ldl[j], ldlv);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) ldv[c] = ldv[c] + ldlv[c];
-      }
       float pd[4], pm[4], pdl[4], pml[4];
       if (!root) {
         unpack(S.cap[p], pd);
         unpack(S.w[p], pm);
         if constexpr (COMP) {
           unpack(S.wl[p], pdl);
-          unpack(S.ml[p], pml);
+          unpack(S.ldl[p], pml);
         } else {
 #pragma unroll
           for (int c = 0; c < 4; ++c) pdl[c] = pml[c] = 0.f;
@@ -1817,7 +1814,7 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
       S.w[q] = pack(mm);
       if constexpr (COMP) {
         S.wl[q] = pack(dl);
-        S.ml[q] = pack(ml);
+        S.ldl[q] = pack(ml);
       }
       S.net[q] = net;
     }
@@ -1842,17 +1839,14 @@ __device__ __forceinline__ void group_outputs(GroupLds<G, COMP>& S, const RctPar
     if (s * kWave < n) {
       const int j = q < n ? S.pos2j[q] : 0;
       const int net = q < n ? S.net[q] : 0;
-      float ldv[4], Lv[4], ldlv[4], d[4], dl[4], mm[4], ml[4];
+      float ldv[4], Lv[4], d[4], dl[4], mm[4], ml[4];
       unpack(S.ld[j], ldv);
       unpack(S.L[j], Lv);
       unpack(S.cap[q], d);
       unpack(S.w[q], mm);
       if constexpr (COMP) {
-        unpack(S.ldl[j], ldlv);
         unpack(S.wl[q], dl);
-        unpack(S.ml[q], ml);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) ldv[c] = ldv[c] + ldlv[c];
+        unpack(S.ldl[q], ml);
       } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) dl[c] = ml[c] = 0.f;
