@@ -86,8 +86,8 @@ extern "C" otg_status otg_create(int device, otg_context** out) {
 
 extern "C" otg_status otg_destroy(otg_context* ctx) {
   if (!ctx) return OTG_OK;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete ctx;
   return OTG_OK;
 }
@@ -246,7 +246,7 @@ namespace {
 struct DevBufs {
   std::vector<void*> p;
   ~DevBufs() {
-    for (void* q : p) hipFree(q);
+    for (void* q : p) (void)hipFree(q);
   }
   template <class T>
   hipError_t alloc(T** out, size_t count) {
@@ -544,8 +544,8 @@ extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tre
       bt = used;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (best_ms >= 1e29) {
     OTG_SET_ERR(ctx, "autotune: no candidate ran");
     return OTG_ERR_DEVICE;

@@ -353,7 +353,7 @@ extern "C" otg_status otg_incr_destroy(otg_incr* I) {
   IncrState& S = I->S;
   for (int32_t* p : {S.flag, S.cone, S.kout, S.kfront, S.cone_ends, S.cone_cnt, S.kends, S.kcnt,
                      S.counters})
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   delete I;
   return OTG_OK;
 }
@@ -449,8 +449,8 @@ extern "C" otg_status otg_graph_launch(otg_context* ctx, otg_graph* g, otg_strea
 
 extern "C" otg_status otg_graph_destroy(otg_graph* g) {
   if (!g) return OTG_OK;
-  if (g->exec) hipGraphExecDestroy(g->exec);
-  if (g->graph) hipGraphDestroy(g->graph);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
   return OTG_OK;
 }

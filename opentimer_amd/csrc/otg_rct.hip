@@ -2301,7 +2301,7 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
     PLAN_HIP(hipMemcpyAsync(hchain.data(), p->d_net_chain, sizeof(int32_t) * N,
                             hipMemcpyDeviceToHost, ctx->stream));
     PLAN_HIP(hipStreamSynchronize(ctx->stream));
-    hipFree(d_bad);
+    (void)hipFree(d_bad);
     if (bad != INT32_MAX) {
       otg_set_error(ctx, "net %d: pid must satisfy 1 <= pid[i] <= i - nodes_start[net]", bad);
       return fail(OTG_ERR_BAD_TOPOLOGY);
@@ -2488,22 +2488,16 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
 
 extern "C" otg_status otg_rct_plan_destroy(otg_rct_plan* p) {
   if (!p) return OTG_OK;
+  // (hipFree of a null pointer is a no-op; failures here cannot be acted on)
   for (int b = 0; b < otg_rct_plan::kMaxBins; ++b) {
-    if (p->d_bin_items[b]) hipFree(p->d_bin_items[b]);
-    if (p->d_bin_sub[b]) hipFree(p->d_bin_sub[b]);
+    (void)hipFree(p->d_bin_items[b]);
+    (void)hipFree(p->d_bin_sub[b]);
   }
-  if (p->d_gitems) hipFree(p->d_gitems);
-  if (p->d_groups) hipFree(p->d_groups);
-  if (p->d_sub_groups) hipFree(p->d_sub_groups);
-  if (p->d_sched) hipFree(p->d_sched);
-  if (p->d_group_flag) hipFree(p->d_group_flag);
-  if (p->d_sub_count) hipFree(p->d_sub_count);
-  if (p->d_net_group) hipFree(p->d_net_group);
-  if (p->d_queue) hipFree(p->d_queue);
-  if (p->d_scratch) hipFree(p->d_scratch);
-  if (p->d_net_maxpid) hipFree(p->d_net_maxpid);
-  if (p->d_net_chain) hipFree(p->d_net_chain);
-  if (p->d_net_scratch) hipFree(p->d_net_scratch);
+  for (void* q : {(void*)p->d_gitems, (void*)p->d_groups, (void*)p->d_sub_groups,
+                  (void*)p->d_sched, (void*)p->d_group_flag, (void*)p->d_sub_count,
+                  (void*)p->d_net_group, (void*)p->d_queue, (void*)p->d_scratch,
+                  (void*)p->d_net_maxpid, (void*)p->d_net_chain, (void*)p->d_net_scratch})
+    (void)hipFree(q);
   delete p;
   return OTG_OK;
 }
