@@ -109,8 +109,8 @@ typedef struct otg_rct_result {
 /* Tuning knobs of the batched launch; zero fields take the defaults (or the
  * values persisted by the autotuner, see otg_rct_autotune). */
 typedef struct otg_rct_tuning {
-  int32_t ring_slots;     /* largest on-chip ring per wavefront: 256, 512,   */
-                          /* 1024 or 2048 (default); nets with parent      */
+  int32_t ring_slots;     /* largest on-chip ring per wavefront: 256, 320,   */
+                          /* 512, 1024 or 2048 (default); nets with parent */
                           /* offsets above ring-64 take the slower         */
                           /* global-window path                            */
   int32_t min_ring;       /* smallest ring (same sizes, <= ring_slots): a    */

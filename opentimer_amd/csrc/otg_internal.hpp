@@ -26,6 +26,12 @@ enum RctItemKind : int32_t {
   ITEM_GLOBAL = 2   // one net that needs a per-node window in global memory
 };
 
+// Ring sizes (slots per wavefront) of the streaming kernel, ascending.  320
+// holds parent offsets up to 256 at 20 KB of LDS in the compensated mode
+// (7 workgroups per CU against 4 for 512).
+constexpr int kRingLadder[] = {256, 320, 512, 1024, 2048};
+constexpr int kRingSteps = 5;
+
 struct RctItem {
   int32_t pos_begin, pos_end;
   int32_t net_begin, net_end;
@@ -44,8 +50,8 @@ struct otg_rct_plan {
   // a net goes to the first bin whose ring holds its parent offsets
   // (maxpid <= ring - 64).  A smaller ring needs less LDS, so more
   // workgroups fit on a CU.
-  static constexpr int kMaxBins = 8;  // rings of 256, 512, 1024, 2048 slots, each for
-                                      // ordinary and (compensated) chain nets
+  static constexpr int kMaxBins = 10;  // rings of kRingLadder, each for ordinary and
+                                       // (compensated) chain nets
   static constexpr int kQueueInts = 4 * kMaxBins + 2;
   int num_bins = 0;
   int bin_ring[kMaxBins] = {};

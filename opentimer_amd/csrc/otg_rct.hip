@@ -907,7 +907,11 @@ template <int W, bool COMP>
 struct SRing {
   StreamLds<W, COMP>& S;
   int base;
-  __device__ __forceinline__ int slot(int i) const { return (i - base) & (W - 1); }
+  // position i >= base of the item; rings need not be powers of two
+  __device__ __forceinline__ int slot(int i) const {
+    if constexpr ((W & (W - 1)) == 0) return (i - base) & (W - 1);
+    else return (int)((unsigned)(i - base) % (unsigned)W);
+  }
   __device__ __forceinline__ void get(int k, float (&x)[4], float (&y)[4], float (&xl)[4],
                                       float (&yl)[4]) const {
     unpack(S.a[k], x);
@@ -2009,6 +2013,7 @@ template <bool COMP, bool CHAIN>
 void* pick_stream_t(int W) {
   switch (W) {
     case 256: return stream_kernel_ptr<256, COMP, CHAIN>();
+    case 320: return stream_kernel_ptr<320, COMP, CHAIN>();
     case 1024: return stream_kernel_ptr<1024, COMP, CHAIN>();
     case 2048: return stream_kernel_ptr<2048, COMP, CHAIN>();
     default: return stream_kernel_ptr<512, COMP, CHAIN>();
@@ -2095,7 +2100,11 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   otg_rct_tuning r{};
   if (t) r = *t;
   const otg_rct_tuning& d = ctx->tuning;
-  auto ring_ok = [](int w) { return w == 256 || w == 512 || w == 1024 || w == 2048; };
+  auto ring_ok = [](int w) {
+    for (int k = 0; k < kRingSteps; ++k)
+      if (kRingLadder[k] == w) return true;
+    return false;
+  };
   // the largest ring by default: a bin holding no nets costs nothing, and
   // nets with parent offsets up to 1984 (high-fanout nets) stay on chip
   if (!r.ring_slots) r.ring_slots = d.ring_slots ? d.ring_slots : 2048;
@@ -2323,12 +2332,15 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   int nb = 0;
   // per ring size a bin of ordinary nets and (compensated) one of chain
   // nets, which take the kernel variant with the chain-window scans
-  for (int ring = p->tuning.min_ring; ring <= W; ring *= 2)
+  for (int k = 0; k < kRingSteps; ++k) {
+    const int ring = kRingLadder[k];
+    if (ring < p->tuning.min_ring || ring > W) continue;
     for (int ch = 0; ch <= (comp ? 1 : 0); ++ch) {
       p->bin_ring[nb] = ring;
       p->bin_chain[nb] = ch;
       ++nb;
     }
+  }
   p->num_bins = nb;
   std::vector<RctItem> bin_list[otg_rct_plan::kMaxBins], groups, gitems;
   std::vector<int64_t> net_scratch(N, -1);
