@@ -126,6 +126,14 @@ __device__ __forceinline__ ff ff_acc(ff a, ff b) {
   const ff s = two_sum(a.hi, b.hi);
   return {s.hi, s.lo + (a.lo + b.lo)};
 }
+// a + b likewise unnormalised: after k such additions along a path the low
+// part is within k units in the last place of the high part and its own
+// rounding adds a relative k^2 u^2 (u = 2^-24), so a path renormalised every
+// few dozen steps stays exact to about 1e-12.
+__device__ __forceinline__ ff ff_acc_f(ff a, float b) {
+  const ff s = two_sum(a.hi, b);
+  return {s.hi, s.lo + a.lo};
+}
 
 __device__ __forceinline__ void unpack(float4 v, float (&a)[4]) {
   a[0] = v.x;
@@ -807,12 +815,14 @@ constexpr int kChunkWin = 4;
 constexpr int kChunkPos = kChunkWin * kWave;  // 256
 constexpr int kClaim = 256;                   // claim table entries (parent slot mod 256)
 
+// Slot W of each ring array is a zero slot, never written after the ring is
+// cleared: the driver's top-down parent.
 template <int W, bool COMP>
 struct StreamLds {
-  float4 a[W];   // bottom-up: load accumulator (hi) | top-down: delay (hi)
-  float4 b[W];   // bottom-up: L accumulator (hi)    | top-down: impulse (hi)
-  float4 al[COMP ? W : 1];
-  float4 bl[COMP ? W : 1];
+  float4 a[W + 1];  // bottom-up: load accumulator (hi) | top-down: delay (hi)
+  float4 b[W + 1];  // bottom-up: L accumulator (hi)    | top-down: impulse (hi)
+  float4 al[COMP ? W + 1 : 1];
+  float4 bl[COMP ? W + 1 : 1];
   int32_t claim[kClaim];  // one writer per parent slot per round
 };
 
@@ -1020,11 +1030,11 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
     ff S[4], V[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      S[c] = valid ? ff_add_f(quick_two_sum(xa[c], xal[c]), cap4[c]) : ff{0.f, 0.f};
+      S[c] = valid ? ff_add_f(ff{xa[c], xal[c]}, cap4[c]) : ff{0.f, 0.f};
     suffix_scan(S, lane);
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      V[c] = valid ? ff_add_f(quick_two_sum(xb[c], xbl[c]), r * S[c].hi * S[c].hi) : ff{0.f, 0.f};
+      V[c] = valid ? ff_add_f(ff{xb[c], xbl[c]}, r * S[c].hi * S[c].hi) : ff{0.f, 0.f};
     suffix_scan(V, lane);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -1074,7 +1084,7 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
           const ff tt = ff_add_f(ff{xa[c], xal[c]}, cap4[c]);
           ld[c] = tt.hi;
           ldl[c] = tt.lo;
-          const ff q = ff_add_f(quick_two_sum(xb[c], xbl[c]), r * tt.hi * tt.hi);
+          const ff q = ff_add_f(ff{xb[c], xbl[c]}, r * tt.hi * tt.hi);
           w[c] = q.hi;
           wl[c] = q.lo;
           ldo[c] = tt.hi;
@@ -1181,8 +1191,9 @@ __device__ __forceinline__ TdIncr td_incr(float rr, const float (&ld)[4], const 
   return t;
 }
 
-// delay = delay_parent + u, impulse = impulse_parent + z.
-template <bool COMP>
+// delay = delay_parent + u, impulse = impulse_parent + z; NORM renormalises
+// the compensated sums (needed once every few dozen steps along a path).
+template <bool COMP, bool NORM>
 __device__ __forceinline__ void td_accum(const TdIncr& t, const float (&pd)[4],
                                          const float (&pdl)[4], const float (&pm)[4],
                                          const float (&pml)[4], float (&d)[4], float (&dl)[4],
@@ -1190,8 +1201,8 @@ __device__ __forceinline__ void td_accum(const TdIncr& t, const float (&pd)[4],
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     if constexpr (COMP) {
-      const ff dd = ff_add_f(ff{pd[c], pdl[c]}, t.u[c]);
-      const ff mm = ff_add_f(ff{pm[c], pml[c]}, t.z[c]);
+      const ff dd = NORM ? ff_add_f(ff{pd[c], pdl[c]}, t.u[c]) : ff_acc_f(ff{pd[c], pdl[c]}, t.u[c]);
+      const ff mm = NORM ? ff_add_f(ff{pm[c], pml[c]}, t.z[c]) : ff_acc_f(ff{pm[c], pml[c]}, t.z[c]);
       d[c] = dd.hi;
       dl[c] = dd.lo;
       m[c] = mm.hi;
@@ -1258,16 +1269,21 @@ __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int p
   float d[4], dl[4], m[4], ml[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) d[c] = dl[c] = m[c] = ml[c] = 0.f;
-  for (int t = 0; t <= maxd; ++t) {
+  // depth 0 (parents in earlier windows; the driver reads the ring's zero
+  // slot): the path sums are renormalised here, once per window, and carried
+  // unnormalised through the window's deeper steps
+  if (dw == 0) {
+    float pd[4], pm[4], pdl[4], pml[4];
+    R.get(par >= 0 ? ps : W, pd, pm, pdl, pml);
+    td_accum<COMP, true>(inc, pd, pdl, pm, pml, d, dl, m, ml);
+    R.put(sl, d, m, dl, ml);
+  }
+  wave_sync();
+  for (int t = 1; t <= maxd; ++t) {
     if (dw == t) {
       float pd[4], pm[4], pdl[4], pml[4];
-      if (par >= 0) {
-        R.get(ps, pd, pm, pdl, pml);
-      } else {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) pd[c] = pm[c] = pdl[c] = pml[c] = 0.f;
-      }
-      td_accum<COMP>(inc, pd, pdl, pm, pml, d, dl, m, ml);
+      R.get(ps, pd, pm, pdl, pml);
+      td_accum<COMP, false>(inc, pd, pdl, pm, pml, d, dl, m, ml);
       R.put(sl, d, m, dl, ml);
     }
     wave_sync();
@@ -1542,7 +1558,7 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
     if (ph == PH_BU) {
       if (c == nc - 1) {
         const SRing<W, COMP> R{S, it.pb};
-        for (int k = lane; k < W; k += kWave) R.zero(k);
+        for (int k = lane; k <= W; k += kWave) R.zero(k);
         wave_sync();
       }
       run_bu_chunk<W, COMP, CHAIN>(S, P, it, c, cur, take, nxt, lane);
@@ -1750,16 +1766,16 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if constexpr (COMP) {
-          // children accumulated unnormalised: renormalise, then the
-          // node's own term (FP32 product, compensated sum)
+          // children accumulated unnormalised: the load sum is
+          // renormalised (its high part enters the products); the L sum is
+          // renormalised by the compensated addition of the node's own term
           const ff l2 = quick_two_sum(ld[c].hi, ld[c].lo);
-          const ff s2 = quick_two_sum(sw[c].hi, sw[c].lo);
-          const ff q2 = ff_add_f(s2, r * l2.hi * l2.hi);
+          const ff q2 = ff_add_f(sw[c], r * l2.hi * l2.hi);
           o_ld[c] = l2.hi;
           o_ldl[c] = l2.lo;
           o_w[c] = q2.hi;
           o_wl[c] = q2.lo;
-          o_L[c] = s2.hi;
+          o_L[c] = sw[c].hi + sw[c].lo;
         } else {
           o_ld[c] = ld[c].hi;
           o_w[c] = sw[c].hi + r * ld[c].hi * ld[c].hi;
@@ -1813,7 +1829,8 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
         for (int c = 0; c < 4; ++c) pd[c] = pm[c] = pdl[c] = pml[c] = 0.f;
       }
       float d[4], dl[4], mm[4], ml[4];
-      td_accum<COMP>(td_incr<COMP>(rr, ldv, Lv), pd, pdl, pm, pml, d, dl, mm, ml);
+      // paths inside a group are at most 255 edges long: no renormalisation
+      td_accum<COMP, false>(td_incr<COMP>(rr, ldv, Lv), pd, pdl, pm, pml, d, dl, mm, ml);
       S.cap[q] = pack(d);
       S.w[q] = pack(mm);
       if constexpr (COMP) {
