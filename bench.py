@@ -483,8 +483,8 @@ def main():
                                          N.BATCHED if args.method == "batched" else
                                          N.BATCHED_COMPENSATED, args.tuning.encode(),
                                          C.byref(best)))
-        log(f"autotuned: ring {best.ring_slots} group target {best.small_target} "
-            f"grid/cu {best.grid_per_cu}")
+        log(f"autotuned: ring {best.ring_slots} min ring {best.min_ring} "
+            f"group target {best.small_target} grid/cu {best.grid_per_cu}")
 
     other = "batched" if args.method == "compensated" else "compensated"
     plans = {}

@@ -127,9 +127,10 @@ class RcSolver:
                                            m, stream))
         return self.out
 
-    def solve_subset(self, net_ids: torch.Tensor, method="batched", stream=None, names=OUTPUTS):
+    def solve_subset(self, net_ids: torch.Tensor, method="batched", stream=None, names=OUTPUTS,
+                     tuning=None):
         m = METHODS[method]
-        plan = self.plan(method)
+        plan = self.plan(method, tuning)
         stream = torch_stream(self.device) if stream is None else stream
         res = self.result(names)
         self.ctx.check(N.lib.otg_rct_solve_subset(self.ctx.ptr, plan,
