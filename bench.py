@@ -52,6 +52,11 @@ def parse():
                     help="nets of the cfg1 distribution written as SPEF for --only spef")
     ap.add_argument("--precision-nets", type=int, default=20_000,
                     help="nets of the cfg3 distribution for --only precision")
+    ap.add_argument("--precision-ref", choices=("auto", "host", "device"), default="auto",
+                    help="FP64 reference of --only precision: the host oracle at every node, or "
+                         "the device FP64 kernel over batches (auto: device above 20k nets)")
+    ap.add_argument("--precision-batch", type=int, default=16_384,
+                    help="nets per batch of a device-reference precision run")
     return ap.parse_args()
 
 
@@ -186,6 +191,106 @@ def run_precision(ctx, nets):
             del g, e
         res[method] = {"ms": a.elapsed_time(b), "rel_err_vs_fp64": errs}
         log(f"precision {method}: {res[method]}")
+    return res
+
+
+def run_precision_streamed(ctx, nets, batch):
+    """cfg3 precision report at full size: the nets of the cfg3 distribution
+    are generated, ordered and solved in batches of `batch` nets (each net
+    drawn from its global index, so the batches make up the whole
+    population), and every output of every node of the plain kernel, the
+    FP32 batched method and the compensated method is compared on the device
+    with the device FP64 reference (otg_rct_reference_f64).  Relative errors
+    use a floor of 1e-7 x the batch's largest reference magnitude (1e-6 x the
+    largest beta for impulse); a batch's largest values are at most the
+    population's, so these errors bound the ones with population-wide
+    floors from above.  The device reference itself is checked against the
+    host FP64 oracle on a sample of nets of the first batch (the largest
+    included)."""
+    import torch
+
+    from opentimer_amd import workloads as W
+    from opentimer_amd.rct import RcSolver, OUTPUTS
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    methods = ("plain", "batched", "compensated")
+    cfg_all = W.config3(num_nets=nets)
+    agg = {m: {k: [0.0, 0.0] for k in OUTPUTS} for m in methods}
+    ms = {m: 0.0 for m in methods}
+    ref_ms = 0.0
+    nvals = 0
+    nodes = 0
+    check = {}
+    t_start = time.perf_counter()
+    for a in range(0, nets, batch):
+        b = min(nets, a + batch)
+        cfg = cfg_all.scaled(b - a)
+        cfg.first_net = a
+        forest = W.generate_rc(cfg, nthreads=16)
+        s = RcSolver.from_edges(forest, ctx)
+        del forest
+        V = s.num_nodes
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ref = s.reference_f64()
+        e1.record()
+        torch.cuda.synchronize()
+        ref_ms += e0.elapsed_time(e1)
+        if a == 0:
+            topo = s.host_topology()
+            ns = topo["nodes_start"]
+            sizes = np.diff(ns)
+            pick = np.unique(np.concatenate([np.arange(min(b - a, 24)), np.argsort(sizes)[-8:]]))
+            idx = np.concatenate([np.arange(4 * ns[k], 4 * ns[k + 1]) for k in pick])
+            nd = np.concatenate([np.arange(ns[k], ns[k + 1]) for k in pick])
+            sub = np.concatenate([[0], np.cumsum(sizes[pick])]).astype(np.int32)
+            n4 = np.concatenate([np.arange(4 * k, 4 * k + 4) for k in pick])
+            r64 = O.rct_all(sub, topo["pid"][nd], topo["pres"][nd], topo["cap"][idx],
+                            topo["slew_in"][n4], "f64")
+            it = torch.from_numpy(idx).to(s.device)
+            for k in OUTPUTS:
+                g = ref[k][it].cpu().numpy()
+                fl = max(float(np.max(np.abs(r64[k if k != "impulse" else "beta"])))
+                          * (1e-6 if k == "impulse" else 1e-7), 1e-300)
+                check[k] = float(np.max(np.abs(g - r64[k]) / np.maximum(np.abs(r64[k]), fl)))
+            check = {"nets": int(len(pick)), "nodes": int(len(nd)), "max_rel_diff": check}
+            log(f"device FP64 reference vs host FP64 oracle on {len(pick)} nets: {check}")
+            del topo
+        floor = {k: max(float(ref[k].abs().max()) * 1e-7, 1e-300) for k in OUTPUTS}
+        floor["impulse"] = max(float(ref["beta"].abs().max()) * 1e-6, 1e-300)
+        for m in methods:
+            s.solve(m)  # builds the batch's plan
+            e0.record()
+            s.solve(m)
+            e1.record()
+            torch.cuda.synchronize()
+            ms[m] += e0.elapsed_time(e1)
+            for k in OUTPUTS:
+                e = (s.out[k].double() - ref[k]).abs_().div_(ref[k].abs().clamp_(min=floor[k]))
+                agg[m][k][0] = max(agg[m][k][0], float(e.max()))
+                agg[m][k][1] += float(e.sum())
+                del e
+        nvals += 4 * V
+        nodes += V
+        del ref
+        s.close()
+        del s
+        torch.cuda.empty_cache()
+        log(f"precision batch nets {a}..{b}: {nodes} nodes so far, "
+            f"compensated max {max(v[0] for v in agg['compensated'].values()):.3g}, "
+            f"{time.perf_counter() - t_start:.0f}s")
+    res = {"workload": f"{cfg_all.name}: {nets} nets, {nodes} nodes (cfg3 distribution), "
+                       f"batches of {batch} nets",
+           "nets": nets, "nodes": nodes,
+           "reference": "device FP64 reference (otg_rct_reference_f64), every node",
+           "reference_check_vs_host_fp64": check, "reference_ms": ref_ms,
+           "wall_s": time.perf_counter() - t_start}
+    for m in methods:
+        res[m] = {"ms": ms[m], "nodes_per_s": nodes / (ms[m] * 1e-3),
+                  "rel_err_vs_fp64": {k: {"max": v[0], "mean": v[1] / nvals}
+                                      for k, v in agg[m].items()}}
+        log(f"precision {m}: {res[m]}")
     return res
 
 
@@ -430,7 +535,12 @@ def main():
         print(json.dumps(run_levelize(ctx, stream, sh)))
         return
     if args.only == "precision":
-        print(json.dumps(run_precision(ctx, args.precision_nets)))
+        dev_ref = args.precision_ref == "device" or (args.precision_ref == "auto"
+                                                     and args.precision_nets > 20_000)
+        if dev_ref:
+            print(json.dumps(run_precision_streamed(ctx, args.precision_nets, args.precision_batch)))
+        else:
+            print(json.dumps(run_precision(ctx, args.precision_nets)))
         return
     if args.only == "incremental":
         print(json.dumps(run_incremental(ctx, args.method)))

@@ -137,6 +137,18 @@ otg_status otg_rct_plan_destroy(otg_rct_plan* plan);
  * [2] streamed nets, [3] global-window nets, [4] max net size. */
 otg_status otg_rct_plan_stats(const otg_rct_plan* plan, int64_t stats[8]);
 
+/* FP64 reference values of P1 for precision reports: the reference's four
+ * sweeps (ot/timer/net.cpp:101-193, impulse = 2 beta - delay^2 as :191)
+ * and the output slew (net.cpp:30-32) carried out in double from the FP32
+ * inputs, one thread per (net, corner).  No reference counterpart: a
+ * measuring instrument that the FP32 methods are compared against on the
+ * device when a workload is too large for a host FP64 check (slow: each
+ * net is walked sequentially).  Outputs are device double arrays [4V] in
+ * the tree's order; slew may be NULL.  Asynchronous on `stream`. */
+otg_status otg_rct_reference_f64(otg_context* ctx, const otg_rct_tree* tree, double* load,
+                                 double* delay, double* ldelay, double* beta, double* impulse,
+                                 double* slew, otg_stream stream);
+
 /* Solve P1 for every net.  Replaces the compute stage of
  * rct_bfs_and_compute_cuda (reference ot/cuda/rct.cu:385-389, kernel
  * compute_net_timing rct.cu:11-59) on device-resident data; adds beta and

@@ -48,6 +48,11 @@ typedef struct otg_gen_rc_params {
   double r_unit;                /* stored R = R * r_unit (1e-3: ohm->kohm)  */
   int32_t relabel;              /* random node numbering inside each net    */
   int32_t random_orient;        /* random (s,t) order of each edge          */
+  int64_t first_net;            /* global index of the first net: the nets  */
+                                /* are nets first_net.. of a larger         */
+                                /* population (each net is drawn from its   */
+                                /* own index), so a large workload can be   */
+                                /* generated and solved in batches          */
 } otg_gen_rc_params;
 
 /* Net sizes: nodes_start[N+1] and the total node count. */

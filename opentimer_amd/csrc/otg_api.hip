@@ -110,6 +110,18 @@ static otg_status check_tree(otg_context* ctx, const otg_rct_tree* t) {
   return OTG_OK;
 }
 
+extern "C" otg_status otg_rct_reference_f64(otg_context* ctx, const otg_rct_tree* tree,
+                                            double* load, double* delay, double* ldelay,
+                                            double* beta, double* impulse, double* slew,
+                                            otg_stream stream) {
+  if (!ctx) return OTG_ERR_INVALID_ARGUMENT;
+  otg_status st = check_tree(ctx, tree);
+  if (st != OTG_OK) return st;
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  return otg_rct_launch_f64(ctx, tree, load, delay, ldelay, beta, impulse, slew,
+                            otg_pick_stream(ctx, stream));
+}
+
 extern "C" otg_status otg_rct_solve(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tree* tree,
                                     const otg_rct_result* out, otg_rct_method method,
                                     otg_stream stream) {

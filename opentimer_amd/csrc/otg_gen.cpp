@@ -78,7 +78,8 @@ int pick_class(const otg_gen_rc_params* p, Rng& r) {
 }
 
 bool valid_params(const otg_gen_rc_params* p) {
-  if (!p || p->num_nets < 0 || p->num_classes < 1 || p->num_classes > 4) return false;
+  if (!p || p->num_nets < 0 || p->num_classes < 1 || p->num_classes > 4 || p->first_net < 0)
+    return false;
   for (int i = 0; i < p->num_classes; ++i) {
     const auto& c = p->classes[i];
     if (c.min_nodes < 1 || c.max_nodes < c.min_nodes || c.weight < 0) return false;
@@ -95,7 +96,7 @@ extern "C" otg_status otg_gen_rc_sizes(const otg_gen_rc_params* p, int32_t* node
   int64_t tot = 0;
   nodes_start[0] = 0;
   for (int32_t k = 0; k < p->num_nets; ++k) {
-    Rng r(p->seed, (uint64_t)k, 1);
+    Rng r(p->seed, (uint64_t)(p->first_net + k), 1);
     const auto& c = p->classes[pick_class(p, r)];
     int64_t n = r.range(c.min_nodes, c.max_nodes);
     tot += n;
@@ -116,12 +117,12 @@ extern "C" otg_status otg_gen_rc_fill(const otg_gen_rc_params* p, const int32_t*
     std::vector<int32_t> par, perm;
     std::vector<uint8_t> leaf;
     for (int64_t k = b; k < e; ++k) {
-      Rng rc(p->seed, (uint64_t)k, 1);
+      Rng rc(p->seed, (uint64_t)(p->first_net + k), 1);
       const auto& cls = p->classes[pick_class(p, rc)];
       const int64_t off = nodes_start[k];
       const int32_t n = nodes_start[k + 1] - nodes_start[k];
       if (n <= 0) { roots[k] = 0; continue; }
-      Rng r(p->seed, (uint64_t)k, 2);
+      Rng r(p->seed, (uint64_t)(p->first_net + k), 2);
       par.assign(n, -1);
       leaf.assign(n, 1);
       for (int32_t i = 1; i < n; ++i) {

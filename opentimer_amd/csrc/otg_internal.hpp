@@ -108,6 +108,9 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
                                       otg_rct_method m);
 
 // Launchers implemented in otg_rct.hip
+otg_status otg_rct_launch_f64(otg_context* ctx, const otg_rct_tree* tree, double* load,
+                              double* delay, double* ldelay, double* beta, double* impulse,
+                              double* slew, hipStream_t s);
 otg_status otg_rct_launch_plain(otg_context* ctx, const otg_rct_tree* tree,
                                 const otg_rct_result* out, hipStream_t s);
 otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tree* tree,

@@ -42,6 +42,47 @@ namespace {
 constexpr int kWave = 64;
 
 // ---------------------------------------------------------------------------
+// FP64 reference values (precision reports): the reference's sweeps
+// (ot/timer/net.cpp:101-193; impulse = 2 beta - delay^2 as :191) and the
+// output slew (net.cpp:30-32) in double from the FP32 inputs, one thread per
+// (net, corner), each net walked in BFS order as the plain kernel walks it.
+// A measuring instrument for the FP32 methods, not a solve method.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void rct_f64_kernel(
+    int32_t num_nets, const int32_t* __restrict__ nodes_start, const int32_t* __restrict__ pid,
+    const float* __restrict__ pres, const float* __restrict__ cap,
+    const float* __restrict__ slew_in, double* load, double* delay, double* ldelay, double* beta,
+    double* impulse, double* slew) {
+  const int net = blockIdx.x * 64 + threadIdx.x;
+  const int c = threadIdx.y;
+  if (net >= num_nets) return;
+  const int64_t st = nodes_start[net], ed = nodes_start[net + 1];
+  if (ed <= st) return;
+  const int64_t st4 = st * 4 + c, ed4 = ed * 4 + c;
+  for (int64_t i = st4; i < ed4; i += 4) load[i] = (double)cap[i];
+  for (int64_t i = ed4 - 4, j = ed - 1; i > st4; i -= 4, --j) load[i - (int64_t)pid[j] * 4] += load[i];
+  delay[st4] = 0.0;
+  for (int64_t i = st4 + 4, j = st + 1; i < ed4; i += 4, ++j)
+    delay[i] = delay[i - (int64_t)pid[j] * 4] + (double)pres[j] * load[i];
+  for (int64_t i = st4; i < ed4; i += 4) ldelay[i] = (double)cap[i] * delay[i];
+  for (int64_t i = ed4 - 4, j = ed - 1; i > st4; i -= 4, --j)
+    ldelay[i - (int64_t)pid[j] * 4] += ldelay[i];
+  beta[st4] = 0.0;
+  for (int64_t i = st4 + 4, j = st + 1; i < ed4; i += 4, ++j)
+    beta[i] = beta[i - (int64_t)pid[j] * 4] + (double)pres[j] * ldelay[i];
+  const double si = slew_in ? (double)slew_in[(int64_t)net * 4 + c] : 0.0;
+  for (int64_t i = st4; i < ed4; i += 4) {
+    const double t = delay[i];
+    const double imp = 2.0 * beta[i] - t * t;
+    impulse[i] = imp;
+    if (slew) {
+      const double v = sqrt(si * si + imp);
+      slew[i] = si < 0.0 ? -v : v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Plain kernel: reference arithmetic, FP32, no contraction.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void rct_plain_kernel(
@@ -2143,6 +2184,22 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
 int otg_rct_blocks_per_cu(otg_context* ctx, const otg_rct_tuning& t, bool comp) {
   (void)ctx;
   return blocks_per_cu(pick_stream(t.ring_slots, comp));
+}
+
+otg_status otg_rct_launch_f64(otg_context* ctx, const otg_rct_tree* tree, double* load,
+                              double* delay, double* ldelay, double* beta, double* impulse,
+                              double* slew, hipStream_t s) {
+  if (!load || !delay || !ldelay || !beta || !impulse) {
+    OTG_SET_ERR(ctx, "FP64 reference needs load, delay, ldelay, beta and impulse");
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  if (tree->num_nets == 0) return OTG_OK;
+  dim3 grid((tree->num_nets + 63) / 64), block(64, 4);
+  hipLaunchKernelGGL(rct_f64_kernel, grid, block, 0, s, tree->num_nets, tree->nodes_start,
+                     tree->pid, tree->pres, tree->cap, tree->slew_in, load, delay, ldelay, beta,
+                     impulse, slew);
+  OTG_HIP(ctx, hipGetLastError());
+  return OTG_OK;
 }
 
 otg_status otg_rct_launch_plain(otg_context* ctx, const otg_rct_tree* tree,

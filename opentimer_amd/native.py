@@ -109,7 +109,8 @@ class GenRcParams(C.Structure):
                 ("pin_min", C.c_double), ("pin_max", C.c_double),
                 ("corner_scale", C.c_double * 4),
                 ("slew_min", C.c_double), ("slew_max", C.c_double),
-                ("r_unit", C.c_double), ("relabel", C.c_int32), ("random_orient", C.c_int32)]
+                ("r_unit", C.c_double), ("relabel", C.c_int32), ("random_orient", C.c_int32),
+                ("first_net", C.c_int64)]
 
 
 class GenDagParams(C.Structure):
@@ -153,6 +154,7 @@ _sig("otg_rct_plan_create", C.c_int, ctx_p, C.POINTER(RctTree), C.POINTER(RctTun
 _sig("otg_rct_plan_destroy", C.c_int, plan_p)
 _sig("otg_rct_plan_stats", C.c_int, plan_p, i64p)
 _sig("otg_rct_solve", C.c_int, ctx_p, plan_p, C.POINTER(RctTree), C.POINTER(RctResult), C.c_int, vp)
+_sig("otg_rct_reference_f64", C.c_int, ctx_p, C.POINTER(RctTree), vp, vp, vp, vp, vp, vp, vp)
 _sig("otg_rct_solve_subset", C.c_int, ctx_p, plan_p, C.POINTER(RctTree), C.POINTER(RctResult),
      C.c_int, vp, C.c_int32, vp)
 _sig("otg_rct_bfs", C.c_int, ctx_p, C.POINTER(RctEdges), vp, vp, vp, vp, vp, vp)
@@ -199,7 +201,7 @@ _sig("otg_spef_destroy", C.c_int, spef_p)
 EXPORTED = [
     "otg_create", "otg_destroy", "otg_synchronize", "otg_status_string", "otg_last_error",
     "otg_version", "otg_rct_plan_create", "otg_rct_plan_destroy", "otg_rct_plan_stats",
-    "otg_rct_solve", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
+    "otg_rct_solve", "otg_rct_reference_f64", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
     "otg_rct_compute", "otg_rct_query",
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize",
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",

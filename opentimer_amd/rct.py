@@ -127,6 +127,18 @@ class RcSolver:
                                            m, stream))
         return self.out
 
+    def reference_f64(self, stream=None) -> dict:
+        """FP64 reference values of every output (otg_rct_reference_f64): the
+        reference's sweeps (ot/timer/net.cpp:101-193) in double on the device,
+        for precision reports on workloads too large for a host check."""
+        V = self.num_nodes
+        ref = {k: torch.empty(max(4 * V, 1), dtype=torch.float64, device=self.device)
+               for k in OUTPUTS}
+        stream = torch_stream(self.device) if stream is None else stream
+        self.ctx.check(N.lib.otg_rct_reference_f64(
+            self.ctx.ptr, C.byref(self.tree()), *[ref[k].data_ptr() for k in OUTPUTS], stream))
+        return ref
+
     def solve_subset(self, net_ids: torch.Tensor, method="batched", stream=None, names=OUTPUTS,
                      tuning=None):
         m = METHODS[method]

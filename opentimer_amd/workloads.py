@@ -46,6 +46,7 @@ class RcConfig:
     r_unit: float = 1e-3                 # ohm -> kohm
     relabel: bool = True
     random_orient: bool = True
+    first_net: int = 0                   # global index of net 0 (batches of a larger population)
 
     def scaled(self, num_nets: int, seed: int | None = None) -> "RcConfig":
         c = RcConfig(**{**self.__dict__})
@@ -93,6 +94,7 @@ def _params(cfg: RcConfig) -> N.GenRcParams:
     p.r_unit = cfg.r_unit
     p.relabel = int(cfg.relabel)
     p.random_orient = int(cfg.random_orient)
+    p.first_net = cfg.first_net
     return p
 
 

@@ -345,3 +345,22 @@ def test_subset_solve_compensated_all_bins(ctx):
     for k in OUT:
         assert rel_err(sub[k][mask], r64[k][mask], fl[k]) < 1e-6, k
         assert np.array_equal(sub[k][~mask], full[k][~mask]), k
+
+
+def test_reference_f64_matches_fp64_oracle(ctx):
+    # the device FP64 reference (used for precision reports of workloads too
+    # large for the host oracle) against the host FP64 oracle: small trees,
+    # branching chains and balanced trees over the cfg3 value ranges; sums
+    # agree to FP64 summation-order noise, impulse (2 beta - delay^2 cancels)
+    # to that noise relative to its floor
+    cfg = W.RcConfig("ref64", 600, [W.SizeClass(0.5, 2, 64), W.SizeClass(0.3, 100, 3000, N.SHAPE_CHAIN, 0.1),
+                                    W.SizeClass(0.2, 64, 1024, N.SHAPE_BALANCED)], seed=23,
+                     r_range=(1e-3, 1e4), c_range=(1e-3, 1e2))
+    f, s = _forest(cfg, ctx)
+    h = s.host_topology()
+    ref = {k: v.cpu().numpy() for k, v in s.reference_f64().items()}
+    r64 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "f64")
+    fl = floors(r64)
+    for k in OUT:
+        tol = 1e-9 if k in ("impulse", "slew") else 1e-12
+        assert rel_err(ref[k], r64[k], fl[k]) < tol, k

@@ -103,6 +103,23 @@ def test_rc_generator_is_seeded_and_well_formed():
     assert np.all(cap > 0)
 
 
+def test_rc_generator_batches_match_the_whole_population():
+    # nets first_net.. of a population are drawn from their global indices, so
+    # batches concatenate to the population generated at once
+    from opentimer_amd import workloads as W
+    cfg = W.config1(num_nets=900, seed=5)
+    whole = W.generate_rc(cfg, nthreads=2)
+    parts = []
+    for a, b in ((0, 250), (250, 600), (600, 900)):
+        c = cfg.scaled(b - a)
+        c.first_net = a
+        parts.append(W.generate_rc(c, nthreads=2))
+    sizes = np.concatenate([np.diff(f.nodes_start) for f in parts])
+    assert np.array_equal(sizes, np.diff(whole.nodes_start))
+    for k in ("edges", "roots", "edge_res", "node_cap", "slew_in"):
+        assert np.array_equal(np.concatenate([getattr(f, k) for f in parts]), getattr(whole, k)), k
+
+
 def test_rc_generator_heavy_tail_classes():
     from opentimer_amd import workloads as W
     f = W.generate_rc(W.config1(num_nets=20000), nthreads=2)
