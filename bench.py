@@ -60,6 +60,17 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(forest_host, topo, budget_s):
     """Oracle (CPU ot::Timer arithmetic, FP32 DFS + slew) over the host cores,
     a thread pool over chunks of 32*256 nets (Timer::_build_rc_timing_tasks
@@ -92,6 +103,7 @@ def cpu_baseline(forest_host, topo, budget_s):
                 break
     med = float(np.median(times))
     return {"value": nnets / med, "unit": "nets/s", "cores": cores, "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"the whole workload ({nnets} nets, {int(ns[-1])} nodes), "
                       f"{len(times)} passes in {sum(times):.1f}s, median pass {med:.3f}s; "
                       f"oracle FP32 DFS + slew, {cores} threads"}
@@ -123,8 +135,25 @@ def run_levelize(ctx, stream, sh, reps=5):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     _, ends, lv = L.host()
+    t0 = time.perf_counter()
     _, oends, olv = O.levelize(csr.n, csr.start, csr.fanin, csr.out_degree, csr.frontier)
+    serial_ms = (time.perf_counter() - t0) * 1e3
     lev["levels_bitexact_vs_oracle"] = bool(np.array_equal(lv, olv) and np.array_equal(ends, oends))
+    # CPU baselines: the sink-first Kahn serially (the run above) and over the
+    # host cores (median of 3 runs, levels checked against the serial run)
+    cores = min(16, len(os.sched_getaffinity(0)))
+    par = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        _, pends, plv = O.levelize_par(csr.n, csr.start, csr.fanin, csr.out_degree, csr.frontier,
+                                       cores)
+        par.append((time.perf_counter() - t0) * 1e3)
+    par_ms = float(np.median(par))
+    lev["cpu_baseline"] = {"serial_ms": serial_ms, "parallel_ms": par_ms, "threads": cores,
+                           "cpu": cpu_model(),
+                           "parallel_levels_equal": bool(np.array_equal(plv, olv)
+                                                         and np.array_equal(pends, oends)),
+                           "gpu_speedup_vs_parallel": par_ms / lms}
     log(f"levelize: {lev}")
     return lev
 

@@ -38,6 +38,7 @@ for name, args, res in [
     ("ora_slew_f32", [C.c_int, vp, vp, vp, vp], None),
     ("ora_slew_f64", [C.c_int, vp, vp, vp, vp], None),
     ("ora_levelize", [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp], C.c_int),
+    ("ora_levelize_par", [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, C.c_int], C.c_int),
     ("ora_levels_longest_path", [C.c_int, vp, vp, vp], None),
     ("ora_counting_argsort", [C.c_int, vp, C.c_int, C.c_int, vp, vp, vp], None),
 ]:
@@ -137,6 +138,24 @@ def levelize(n, fanin_start, fanin, out_degree, frontier0):
     ends = np.zeros(n + 2, np.int32)
     lv = np.zeros(max(n, 1), np.int32)
     ne = _lib.ora_levelize(n, _p(fs), _p(fi), _p(out), len(f0), _p(fr), _p(ends), _p(lv))
+    return fr[:n], ends[:ne], lv[:n]
+
+
+def levelize_par(n, fanin_start, fanin, out_degree, frontier0, nthreads):
+    """The same sink-first Kahn over nthreads threads (the levelizer's CPU
+    baseline).  Levels and frontier ends equal levelize's; the pin order
+    inside a frontier depends on the interleaving.  Returns (frontiers,
+    ends, level)."""
+    fs = _i32(fanin_start)
+    fi = _i32(fanin) if len(fanin) else np.zeros(1, np.int32)
+    out = _i32(out_degree).copy()
+    fr = np.zeros(max(n, 1), np.int32)
+    f0 = _i32(frontier0)
+    fr[: len(f0)] = f0
+    ends = np.zeros(n + 2, np.int32)
+    lv = np.zeros(max(n, 1), np.int32)
+    ne = _lib.ora_levelize_par(n, _p(fs), _p(fi), _p(out), len(f0), _p(fr), _p(ends), _p(lv),
+                               int(nthreads))
     return fr[:n], ends[:ne], lv[:n]
 
 

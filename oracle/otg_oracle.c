@@ -22,7 +22,9 @@
  *   corner index c = el*2 + rf  (ot/headerdef.hpp:172-177)
  * Build with -ffp-contract=off so each float operation rounds once.
  */
+#define _POSIX_C_SOURCE 200809L /* pthread barriers */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -352,6 +354,113 @@ int ora_levelize(int n, const int32_t* fanin_start, const int32_t* fanin, int32_
     ++lv;
   }
   return nends;
+}
+
+/* Parallel sink-first Kahn (a CPU baseline for the levelizer): the same     */
+/* frontiers as ora_levelize, each frontier split into nthreads contiguous    */
+/* parts with atomic out-degree decrements; a thread appends the pins it     */
+/* releases in blocks of up to 1024 (one atomic reservation per block).  The  */
+/* order inside a frontier depends on the thread interleaving; the level of   */
+/* every pin and the frontier sizes do not.                                   */
+typedef struct {
+  int n, nthreads;
+  const int32_t *fanin_start, *fanin;
+  int32_t *out, *frontiers, *ends, *level;
+  int beg, size, lv, nends, done;
+  int cnt; /* pins appended to the next frontier */
+  pthread_barrier_t bar;
+} LvPar;
+
+typedef struct {
+  LvPar* s;
+  int tid;
+} LvArg;
+
+static void* lv_worker(void* vp) {
+  LvArg* a = (LvArg*)vp;
+  LvPar* s = a->s;
+  int32_t buf[1024];
+  for (;;) {
+    const int beg = s->beg, size = s->size, lv = s->lv;
+    int32_t* next = s->frontiers + beg + size;
+    const int lo = (int)((int64_t)size * a->tid / s->nthreads);
+    const int hi = (int)((int64_t)size * (a->tid + 1) / s->nthreads);
+    int nb = 0;
+    for (int id = lo; id < hi; ++id) {
+      const int u = s->frontiers[beg + id];
+      for (int j = s->fanin_start[u]; j < s->fanin_start[u + 1]; ++j) {
+        const int v = s->fanin[j];
+        if (__atomic_fetch_sub(&s->out[v], 1, __ATOMIC_RELAXED) == 1) {
+          s->level[v] = lv + 1;
+          buf[nb++] = v;
+          if (nb == 1024) {
+            const int k = __atomic_fetch_add(&s->cnt, nb, __ATOMIC_RELAXED);
+            memcpy(next + k, buf, sizeof(int32_t) * nb);
+            nb = 0;
+          }
+        }
+      }
+    }
+    if (nb) {
+      const int k = __atomic_fetch_add(&s->cnt, nb, __ATOMIC_RELAXED);
+      memcpy(next + k, buf, sizeof(int32_t) * nb);
+    }
+    pthread_barrier_wait(&s->bar);
+    if (a->tid == 0) {
+      if (s->cnt == 0) {
+        s->done = 1;
+      } else {
+        s->beg += s->size;
+        s->size = s->cnt;
+        s->ends[s->nends++] = s->beg + s->size;
+        s->lv += 1;
+        s->cnt = 0;
+      }
+    }
+    pthread_barrier_wait(&s->bar);
+    if (s->done) break;
+  }
+  return NULL;
+}
+
+int ora_levelize_par(int n, const int32_t* fanin_start, const int32_t* fanin, int32_t* out,
+                     int first_size, int32_t* frontiers, int32_t* ends, int32_t* level,
+                     int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  for (int i = 0; i < n; ++i) level[i] = -1;
+  for (int i = 0; i < first_size; ++i) level[frontiers[i]] = 0;
+  LvPar s;
+  s.n = n;
+  s.nthreads = nthreads;
+  s.fanin_start = fanin_start;
+  s.fanin = fanin;
+  s.out = out;
+  s.frontiers = frontiers;
+  s.ends = ends;
+  s.level = level;
+  s.beg = 0;
+  s.size = first_size;
+  s.lv = 0;
+  s.nends = 0;
+  s.done = 0;
+  s.cnt = 0;
+  ends[s.nends++] = 0;
+  ends[s.nends++] = first_size;
+  if (first_size == 0) return s.nends;
+  pthread_barrier_init(&s.bar, NULL, (unsigned)nthreads);
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+  LvArg* args = (LvArg*)malloc(sizeof(LvArg) * nthreads);
+  for (int t = 0; t < nthreads; ++t) {
+    args[t].s = &s;
+    args[t].tid = t;
+    if (t) pthread_create(&th[t], NULL, lv_worker, &args[t]);
+  }
+  lv_worker(&args[0]);
+  for (int t = 1; t < nthreads; ++t) pthread_join(th[t], NULL);
+  pthread_barrier_destroy(&s.bar);
+  free(th);
+  free(args);
+  return s.nends;
 }
 
 /* Longest-path definition of the same levels, by memoised DFS over fanouts: */

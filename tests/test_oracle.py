@@ -362,3 +362,19 @@ def test_levelize_chain_and_empty():
     frontiers, ends, lv = O.levelize(0, np.zeros(1, np.int32), np.zeros(0, np.int32),
                                      np.zeros(0, np.int32), np.zeros(0, np.int32))
     assert list(ends) == [0, 0]
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_parallel_levelize_matches_serial(threads):
+    # the threaded CPU baseline gives the serial oracle's levels and frontier
+    # ends (pin order inside a frontier may differ), on a DAG whose frontiers
+    # exceed one release block
+    from opentimer_amd import workloads as W
+    dag = W.generate_dag(W.DagConfig("par", 60_000, 150_000, 30, seed=3), nthreads=2)
+    c = W.fanin_csr(dag)
+    fr, ends, lv = O.levelize(c.n, c.start, c.fanin, c.out_degree, c.frontier)
+    pfr, pends, plv = O.levelize_par(c.n, c.start, c.fanin, c.out_degree, c.frontier, threads)
+    assert np.array_equal(plv, lv)
+    assert np.array_equal(pends, ends)
+    for l in range(len(ends) - 1):
+        assert np.array_equal(np.sort(pfr[ends[l]: ends[l + 1]]), np.sort(fr[ends[l]: ends[l + 1]]))
