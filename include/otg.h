@@ -121,6 +121,13 @@ typedef struct otg_rct_tuning {
                           /* compensated method, 512 for FP32)               */
   int32_t small_target;   /* node budget of one packed group (64..256)      */
   int32_t grid_per_cu;    /* persistent workgroups per CU (0 = occupancy)   */
+  int32_t streams;        /* streams the streaming launches of a solve (one */
+                          /* per ring size, global-window nets) are spread  */
+                          /* over, 1..3 (default 3): forked from the        */
+                          /* caller's stream after the packed groups and    */
+                          /* joined back, so that one launch's last, largest*/
+                          /* nets overlap the others' work; 1 under graph   */
+                          /* capture                                        */
 } otg_rct_tuning;
 
 /* Build the work list of a topology (net sizes and parent offsets only):

@@ -88,6 +88,14 @@ extern "C" otg_status otg_destroy(otg_context* ctx) {
   if (!ctx) return OTG_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (int k = 0; k < otg_context::kAux; ++k) {
+    if (ctx->aux[k]) {
+      (void)hipStreamSynchronize(ctx->aux[k]);
+      (void)hipStreamDestroy(ctx->aux[k]);
+    }
+    if (ctx->ev_join[k]) (void)hipEventDestroy(ctx->ev_join[k]);
+  }
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   delete ctx;
   return OTG_OK;
 }
@@ -467,8 +475,8 @@ static bool write_tuning(const char* path, const otg_rct_tuning& t, otg_rct_meth
   if (!f) return false;
   fprintf(f,
           "{\"ring_slots\": %d, \"min_ring\": %d, \"small_target\": %d, "
-          "\"grid_per_cu\": %d, \"method\": %d, \"ms\": %.6f}\n",
-          t.ring_slots, t.min_ring, t.small_target, t.grid_per_cu, (int)m, ms);
+          "\"grid_per_cu\": %d, \"streams\": %d, \"method\": %d, \"ms\": %.6f}\n",
+          t.ring_slots, t.min_ring, t.small_target, t.grid_per_cu, t.streams, (int)m, ms);
   fclose(f);
   return true;
 }
@@ -499,6 +507,7 @@ extern "C" otg_status otg_rct_tuning_load(otg_context* ctx, const char* path, ot
   r.min_ring = json_int(s, "min_ring", 0);
   r.small_target = json_int(s, "small_target", 0);
   r.grid_per_cu = json_int(s, "grid_per_cu", 0);
+  r.streams = json_int(s, "streams", 0);
   ctx->tuning = r;
   if (t) *t = r;
   return OTG_OK;

@@ -16,6 +16,13 @@ struct otg_context {
   int lds_per_cu = 0;
   std::string last_error;
   otg_rct_tuning tuning{};  // defaults for new plans (autotuner may update)
+  // Auxiliary streams that the streaming launches of one batched solve are
+  // spread over (forked from and joined back into the caller's stream),
+  // created on first use.
+  static constexpr int kAux = 2;
+  hipStream_t aux[kAux] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kAux] = {};
 };
 
 // One unit of work of the batched P1 launch: a contiguous range of whole

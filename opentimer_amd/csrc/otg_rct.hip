@@ -2171,6 +2171,8 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   // a packed group holds at most 256 positions (8-bit schedule fields)
   r.small_target = std::max(64, std::min(r.small_target, 256));
   if (!r.grid_per_cu) r.grid_per_cu = d.grid_per_cu;
+  if (!r.streams) r.streams = d.streams ? d.streams : 3;
+  r.streams = std::max(1, std::min(r.streams, 1 + otg_context::kAux));
   // the compensated kernel's ring entries are twice as large, and its
   // occupancy with the 512-slot ring (4 wavefronts per CU) leaves LDS
   // latency exposed: its default splits the streamed nets by ring size
@@ -2272,43 +2274,45 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
     void* args[] = {&Q};
     OTG_HIP(ctx, hipLaunchKernel(pick_group(G, comp), dim3(std::max(1, grid)), dim3(64), args, 0, s));
   }
-  // ---- streamed nets, one launch per ring size (largest ring first) ----------
-  if (plan->num_bins > 0) {
-    int32_t* n_sub = plan->d_queue + 2 * otg_rct_plan::kMaxBins + 2;
-    if (subset && plan->stats[2] > 0) {
-      SubBins sb{};
-      sb.num_bins = plan->num_bins;
-      for (int b = 0; b < plan->num_bins; ++b) {
-        sb.items[b] = plan->d_bin_sub[b];
-        sb.lim[b] = plan->bin_ring[b] - kWave;
-        sb.chain[b] = plan->bin_chain[b];
-      }
-      hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, n_sub, plan->num_bins);
-      hipLaunchKernelGGL(rct_subset_items_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
-                         count, subset, plan->num_nets, tree->nodes_start, plan->d_net_maxpid,
-                         plan->d_net_scratch, plan->d_net_chain, G, sb, n_sub);
-      OTG_HIP(ctx, hipGetLastError());
+  // ---- streamed nets, one launch per ring size (largest ring first), and the
+  // nets whose parent offsets exceed every ring --------------------------------
+  int32_t* n_sub = plan->d_queue + 2 * otg_rct_plan::kMaxBins + 2;
+  if (plan->num_bins > 0 && subset && plan->stats[2] > 0) {
+    SubBins sb{};
+    sb.num_bins = plan->num_bins;
+    for (int b = 0; b < plan->num_bins; ++b) {
+      sb.items[b] = plan->d_bin_sub[b];
+      sb.lim[b] = plan->bin_ring[b] - kWave;
+      sb.chain[b] = plan->bin_chain[b];
     }
-    for (int b = plan->num_bins - 1; b >= 0; --b) {
-      if (plan->bin_items[b] <= 0) continue;
-      RctParams Q = P;
-      Q.queue = plan->d_queue + 2 * b;
-      int total = plan->bin_items[b];
-      Q.items = plan->d_bin_items[b];
-      if (subset) {
-        Q.items = plan->d_bin_sub[b];
-        Q.num_items_dev = n_sub + b;
-        total = std::min(count, plan->bin_items[b]);  // bound; the count is read on the device
-      }
-      Q.num_items = total;
-      const int grid = std::min(plan->bin_grid[b], total);
-      void* args[] = {&Q};
-      OTG_HIP(ctx, hipLaunchKernel(pick_stream(plan->bin_ring[b], comp, plan->bin_chain[b]),
-                                   dim3(grid), dim3(64),
-                                   args, 0, s));
-    }
+    hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, n_sub, plan->num_bins);
+    hipLaunchKernelGGL(rct_subset_items_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+                       count, subset, plan->num_nets, tree->nodes_start, plan->d_net_maxpid,
+                       plan->d_net_scratch, plan->d_net_chain, G, sb, n_sub);
+    OTG_HIP(ctx, hipGetLastError());
   }
-  // ---- nets whose parent offsets exceed the ring ------------------------------
+  struct Launch {
+    void* fn;
+    int grid;
+    RctParams Q;
+  };
+  Launch L[otg_rct_plan::kMaxBins + 1];
+  int nl = 0;
+  for (int b = plan->num_bins - 1; b >= 0; --b) {
+    if (plan->bin_items[b] <= 0) continue;
+    RctParams Q = P;
+    Q.queue = plan->d_queue + 2 * b;
+    int total = plan->bin_items[b];
+    Q.items = plan->d_bin_items[b];
+    if (subset) {
+      Q.items = plan->d_bin_sub[b];
+      Q.num_items_dev = n_sub + b;
+      total = std::min(count, plan->bin_items[b]);  // bound; the count is read on the device
+    }
+    Q.num_items = total;
+    L[nl++] = Launch{pick_stream(plan->bin_ring[b], comp, plan->bin_chain[b]),
+                     std::min(plan->bin_grid[b], total), Q};
+  }
   if (plan->stats[3] > 0) {
     RctParams Q = P;
     Q.queue = plan->d_queue + 2 * otg_rct_plan::kMaxBins;
@@ -2322,10 +2326,49 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
       Q.num_items = plan->num_gitems;
     }
     const int gt = subset ? count : plan->num_gitems;
-    const int grid = std::min(plan->grid, std::max(1, gt));
-    void* args[] = {&Q};
-    OTG_HIP(ctx, hipLaunchKernel(pick_global(comp), dim3(grid), dim3(64), args, 0, s));
+    L[nl++] = Launch{pick_global(comp), std::min(plan->grid, std::max(1, gt)), Q};
   }
+  // These launches take their nets from queues, so they may share the GPU
+  // (the packed-group kernel deals its groups out to a grid that it assumes
+  // resident, so it runs alone, before them).  Launch i goes to stream
+  // n-1-(i mod n) of n: the caller's stream and up to two auxiliary streams
+  // forked from it after the packed groups (one stream under graph
+  // capture).  The launches then start together: one holds the CUs first,
+  // and as its workgroups run out of nets the others take their place, so
+  // the last, largest nets of one launch overlap the work of the others
+  // instead of leaving CUs idle.
+  hipStream_t st[1 + otg_context::kAux];
+  int ns = 1;
+  st[0] = s;
+  const int want = std::min(std::min(plan->tuning.streams, nl), 1 + otg_context::kAux);
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusActive;
+  if (want > 1 && hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone) {
+    if (!ctx->ev_fork)
+      OTG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    for (int k = 0; k + 1 < want; ++k) {
+      if (!ctx->aux[k]) {
+        OTG_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
+        OTG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join[k], hipEventDisableTiming));
+      }
+    }
+    OTG_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
+    for (int k = 0; k + 1 < want; ++k) {
+      OTG_HIP(ctx, hipStreamWaitEvent(ctx->aux[k], ctx->ev_fork, 0));
+      st[ns++] = ctx->aux[k];
+    }
+  }
+  hipError_t err = hipSuccess;
+  for (int i = 0; i < nl && err == hipSuccess; ++i) {
+    void* args[] = {&L[i].Q};
+    err = hipLaunchKernel(L[i].fn, dim3(L[i].grid), dim3(64), args, 0, st[ns - 1 - i % ns]);
+  }
+  // joined on every path, so that the caller's stream orders after all of it
+  for (int k = 1; k < ns; ++k) {
+    const hipError_t e1 = hipEventRecord(ctx->ev_join[k - 1], st[k]);
+    const hipError_t e2 = hipStreamWaitEvent(s, ctx->ev_join[k - 1], 0);
+    if (err == hipSuccess) err = e1 != hipSuccess ? e1 : e2;
+  }
+  OTG_HIP(ctx, err);
   return OTG_OK;
 }
 

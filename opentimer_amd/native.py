@@ -64,7 +64,8 @@ class RctResult(C.Structure):
 
 class RctTuning(C.Structure):
     _fields_ = [("ring_slots", C.c_int32), ("min_ring", C.c_int32),
-                ("small_target", C.c_int32), ("grid_per_cu", C.c_int32)]
+                ("small_target", C.c_int32), ("grid_per_cu", C.c_int32),
+                ("streams", C.c_int32)]
 
 
 class RctEdges(C.Structure):

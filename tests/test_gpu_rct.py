@@ -364,3 +364,31 @@ def test_reference_f64_matches_fp64_oracle(ctx):
     for k in OUT:
         tol = 1e-9 if k in ("impulse", "slew") else 1e-12
         assert rel_err(ref[k], r64[k], fl[k]) < tol, k
+
+
+@pytest.mark.parametrize("method", ["batched", "compensated"])
+def test_streams_do_not_change_results(ctx, method):
+    # the streaming launches spread over 1 or 3 streams give identical bits
+    # (every net is solved by one workgroup, whatever runs beside it), for
+    # full and subset solves
+    cfg = W.RcConfig("mixstreams", 2500, [W.SizeClass(0.7, 2, 40), W.SizeClass(0.2, 100, 3000, N.SHAPE_CHAIN, 0.1),
+                                          W.SizeClass(0.05, 200, 3000, N.SHAPE_CHAIN, 0.0),
+                                          W.SizeClass(0.05, 600, 2500, N.SHAPE_BALANCED)], seed=43)
+    f, s = _forest(cfg, ctx)
+    out = {}
+    for streams in (1, 3):
+        tun = (0, 0, 0, 0, streams)
+        s.solve(method, tuning=tun)
+        out[streams] = {k: v.clone() for k, v in s.out.items()}
+    st = s.plan_stats(method, (0, 0, 0, 0, 3))
+    assert st["stream_nets"] > 0
+    for k in OUT:
+        assert torch.equal(out[1][k], out[3][k]), k
+    ids = torch.arange(0, cfg.num_nets, 7, dtype=torch.int32, device=s.device)
+    for streams in (1, 3):
+        for k in OUT:
+            s.out[k].zero_()
+        s.solve_subset(ids, method, tuning=(0, 0, 0, 0, streams))
+        out[streams] = {k: v.clone() for k, v in s.out.items()}
+    for k in OUT:
+        assert torch.equal(out[1][k], out[3][k]), k
