@@ -242,11 +242,183 @@ __device__ __forceinline__ void bfs_net(const BfsParams& P, int net, int32_t* ba
   M::sync();
 }
 
+// ---------------------------------------------------------------------------
+// Small nets (at most kSmallNodes nodes): consecutive small nets are packed
+// into one wavefront, one lane per node and one lane per edge.  Distances
+// come from rounds of edge relaxation (one level per round: a round's reads
+// of dist are issued before its writes); the position of node u is counted
+// directly from its net's distances, #(smaller distance) + #(equal distance,
+// larger id), the rule of the reference's counting sort.
+// ---------------------------------------------------------------------------
+constexpr int kSmallNodes = 64;
+
+// Orders this wavefront's LDS accesses around it (LDS operations of one
+// wavefront execute in issue order; the clobber keeps the compiler from
+// carrying LDS values in registers across it).
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_wave_barrier();
+  __asm__ __volatile__("" ::: "memory");
+}
+
+// Inclusive prefix maximum over the wavefront.
+__device__ __forceinline__ int wave_prefix_max(int v, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int t = __shfl_up(v, d);
+    if (lane >= d) v = max(v, t);
+  }
+  return v;
+}
+
+// Smallest failing net of the batch (-1: none); reported like the large-net
+// path reports its nets.
+__device__ __forceinline__ bool small_batch_failed(const BfsParams& P, int a, int badj, int lane) {
+  const int b = __shfl(wave_prefix_max(badj >= 0 ? -badj : INT32_MIN, lane), kWave - 1);
+  if (b == INT32_MIN) return false;
+  if (lane == 0) atomicMin(P.bad, a - b);
+  return true;
+}
+
+// Solves nets a .. a+k-1, which start at position `base` and hold at most
+// kSmallNodes nodes together.
+__device__ __forceinline__ void bfs_small_batch(const BfsParams& P, int a, int k, int base,
+                                                int32_t* dist, int32_t* pos, int32_t* mark,
+                                                int lane) {
+  // this lane as net j of the batch: its first slot and size
+  const int j = lane;
+  const bool isnet = j < k;
+  const int sj = isnet ? P.nodes_start[a + j] - base : 0;
+  const int nj = isnet ? P.nodes_start[a + j + 1] - base - sj : 0;
+  const int rootj = isnet ? P.roots[a + j] : 0;
+  const int m = __shfl(sj + nj, k - 1);  // nodes of the batch
+  const int me = m - k;                   // edges of the batch
+  const int ebase = base - a;             // first edge of net a
+  int badj = -1;                          // a failing net of this lane
+  if (isnet && nj > 0 && (rootj < 0 || rootj >= nj)) badj = j;
+  // node slot -> net: prefix maximum over marks at each net's first slot
+  // (nets without nodes own no slot)
+  mark[lane] = -1;
+  lds_sync();
+  if (isnet && nj > 0) mark[sj] = j;
+  lds_sync();
+  const int nnet = wave_prefix_max(mark[lane], lane);
+  // edge slot -> net: marks at each net's first edge (nets with edges only)
+  lds_sync();
+  mark[lane] = -1;
+  lds_sync();
+  if (isnet && nj > 1) mark[sj - j] = j;
+  lds_sync();
+  const int enet = wave_prefix_max(mark[lane], lane);
+  // node: local id, net start slot and size
+  const bool isnode = lane < m;
+  const int ns0 = __shfl(sj, nnet < 0 ? 0 : nnet);
+  const int nn = __shfl(nj, nnet < 0 ? 0 : nnet);
+  const int rt = __shfl(rootj, nnet < 0 ? 0 : nnet);
+  const int u = lane - ns0;
+  dist[lane] = (isnode && u == rt) ? 0 : -1;
+  // edge: its endpoints as slots
+  const bool isedge = lane < me;
+  const int es0 = __shfl(sj, enet < 0 ? 0 : enet);
+  const int en = __shfl(nj, enet < 0 ? 0 : enet);
+  int es = 0, et = 0, ss = 0, ts = 0;
+  if (isedge) {
+    es = P.edges_in[2LL * (ebase + lane)];
+    et = P.edges_in[2LL * (ebase + lane) + 1];
+    if (es < 0 || es >= en || et < 0 || et >= en || es == et) badj = enet;
+    ss = es0 + (es < 0 || es >= en ? 0 : es);
+    ts = es0 + (et < 0 || et >= en ? 0 : et);
+  }
+  if (small_batch_failed(P, a, badj, lane)) return;
+  lds_sync();
+  // relaxation rounds (at most one per node)
+  for (int r = 0; r < m; ++r) {
+    bool ch = false;
+    if (isedge) {
+      const int ds = dist[ss], dt = dist[ts];
+      if (ds >= 0 && dt < 0) {
+        dist[ts] = ds + 1;
+        ch = true;
+      } else if (dt >= 0 && ds < 0) {
+        dist[ss] = dt + 1;
+        ch = true;
+      }
+    }
+    lds_sync();
+    if (!__any(ch)) break;
+  }
+  const int du = isnode ? dist[lane] : 0;
+  // a node left unreached: not a tree
+  if (small_batch_failed(P, a, isnode && du < 0 ? nnet : -1, lane)) return;
+  // position inside the net
+  int p = 0;
+  const int nmax = __shfl(wave_prefix_max(isnode ? nn : 0, lane), kWave - 1);
+  for (int v = 0; v < nmax; ++v) {
+    const int dv = dist[ns0 + (v < nn ? v : 0)];
+    if (isnode && v < nn && (dv < du || (dv == du && v > u))) ++p;
+  }
+  pos[lane] = p;
+  lds_sync();
+  if (isnode) {
+    const int64_t g = (int64_t)base + lane;
+    P.node2bfs[g] = p;
+    P.cap_bfs[(int64_t)base + ns0 + p] = P.node_cap[g];
+  }
+  if (isnet && nj > 0) {
+    P.pid[base + sj] = 0;
+    P.pres[base + sj] = 0.0f;
+  }
+  if (isedge) {
+    int s0 = es, t0 = et;
+    int ps = pos[ss], pt = pos[ts];
+    if (ps > pt) {
+      const int x = s0; s0 = t0; t0 = x;
+      const int y = ps; ps = pt; pt = y;
+    }
+    P.edges_out[2LL * (ebase + lane)] = s0;
+    P.edges_out[2LL * (ebase + lane) + 1] = t0;
+    P.pid[(int64_t)base + es0 + pt] = pt - ps;
+    P.pres[(int64_t)base + es0 + pt] = P.edge_res[ebase + lane];
+  }
+  lds_sync();
+}
+
+// Each wavefront takes ranges of kSmallRange consecutive nets and packs the
+// small ones; larger nets are left to the large-net kernels.
+constexpr int kSmallRange = 256;
+
+__global__ __launch_bounds__(64) void rct_bfs_small_kernel(BfsParams P) {
+  __shared__ int32_t dist[kWave], pos[kWave], mark[kWave];
+  const int lane = threadIdx.x;
+  for (int r0 = blockIdx.x * kSmallRange; r0 < P.num_nets; r0 += gridDim.x * kSmallRange) {
+    const int r1 = min(P.num_nets, r0 + kSmallRange);
+    int a = r0;
+    while (a < r1) {
+      // nets a, a+1, ... while the batch holds at most 64 nodes
+      const int base = P.nodes_start[a];
+      const int l = a + lane;
+      // (a net without nodes ends the run: it has no edge slots, so the
+      // nets after it would start their edges before the batch's first)
+      const bool fits = l < r1 && P.nodes_start[l + 1] - base <= kSmallNodes &&
+                        P.nodes_start[l + 1] > P.nodes_start[l];
+      const uint64_t fm = __ballot(fits);
+      // fits is monotone in the lane: the batch is the run of leading lanes
+      const int k = fm == ~0ull ? kWave : (int)__builtin_ctzll(~fm);
+      if (k == 0) {
+        ++a;  // a large net (or one without nodes)
+        continue;
+      }
+      bfs_small_batch(P, a, k, base, dist, pos, mark, lane);
+      a += k;
+    }
+  }
+}
+
 __global__ __launch_bounds__(64) void rct_bfs_kernel(BfsParams P) {
   __shared__ int32_t lds[6 * kLdsNodes + 2];
   const int lane = threadIdx.x;
   for (int net = blockIdx.x; net < P.num_nets; net += gridDim.x) {
     const int n = P.nodes_start[net + 1] - P.nodes_start[net];
+    if (n <= kSmallNodes) continue;  // rct_bfs_small_kernel
     if (n <= kLdsNodes) {
       bfs_net<false>(P, net, lds, lds, 0, lane);
     } else {
@@ -284,6 +456,9 @@ otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t
                                                    64, 0) != hipSuccess || per_cu <= 0)
     per_cu = 8;
   const int grid = std::max(1, std::min(in->num_nets, ctx->num_cus * per_cu));
+  const int sgrid = std::max(1, std::min((in->num_nets + kSmallRange - 1) / kSmallRange,
+                                         ctx->num_cus * 16));
+  hipLaunchKernelGGL(rct_bfs_small_kernel, dim3(sgrid), dim3(64), 0, s, P);
   hipLaunchKernelGGL(rct_bfs_kernel, dim3(grid), dim3(64), 0, s, P);
   hipError_t le = hipGetLastError();
   OTG_HIP(ctx, hipFreeAsync(gs, s));

@@ -66,6 +66,44 @@ def test_bfs_bitexact(cfg0):
     assert np.array_equal(h["cap"].view(np.uint32), cap.view(np.uint32))
 
 
+@pytest.mark.parametrize("seed", [31, 32])
+def test_bfs_bitexact_mixed_sizes(ctx, seed):
+    # every BFS path against the oracle's ordering: packed small nets (up to
+    # 64 nodes, one-node nets included), large nets on chip and nets larger
+    # than the on-chip scratch (chains, balanced trees, random trees)
+    cfg = W.RcConfig("bfsmix", 1200, [W.SizeClass(0.45, 1, 12), W.SizeClass(0.25, 13, 64),
+                                      W.SizeClass(0.2, 65, 3000, N.SHAPE_CHAIN, 0.1),
+                                      W.SizeClass(0.1, 500, 20000, N.SHAPE_RANDOM)], seed=seed)
+    f = W.generate_rc(cfg)
+    s = RcSolver.from_edges(f, ctx)
+    e, n2b, pid, pres, cap = oracle_inputs(f)
+    h = s.host_topology()
+    assert np.array_equal(h["node2bfs"], n2b)
+    assert np.array_equal(h["pid"], pid)
+    assert np.array_equal(s.edges_directed.cpu().numpy(), e)
+    assert np.array_equal(h["pres"].view(np.uint32), pres.view(np.uint32))
+    assert np.array_equal(h["cap"].view(np.uint32), cap.view(np.uint32))
+
+
+def test_not_a_tree_rejected_in_each_bfs_path(ctx):
+    # a broken net among small nets, and one among large nets: the smallest
+    # failing net is reported whichever path met it
+    cfg = W.RcConfig("bfsbad", 300, [W.SizeClass(0.8, 2, 20), W.SizeClass(0.2, 100, 2000, N.SHAPE_CHAIN, 0.1)],
+                     seed=7)
+    for pick in ("small", "large"):
+        f = W.generate_rc(cfg)
+        sizes = np.diff(f.nodes_start)
+        k = int(np.flatnonzero(sizes <= 20)[5] if pick == "small" else np.flatnonzero(sizes >= 100)[0])
+        e0 = int(f.nodes_start[k] - k)
+        bad = f.edges.copy()
+        bad[2 * e0], bad[2 * e0 + 1] = bad[2 * e0 + 2], bad[2 * e0 + 3]  # a duplicated edge
+        f.edges = bad
+        with pytest.raises(N.OtgError) as ei:
+            RcSolver.from_edges(f, ctx)
+        assert ei.value.status == 2
+        assert f"net {k}" in str(ei.value), (pick, k, str(ei.value))
+
+
 def test_plain_bitexact_vs_reference_arithmetic(cfg0):
     f, s = cfg0
     h = s.host_topology()
