@@ -41,12 +41,13 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-levelize", action="store_true")
+    ap.add_argument("--no-fprop", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--tuning", default=os.path.join(ROOT, "opentimer_amd", "tuning.json"))
     ap.add_argument("--only", default="",
-                    help="rct | plain | levelize | copy (profiling runs) | precision (cfg3 report) "
+                    help="rct | plain | levelize | fprop | copy (profiling runs) | precision (cfg3 report) "
                          "| incremental (cfg4 report) | spef (SPEF loader + GPU solve)")
     ap.add_argument("--spef-nets", type=int, default=400_000,
                     help="nets of the cfg1 distribution written as SPEF for --only spef")
@@ -539,6 +540,75 @@ def run_spef(ctx, nets, method):
             pass
 
 
+def run_fprop(ctx, stream, sh, reps=10, cpu=True):
+    """Level-wise forward propagation (slew, arc delay, arrival with NLDM
+    lookups) over the levelization graph: timed with HIP events on the launch
+    stream around `reps` whole propagations (one launch per level), checked
+    bit for bit against the oracle on the whole graph; the oracle's run is
+    the single-core CPU baseline."""
+    import torch
+
+    from opentimer_amd import workloads as W
+    from opentimer_amd.levelize import Levelizer
+    from opentimer_amd.prop import Propagator
+    dcfg = W.config2()
+    dag = W.generate_dag(dcfg, nthreads=16)
+    csr = W.fanin_csr(dag)
+    del dag
+    lib = W.prop_library()
+    g = W.prop_graph(csr, lib)
+    L = Levelizer(csr, ctx)
+    L.run(stream=sh)
+    levels = L.num_ends - 1
+    P = Propagator(g, lib, ctx)
+    for _ in range(2):
+        P.run(L.frontiers, L.ends, levels, stream=sh)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        P.run(L.frontiers, L.ends, levels, stream=sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    n, m = csr.n, len(csr.fanin)
+    # per pin: fanin_start, frontier entry, load, PI slew and arrival, slew
+    # and arrival out (4 + 4 + 5 x 16 B); per arc: source, kind, timing,
+    # RC delay and impulse, the source's slew and arrival, 8 arc delays out
+    # (3 x 4 + 4 x 16 + 32 B)
+    alg = 88 * n + 108 * m
+    out = {"workload": f"{dcfg.name} graph, synthetic 7x8 NLDM library ({lib.num_timings} "
+                       f"timings, {lib.num_luts} tables)",
+           "pins": n, "arcs": m, "levels": levels, "ms": ms, "pins_per_s": n / (ms / 1e3),
+           "arcs_per_s": m / (ms / 1e3), "us_per_level": ms * 1e3 / levels,
+           "algorithmic_bytes": alg, "achieved_GBps": alg / (ms / 1e3) / 1e9,
+           "frac": alg / (ms / 1e3) / 1e9 / PEAK_HBM_GBS}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        slew, at, ad = P.host()
+        fr, ends, _ = L.host()
+        gk = ("fanin_start", "fanin", "arc_kind", "arc_timing", "arc_rc_delay", "arc_rc_impulse",
+              "pin_load", "pi_slew", "pi_at")
+        lk = ("lut_hdr", "lut_val", "timing_sense", "timing_edge", "timing_lut", "timing_has")
+        t0 = time.perf_counter()
+        os_, oa, od = O.fprop({k: getattr(g, k) for k in gk}, {k: getattr(lib, k) for k in lk},
+                              fr, ends)
+        cms = (time.perf_counter() - t0) * 1e3
+
+        def same(a, b):
+            return bool(np.array_equal(np.isnan(a), np.isnan(b))
+                        and np.array_equal(a[~np.isnan(a)].view(np.int32),
+                                           b[~np.isnan(b)].view(np.int32)))
+        out["bitexact_vs_oracle"] = {"slew": same(slew, os_), "at": same(at, oa),
+                                     "arc_delay": same(ad, od)}
+        out["cpu_baseline"] = {"value": n / (cms / 1e3), "unit": "pins/s", "cores": 1,
+                               "kind": "port", "cpu": cpu_model(), "ms": cms,
+                               "sample": "the whole graph, one propagation"}
+    log(f"fprop: {out}")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -562,6 +632,9 @@ def main():
 
     if args.only == "levelize":
         print(json.dumps(run_levelize(ctx, stream, sh)))
+        return
+    if args.only == "fprop":
+        print(json.dumps(run_fprop(ctx, stream, sh, reps=args.steps, cpu=not args.no_cpu)))
         return
     if args.only == "precision":
         dev_ref = args.precision_ref == "device" or (args.precision_ref == "auto"
@@ -705,6 +778,9 @@ def main():
     lev = None
     if not args.no_levelize:
         lev = run_levelize(ctx, stream, sh)
+    fprop = None
+    if not args.no_fprop:
+        fprop = run_fprop(ctx, stream, sh, cpu=not args.no_cpu)
 
     achieved = alg_bytes / (ms / 1e3) / 1e9
     traffic = None
@@ -750,6 +826,7 @@ def main():
         "bfs_ordering_ms": bfs_ms,
         "plan_build_ms": {m: plans[m]["build_ms"] for m in plans},
         "levelize": lev,
+        "fprop": fprop,
         "parity": parity,
     }
     print(json.dumps(out))

@@ -97,6 +97,23 @@ class IncrGraph(C.Structure):
                 ("out_degree", vp), ("net_driver", vp), ("level", vp)]
 
 
+class PropLib(C.Structure):
+    _fields_ = [("num_luts", C.c_int32), ("num_timings", C.c_int32),
+                ("num_lut_vals", C.c_int32), ("lut_hdr", vp),
+                ("lut_val", vp), ("timing_sense", vp), ("timing_edge", vp),
+                ("timing_lut", vp), ("timing_has", vp)]
+
+
+class PropGraph(C.Structure):
+    _fields_ = [("num_pins", C.c_int32), ("num_arcs", C.c_int32), ("fanin_start", vp),
+                ("fanin", vp), ("arc_kind", vp), ("arc_timing", vp), ("arc_rc_delay", vp),
+                ("arc_rc_impulse", vp), ("pin_load", vp), ("pi_slew", vp), ("pi_at", vp)]
+
+
+class PropResult(C.Structure):
+    _fields_ = [("slew", vp), ("at", vp), ("arc_delay", vp)]
+
+
 class GenSizeClass(C.Structure):
     _fields_ = [("weight", C.c_double), ("min_nodes", C.c_int32), ("max_nodes", C.c_int32),
                 ("shape", C.c_int32), ("pad", C.c_int32), ("branch_prob", C.c_double)]
@@ -175,6 +192,8 @@ _sig("otg_incr_destroy", C.c_int, C.c_void_p)
 _sig("otg_incr_relevelize", C.c_int, C.c_void_p, vp, C.c_int32, vp)
 _sig("otg_incr_query", C.c_int, C.c_void_p, vp, i32p, i32p, C.POINTER(C.c_void_p),
      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p))
+_sig("otg_fprop", C.c_int, ctx_p, C.POINTER(PropGraph), C.POINTER(PropLib), vp, vp, C.c_int32,
+     C.POINTER(PropResult), vp)
 _sig("otg_graph_begin", C.c_int, ctx_p, vp)
 _sig("otg_graph_end", C.c_int, ctx_p, vp, C.POINTER(C.c_void_p))
 _sig("otg_graph_launch", C.c_int, ctx_p, C.c_void_p, vp)
@@ -206,7 +225,7 @@ EXPORTED = [
     "otg_rct_compute", "otg_rct_query",
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize",
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",
-    "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
+    "otg_fprop", "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
     "otg_gen_rc_sizes", "otg_gen_rc_fill", "otg_gen_dag", "otg_build_fanin_csr",
     "otg_spef_read", "otg_spef_read_buffer", "otg_spef_last_error", "otg_spef_forest_get",
     "otg_spef_net_name", "otg_spef_node_name", "otg_spef_find_net", "otg_spef_find_node",

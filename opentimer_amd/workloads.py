@@ -213,3 +213,150 @@ def fanin_csr(dag: Dag, cand=None, skip_arc=None) -> FaninCsr:
                                       N.addr(s), N.addr(start), N.addr(fanin), N.addr(out),
                                       N.addr(fr), C.byref(fs), C.byref(kept)))
     return FaninCsr(n, start, fanin[: kept.value], out[:n], fr[: fs.value])
+
+
+# ---------------------------------------------------------------------------
+# Forward propagation: a synthetic NLDM library and per-arc / per-pin inputs
+# over a levelization DAG.  Layout as in include/otg_prop.h.
+# ---------------------------------------------------------------------------
+
+@dataclass
+class PropLibrary:
+    lut_hdr: np.ndarray       # [4L] int32 {n1, n2, var1, off}
+    lut_val: np.ndarray       # float32 pool
+    timing_sense: np.ndarray  # [T] int32
+    timing_edge: np.ndarray   # [T] int32
+    timing_lut: np.ndarray    # [8T] int32
+    timing_has: np.ndarray    # [2T] int32
+
+    @property
+    def num_luts(self):
+        return len(self.lut_hdr) // 4
+
+    @property
+    def num_timings(self):
+        return len(self.timing_sense)
+
+
+class _LutPool:
+    def __init__(self):
+        self.hdr, self.val = [], []
+
+    def add(self, i1, i2, table, var1):
+        off = sum(len(v) for v in self.val)
+        self.hdr += [len(i1), len(i2), var1, off]
+        self.val.append(np.concatenate([np.asarray(i1, np.float32), np.asarray(i2, np.float32),
+                                        np.asarray(table, np.float32).reshape(-1)]))
+        return len(self.hdr) // 4 - 1
+
+    def arrays(self):
+        return (np.asarray(self.hdr, np.int32),
+                np.concatenate(self.val) if self.val else np.zeros(1, np.float32))
+
+
+def prop_library(num_timings: int = 64, seed: int = 11) -> PropLibrary:
+    """A seeded library shaped like the TAU 2015 cells: 7x8 slew-by-load
+    tables (input transition 5..350 ps, output load 1..200 fF) with delays of
+    5..40 ps and output transitions of 3..30 ps, monotone in both operands.
+    Mixed in: load-first templates, 1xN / Nx1 tables, scalar tables, absent
+    tables, all three senses, rising/falling-edge triggers and timings with
+    no view for one split."""
+    rng = np.random.default_rng(seed)
+    pool = _LutPool()
+    slews = np.array([5, 30, 50, 80, 140, 200, 350], np.float32)
+    loads = np.array([1, 5, 10, 15, 20, 50, 100, 200], np.float32)
+
+    def table(kind):
+        base = rng.uniform(3, 8) if kind == "delay" else rng.uniform(2, 4)
+        ks = rng.uniform(0.005, 0.02) if kind == "delay" else rng.uniform(0.002, 0.01)
+        kl = rng.uniform(0.05, 0.15) if kind == "delay" else rng.uniform(0.04, 0.12)
+        t = base + ks * slews[:, None] + kl * loads[None, :]
+        return np.round(t + rng.uniform(0, 0.01, t.shape), 3)
+
+    def make(kind):
+        shape = rng.choice(5, p=[0.70, 0.12, 0.06, 0.06, 0.06])
+        t = table(kind)
+        if shape == 0:
+            return pool.add(slews, loads, t, 0)
+        if shape == 1:  # load-first template
+            return pool.add(loads, slews, t.T, 1)
+        if shape == 2:  # 1 x N: one slew, indexed by load
+            return pool.add(slews[:1], loads, t[:1], 0)
+        if shape == 3:  # N x 1: indexed by slew, one load
+            return pool.add(slews, loads[:1], t[:, :1], 0)
+        return pool.add([0.0], [0.0], t[:1, :1], -1)  # scalar, no template
+
+    T = num_timings
+    sense = rng.choice(3, size=T, p=[0.35, 0.45, 0.20]).astype(np.int32)
+    edge = np.zeros(T, np.int32)
+    trig = rng.random(T) < 0.08
+    edge[trig] = rng.choice([1, 2], size=int(trig.sum())).astype(np.int32)
+    sense[trig] = 2
+    tlut = np.full(8 * T, -1, np.int32)
+    has = np.ones(2 * T, np.int32)
+    for t in range(T):
+        for el in range(2):
+            for k in range(4):
+                if rng.random() < 0.02:
+                    continue  # an absent table
+                tlut[8 * t + 4 * el + k] = make("delay" if k < 2 else "slew")
+        if rng.random() < 0.04:
+            has[2 * t + int(rng.integers(2))] = 0
+    hdr, val = pool.arrays()
+    return PropLibrary(hdr, val, sense, edge, tlut, has)
+
+
+@dataclass
+class PropGraphData:
+    fanin_start: np.ndarray     # [n+1] int32
+    fanin: np.ndarray           # [m] int32
+    arc_kind: np.ndarray        # [m] int32
+    arc_timing: np.ndarray      # [m] int32
+    arc_rc_delay: np.ndarray    # [m,4] float32
+    arc_rc_impulse: np.ndarray  # [m,4] float32
+    pin_load: np.ndarray        # [n,4] float32
+    pi_slew: np.ndarray         # [n,4] float32
+    pi_at: np.ndarray           # [n,4] float32
+
+    @property
+    def n(self):
+        return len(self.fanin_start) - 1
+
+    @property
+    def m(self):
+        return len(self.fanin)
+
+
+def prop_graph(csr: "FaninCsr", lib: PropLibrary, seed: int = 12, net_fraction: float = 0.5,
+               empty_fraction: float = 0.03, skip_fraction: float = 0.01,
+               missing_sink_fraction: float = 0.005) -> PropGraphData:
+    """Per-arc and per-pin inputs over a fanin CSR: arcs are RC net arcs
+    (delay 0..5 ps, impulse 0..40 ps^2), net arcs without parasitics, cell
+    arcs of a random timing, or loop breakers; a few RC net arcs lead to a
+    sink outside the RC tree (NaN delay and impulse).  Every pin without
+    fanin is a primary input (slew 5..20 ps, arrival 0..10 ps) except 2% left
+    unconstrained; pin loads are 1..200 fF per corner."""
+    rng = np.random.default_rng(seed)
+    n, m = csr.n, len(csr.fanin)
+    u = rng.random(m)
+    kind = np.full(m, 2, np.int32)
+    kind[u < net_fraction] = 0
+    kind[(u >= net_fraction) & (u < net_fraction + empty_fraction)] = 1
+    kind[u >= 1.0 - skip_fraction] = 3
+    timing = rng.integers(0, max(lib.num_timings, 1), size=m, dtype=np.int32)
+    rcd = rng.uniform(0.0, 5.0, size=(m, 4)).astype(np.float32)
+    rci = rng.uniform(0.0, 40.0, size=(m, 4)).astype(np.float32)
+    miss = (kind == 0) & (rng.random(m) < missing_sink_fraction)
+    rcd[miss] = np.nan
+    rci[miss] = np.nan
+    load = rng.uniform(1.0, 200.0, size=(n, 4)).astype(np.float32)
+    deg = np.diff(csr.start)
+    pi = (deg == 0) & (rng.random(n) >= 0.02)
+    pis = np.full((n, 4), np.nan, np.float32)
+    pia = np.full((n, 4), np.nan, np.float32)
+    k = int(pi.sum())
+    pis[pi] = rng.uniform(5.0, 20.0, size=(k, 4)).astype(np.float32)
+    pia[pi] = rng.uniform(0.0, 10.0, size=(k, 4)).astype(np.float32)
+    return PropGraphData(np.ascontiguousarray(csr.start, np.int32),
+                         np.ascontiguousarray(csr.fanin, np.int32), kind, timing, rcd, rci,
+                         load, pis, pia)

@@ -41,6 +41,8 @@ for name, args, res in [
     ("ora_levelize_par", [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, C.c_int], C.c_int),
     ("ora_levels_longest_path", [C.c_int, vp, vp, vp], None),
     ("ora_counting_argsort", [C.c_int, vp, C.c_int, C.c_int, vp, vp, vp], None),
+    ("ora_lut_eval", [vp, vp, C.c_int, C.c_float, C.c_float], C.c_float),
+    ("ora_fprop", [C.c_int] + [vp] * 17 + [C.c_int, vp, vp, vp], None),
 ]:
     f = getattr(_lib, name)
     f.argtypes = args
@@ -176,3 +178,40 @@ def levels_longest_path(n, fanout_start, fanout):
     lv = np.zeros(max(n, 1), np.int32)
     _lib.ora_levels_longest_path(n, _p(fs), _p(fo), _p(lv))
     return lv[:n]
+
+
+def lut_eval(hdr, pool, l, val1, val2):
+    """Lut::operator() (ot/liberty/lut.cpp:161-241) on table l of a flat pool
+    laid out as in include/otg_prop.h."""
+    h = _i32(hdr)
+    p = _f32(pool)
+    return float(_lib.ora_lut_eval(_p(h), _p(p), int(l), float(val1), float(val2)))
+
+
+def fprop(graph, lib, frontiers, ends):
+    """Level-wise forward propagation (timer.cpp:733-783, arc.cpp:82-147).
+
+    graph: dict with fanin_start, fanin, arc_kind, arc_timing, arc_rc_delay
+    [m,4], arc_rc_impulse [m,4], pin_load [n,4], pi_slew [n,4], pi_at [n,4];
+    lib: dict with lut_hdr, lut_val, timing_sense, timing_edge, timing_lut,
+    timing_has.  Returns (slew [n,4], at [n,4], arc_delay [m,8]); pins outside
+    the frontiers keep NaN."""
+    fs = _i32(graph["fanin_start"])
+    n = len(fs) - 1
+    m = int(fs[-1])
+    one_i = lambda a: _i32(a) if len(a) else np.zeros(1, np.int32)
+    one_f = lambda a: _f32(a).reshape(-1) if np.size(a) else np.zeros(1, np.float32)
+    fi, kind, tim = one_i(graph["fanin"]), one_i(graph["arc_kind"]), one_i(graph["arc_timing"])
+    rcd, rci = one_f(graph["arc_rc_delay"]), one_f(graph["arc_rc_impulse"])
+    load, pis, pia = one_f(graph["pin_load"]), one_f(graph["pi_slew"]), one_f(graph["pi_at"])
+    sense, edge = one_i(lib["timing_sense"]), one_i(lib["timing_edge"])
+    tlut, thas = one_i(lib["timing_lut"]), one_i(lib["timing_has"])
+    hdr, pool = one_i(lib["lut_hdr"]), one_f(lib["lut_val"])
+    fr, en = one_i(frontiers), _i32(ends)
+    slew = np.full(max(4 * n, 1), np.nan, np.float32)
+    at = np.full(max(4 * n, 1), np.nan, np.float32)
+    ad = np.full(max(8 * m, 1), np.nan, np.float32)
+    _lib.ora_fprop(n, _p(fs), _p(fi), _p(kind), _p(tim), _p(rcd), _p(rci), _p(load), _p(pis),
+                   _p(pia), _p(sense), _p(edge), _p(tlut), _p(thas), _p(hdr), _p(pool), _p(fr),
+                   _p(en), len(en) - 1, _p(slew), _p(at), _p(ad))
+    return slew[: 4 * n].reshape(n, 4), at[: 4 * n].reshape(n, 4), ad[: 8 * m].reshape(m, 8)

@@ -23,6 +23,7 @@
  * Build with -ffp-contract=off so each float operation rounds once.
  */
 #define _POSIX_C_SOURCE 200809L /* pthread barriers */
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
@@ -492,4 +493,134 @@ void ora_levels_longest_path(int n, const int32_t* fanout_start, const int32_t* 
     }
   }
   free(stack); free(iter);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Forward propagation (slew, arc delay, arrival) over a levelized graph     */
+/* with NLDM tables.  NaN stands for the reference's unset std::optional.   */
+/* Table layout, arc kinds, senses and triggers as in include/otg_prop.h.   */
+/* ------------------------------------------------------------------------ */
+
+/* Lut::operator() (ot/liberty/lut.cpp:161-241): the linear inter/extra-     */
+/* polation lambda, lower_bound on each index clamped to [1, size-1], and   */
+/* the 1xN, Nx1 and NxN cases.                                              */
+static float lut_interpolate(float x, float x1, float x2, float y1, float y2) {
+  if (x >= FLT_MAX || x <= -FLT_MAX) return x;
+  const float slope = (y2 - y1) / (x2 - x1);
+  if (x < x1) return y1 - (x1 - x) * slope;
+  else if (x > x2) return y2 + (x - x2) * slope;
+  else if (x == x1) return y1;
+  else if (x == x2) return y2;
+  else return y1 + (x - x1) * slope;
+}
+
+static int lut_lower_bound(const float* a, int n, float v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+float ora_lut_eval(const int32_t* hdr, const float* pool, int l, float val1, float val2) {
+  const int n1 = hdr[4 * l], n2 = hdr[4 * l + 1], off = hdr[4 * l + 3];
+  const float* i1 = pool + off;
+  const float* i2 = i1 + n1;
+  const float* tab = i2 + n2;
+  if (n1 == 1 && n2 == 1) return tab[0];
+  int a1 = lut_lower_bound(i1, n1, val1), b1 = lut_lower_bound(i2, n2, val2);
+  a1 = a1 < n1 - 1 ? a1 : n1 - 1;
+  a1 = a1 > 1 ? a1 : 1;
+  b1 = b1 < n2 - 1 ? b1 : n2 - 1;
+  b1 = b1 > 1 ? b1 : 1;
+  const int a0 = a1 - 1, b0 = b1 - 1;
+  if (n1 == 1) return lut_interpolate(val2, i2[b0], i2[b1], tab[b0], tab[b1]);
+  if (n2 == 1) return lut_interpolate(val1, i1[a0], i1[a1], tab[a0 * n2], tab[a1 * n2]);
+  const float v0 = lut_interpolate(val1, i1[a0], i1[a1], tab[a0 * n2 + b0], tab[a1 * n2 + b0]);
+  const float v1 = lut_interpolate(val1, i1[a0], i1[a1], tab[a0 * n2 + b1], tab[a1 * n2 + b1]);
+  return lut_interpolate(val2, i2[b0], i2[b1], v0, v1);
+}
+
+/* Timing::delay / Timing::slew (ot/liberty/timing.cpp:197-232, 411-540):  */
+/* transition rule, table by output transition (k = 0/1 cell_rise/fall,     */
+/* 2/3 rise/fall_transition), operands in the template's variable order.   */
+static float timing_value(const int32_t* sense, const int32_t* edge, const int32_t* tlut,
+                          const int32_t* hdr, const float* pool, int t, int el, int frf, int trf,
+                          int kind, float slew, float load) {
+  if (edge[t] == 1 && frf != 0) return NAN;
+  if (edge[t] == 2 && frf != 1) return NAN;
+  if (sense[t] == 0 && frf != trf) return NAN;
+  if (sense[t] == 1 && frf == trf) return NAN;
+  const int l = tlut[8 * t + 4 * el + kind + trf];
+  if (l < 0) return NAN;
+  if (hdr[4 * l + 2] == 1) return ora_lut_eval(hdr, pool, l, load, slew);
+  return ora_lut_eval(hdr, pool, l, slew, load);
+}
+
+/* Pin::_relax_slew / _relax_at (ot/timer/pin.cpp:324-360): MIN keeps the   */
+/* smaller value, MAX the larger; an unset value takes any.                  */
+static float relax(float cur, float v, int el) {
+  if (isnan(v)) return cur;
+  if (isnan(cur)) return v;
+  return el == 0 ? (v < cur ? v : cur) : (v > cur ? v : cur);
+}
+
+/* Timer::_fprop_slew / _fprop_delay / _fprop_at (ot/timer/timer.cpp:      */
+/* 733-783) with Arc::_fprop_* (ot/timer/arc.cpp:82-147) for every pin of   */
+/* the levels, from the last level (sources) to level 0.                    */
+void ora_fprop(int n, const int32_t* fs, const int32_t* fi, const int32_t* kind,
+               const int32_t* timing, const float* rc_delay, const float* rc_imp,
+               const float* load, const float* pi_slew, const float* pi_at, const int32_t* sense,
+               const int32_t* edge, const int32_t* tlut, const int32_t* thas, const int32_t* hdr,
+               const float* pool, const int32_t* frontiers, const int32_t* ends, int num_levels,
+               float* slew, float* at, float* arc_delay) {
+  (void)n;
+  for (int lv = num_levels - 1; lv >= 0; --lv) {
+    for (int q = ends[lv]; q < ends[lv + 1]; ++q) {
+      const int v = frontiers[q];
+      for (int el = 0; el < 2; ++el)
+        for (int trf = 0; trf < 2; ++trf) {
+          const int c = el * 2 + trf;
+          float s = pi_slew[4 * v + c], a = pi_at[4 * v + c];
+          for (int j = fs[v]; j < fs[v + 1]; ++j) {
+            const int u = fi[j], k = kind[j];
+            for (int frf = 0; frf < 2; ++frf) {
+              const float su = slew[4 * u + el * 2 + frf];
+              const float au = at[4 * u + el * 2 + frf];
+              /* every delay of the pin's fanin arcs is reset first (timer.cpp:757-759) */
+              float so = NAN, d = NAN;
+              if (k == 3) {
+                /* loop breaker: nothing propagates */
+              } else if (k == 0 || k == 1) {
+                /* net arcs carry rf -> rf only */
+                if (frf == trf) {
+                  if (k == 0) {
+                    const float imp = rc_imp[4 * j + c];
+                    if (!isnan(su)) so = su < 0.0f ? -sqrtf(su * su + imp) : sqrtf(su * su + imp);
+                    d = rc_delay[4 * j + c];
+                  } else {
+                    so = su;
+                    d = 0.0f;
+                  }
+                }
+              } else {
+                const int t = timing[j];
+                if (thas[2 * t + el] && !isnan(su)) {
+                  const float lc = load[4 * v + c];
+                  so = timing_value(sense, edge, tlut, hdr, pool, t, el, frf, trf, 2, su, lc);
+                  d = timing_value(sense, edge, tlut, hdr, pool, t, el, frf, trf, 0, su, lc);
+                }
+              }
+              s = relax(s, so, el);
+              if (arc_delay) arc_delay[8 * j + 4 * el + 2 * frf + trf] = d;
+              if (!isnan(d) && !isnan(au)) a = relax(a, d + au, el);
+            }
+          }
+          slew[4 * v + c] = s;
+          at[4 * v + c] = a;
+        }
+    }
+  }
 }

@@ -1,0 +1,121 @@
+"""GPU parity of level-wise forward propagation (otg_fprop) against the
+oracle's restatement of the reference (timer.cpp:733-783, arc.cpp:82-147,
+lut.cpp:161-241).
+
+Bar: bit-exact.  Slew, arrival and every arc delay equal the oracle's FP32
+values exactly, NaN (unset) in the same places: the kernel performs the same
+operations in the same order with contraction off.  The level order comes
+from the GPU levelizer; the oracle is run on the same frontiers (the order
+of pins inside a level does not change any value).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+import oracle as O
+from opentimer_amd import native as N
+from opentimer_amd import workloads as W
+from opentimer_amd.levelize import Levelizer
+from opentimer_amd.prop import Propagator
+
+GKEYS = ("fanin_start", "fanin", "arc_kind", "arc_timing", "arc_rc_delay", "arc_rc_impulse",
+         "pin_load", "pi_slew", "pi_at")
+LKEYS = ("lut_hdr", "lut_val", "timing_sense", "timing_edge", "timing_lut", "timing_has")
+
+
+def _oracle(g, lib, fr, ends):
+    return O.fprop({k: getattr(g, k) for k in GKEYS}, {k: getattr(lib, k) for k in LKEYS},
+                   fr, ends)
+
+
+def _same(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape
+    assert np.array_equal(np.isnan(a), np.isnan(b)), int((np.isnan(a) != np.isnan(b)).sum())
+    np.testing.assert_array_equal(a[~np.isnan(a)].view(np.int32), b[~np.isnan(b)].view(np.int32))
+
+
+def _run(ctx, dcfg, lib_seed=11, graph_seed=12, num_timings=64, **kw):
+    print(f"fprop case {dcfg.name} pins={dcfg.num_pins} arcs={dcfg.num_arcs} "
+          f"levels={dcfg.num_layers} lib_seed={lib_seed} graph_seed={graph_seed} "
+          f"timings={num_timings} {kw}", flush=True)
+    dag = W.generate_dag(dcfg, nthreads=8)
+    csr = W.fanin_csr(dag)
+    lib = W.prop_library(num_timings=num_timings, seed=lib_seed)
+    g = W.prop_graph(csr, lib, seed=graph_seed, **kw)
+    L = Levelizer(csr, ctx)
+    L.run()
+    P = Propagator(g, lib, ctx)
+    P.run(L.frontiers, L.ends, L.num_ends - 1)
+    slew, at, ad = P.host()
+    fr, ends, _ = L.host()
+    oslew, oat, oad = _oracle(g, lib, fr, ends)
+    _same(slew, oslew)
+    _same(at, oat)
+    _same(ad, oad)
+    return slew, at, ad, P, L
+
+
+@pytest.mark.parametrize("cfg", [W.config0_dag(),
+                                 W.DagConfig("dag-300k", 300_000, 750_000, 60, seed=21)],
+                         ids=lambda c: c.name)
+def test_fprop_bitexact(ctx, cfg):
+    slew, at, _, _, _ = _run(ctx, cfg)
+    assert np.isfinite(at).mean() > 0.9
+
+
+def _lds_words(lib):
+    return 4 * lib.num_luts + 12 * lib.num_timings + len(lib.lut_val)
+
+
+@pytest.mark.parametrize("num_timings", [3, 160])
+def test_fprop_library_in_lds_and_in_device_memory(ctx, num_timings):
+    # 36864 words is the LDS copy's limit: the first library is read from
+    # LDS, the second from device memory
+    lib = W.prop_library(num_timings=num_timings, seed=11)
+    assert (_lds_words(lib) <= 36864) == (num_timings == 3)
+    _run(ctx, W.config0_dag(seed=13), num_timings=num_timings)
+
+
+def test_fprop_all_arc_kinds_and_dense_misses(ctx):
+    # many net arcs without parasitics, loop breakers and sinks outside the tree
+    cfg = W.DagConfig("dag-50k-mixed", 50_000, 140_000, 30, seed=22)
+    _run(ctx, cfg, lib_seed=5, graph_seed=6, net_fraction=0.3, empty_fraction=0.2,
+         skip_fraction=0.1, missing_sink_fraction=0.2)
+
+
+def test_fprop_rerun_and_no_arc_delay(ctx):
+    # a second run over the same outputs gives the same values; without the
+    # arc-delay output the pin values do not change
+    cfg = W.config0_dag(seed=9)
+    slew, at, ad, P, L = _run(ctx, cfg)
+    P.arc_delay.fill_(1234.0)
+    P.run(L.frontiers, L.ends, L.num_ends - 1, arc_delay=False)
+    s2, a2, ad2 = P.host()
+    _same(s2, slew)
+    _same(a2, at)
+    assert (ad2 == 1234.0).all()
+
+
+def test_fprop_empty_and_invalid(ctx):
+    lib = W.prop_library(num_timings=2)
+    g = W.PropGraphData(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32),
+                        np.zeros(0, np.int32), np.zeros((0, 4), np.float32),
+                        np.zeros((0, 4), np.float32), np.zeros((0, 4), np.float32),
+                        np.zeros((0, 4), np.float32), np.zeros((0, 4), np.float32))
+    P = Propagator(g, lib, ctx)
+    z = torch.zeros(2, dtype=torch.int32, device="cuda")
+    P.run(z, z, 0)
+    torch.cuda.synchronize()
+    out = N.PropResult(P.slew.data_ptr(), P.at.data_ptr(), None)
+    st = N.lib.otg_fprop(ctx.ptr, C.byref(P.graph), C.byref(P.lib), z.data_ptr(), z.data_ptr(),
+                         -1, C.byref(out), None)
+    assert st == 1  # OTG_ERR_INVALID_ARGUMENT
+    st = N.lib.otg_fprop(ctx.ptr, None, C.byref(P.lib), z.data_ptr(), z.data_ptr(), 1,
+                         C.byref(out), None)
+    assert st == 1

@@ -1,0 +1,338 @@
+"""CPU pinning of the oracle's forward propagation (oracle/otg_oracle.c,
+ora_lut_eval / ora_fprop) against what the mathematics fixes:
+
+* NLDM lookup (ot/liberty/lut.cpp:161-241): bilinear interpolation reproduces
+  a bilinear function exactly at every point, inside the grid and by linear
+  extrapolation outside it; grid points return the table entry bit for bit;
+  1xN / Nx1 tables are numpy.interp inside their range; scalars ignore the
+  operands; an infinite operand is returned as is.
+* propagation (timer.cpp:733-783, arc.cpp:82-147): on random DAGs with
+  scalar tables the arrival and slew of every pin equal the min (early) /
+  max (late) over every source-to-pin path enumerated one by one, each path
+  evaluated in its own order through the sense and trigger rules -- exact
+  equality, since every step (d + at, sqrt(s*s + impulse), a constant) is
+  monotone, so relaxing per pin and per path give the same floats.
+* the rules each arc kind follows, on hand-built cases.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle as O  # noqa: E402
+
+NAN = float("nan")
+
+
+def _pool(*tables):
+    hdr, val = [], []
+    for i1, i2, tab, var1 in tables:
+        hdr += [len(i1), len(i2), var1, sum(len(v) for v in val)]
+        val.append(np.concatenate([np.asarray(i1, np.float32), np.asarray(i2, np.float32),
+                                   np.asarray(tab, np.float32).reshape(-1)]))
+    return np.asarray(hdr, np.int32), np.concatenate(val)
+
+
+# -------------------------------------------------------------------- LUT --
+def test_lut_bilinear_function_reproduced_everywhere():
+    rng = np.random.default_rng(0)
+    i1 = np.array([5, 30, 50, 80, 140, 200, 350], np.float64)
+    i2 = np.array([1, 5, 10, 15, 20, 50, 100, 200], np.float64)
+    a, b, c, d = 2.5, 0.125, 0.0625, 0.001953125  # dyadic: table entries exact in fp32
+    f = lambda x, y: a + b * x + c * y + d * x * y
+    hdr, pool = _pool((i1, i2, f(i1[:, None], i2[None, :]), 0))
+    xs = np.concatenate([rng.uniform(-100, 600, 300), i1, [5.0, 350.0, 0.0, 1000.0]])
+    ys = np.concatenate([rng.uniform(-50, 400, 300), i2[:7], [1.0, 200.0, 0.5, 900.0, 7.0]])
+    for x, y in zip(xs.astype(np.float32), ys.astype(np.float32)):
+        got = O.lut_eval(hdr, pool, 0, x, y)
+        want = f(float(x), float(y))
+        assert abs(got - want) <= 2e-5 * max(1.0, abs(want)), (x, y, got, want)
+
+
+def test_lut_grid_points_exact_and_cell_corners():
+    rng = np.random.default_rng(1)
+    i1 = np.array([5, 30, 50, 80, 140, 200, 350], np.float32)
+    i2 = np.array([1, 5, 10, 15, 20, 50, 100, 200], np.float32)
+    tab = rng.uniform(1, 30, (7, 8)).astype(np.float32)
+    hdr, pool = _pool((i1, i2, tab, 0))
+    for i in range(7):
+        for j in range(8):
+            assert O.lut_eval(hdr, pool, 0, i1[i], i2[j]) == tab[i, j]
+
+
+def test_lut_one_dimensional_tables_match_numpy_interp():
+    rng = np.random.default_rng(2)
+    i2 = np.array([1, 5, 10, 15, 20, 50, 100, 200], np.float32)
+    row = np.sort(rng.uniform(1, 30, 8)).astype(np.float32)
+    i1 = np.array([5, 30, 50, 80, 140, 200, 350], np.float32)
+    col = np.sort(rng.uniform(1, 30, 7)).astype(np.float32)
+    hdr, pool = _pool(([7.0], i2, row[None, :], 0), (i1, [3.0], col[:, None], 0))
+    for y in rng.uniform(1, 200, 200).astype(np.float32):
+        # the first operand of a 1xN table is ignored
+        got = O.lut_eval(hdr, pool, 0, 1e6, y)
+        assert abs(got - np.interp(y, i2, row)) <= 1e-5 * 30
+    for x in rng.uniform(5, 350, 200).astype(np.float32):
+        got = O.lut_eval(hdr, pool, 1, x, -1e6)
+        assert abs(got - np.interp(x, i1, col)) <= 1e-5 * 30
+    # extrapolation continues the end segments
+    s0 = (row[1] - row[0]) / (i2[1] - i2[0])
+    assert abs(O.lut_eval(hdr, pool, 0, 0, -9.0) - (row[0] - 10.0 * s0)) <= 1e-4
+    s1 = (col[-1] - col[-2]) / (i1[-1] - i1[-2])
+    assert abs(O.lut_eval(hdr, pool, 1, 450.0, 0) - (col[-1] + 100.0 * s1)) <= 1e-4
+
+
+def test_lut_scalar_and_infinite_operands():
+    hdr, pool = _pool(([0.0], [0.0], [[4.25]], -1), ([1, 2], [10, 20], [[1, 2], [3, 4]], 0))
+    for x, y in [(0, 0), (-5, 1e9), (NAN, NAN), (3.0, 7.0)]:
+        assert O.lut_eval(hdr, pool, 0, x, y) == np.float32(4.25)
+    big = np.float32(np.finfo(np.float32).max)
+    # the interpolation returns an operand at +-FLT_MAX unchanged
+    assert O.lut_eval(hdr, pool, 1, big, 15.0) == big
+    assert O.lut_eval(hdr, pool, 1, 1.5, float("inf")) == float("inf")
+
+
+# -------------------------------------------------------------- helpers --
+def _graph(n, arcs, kind, timing=None, rcd=None, rci=None, load=None, pis=None, pia=None):
+    """arcs: list of (u, v); builds the fanin CSR in arc order per pin."""
+    order = sorted(range(len(arcs)), key=lambda j: (arcs[j][1], j))
+    fs = np.zeros(n + 1, np.int32)
+    for _, v in arcs:
+        fs[v + 1] += 1
+    fs = np.cumsum(fs).astype(np.int32)
+    m = len(arcs)
+    pick = lambda a, shape, fill: (np.full(shape, fill, np.float32) if a is None
+                                   else np.asarray(a, np.float32).reshape(shape))
+    g = {
+        "fanin_start": fs,
+        "fanin": np.array([arcs[j][0] for j in order], np.int32),
+        "arc_kind": np.array([kind[j] for j in order], np.int32),
+        "arc_timing": np.array([(timing[j] if timing is not None else 0) for j in order], np.int32),
+        "arc_rc_delay": pick(rcd, (m, 4), 0.0)[order] if m else np.zeros((0, 4), np.float32),
+        "arc_rc_impulse": pick(rci, (m, 4), 0.0)[order] if m else np.zeros((0, 4), np.float32),
+        "pin_load": pick(load, (n, 4), 1.0),
+        "pi_slew": pick(pis, (n, 4), NAN),
+        "pi_at": pick(pia, (n, 4), NAN),
+    }
+    return g, order
+
+
+def _levels(g):
+    n = len(g["fanin_start"]) - 1
+    fo = [[] for _ in range(n)]
+    for v in range(n):
+        for j in range(g["fanin_start"][v], g["fanin_start"][v + 1]):
+            fo[g["fanin"][j]].append(v)
+    out = np.array([len(x) for x in fo], np.int32)
+    fr0 = np.flatnonzero(out == 0).astype(np.int32)
+    fr, ends, _ = O.levelize(n, g["fanin_start"], g["fanin"], out, fr0)
+    return fr, ends
+
+
+def _scalar_lib(T, rng, senses=None, edges=None):
+    """Per timing and split: constant delay / slew tables (scalars)."""
+    tables, tlut = [], np.full(8 * T, -1, np.int32)
+    vals = {}
+    for t in range(T):
+        for el in range(2):
+            for k in range(4):
+                v = float(np.float32(rng.uniform(1, 9) if k < 2 else rng.uniform(2, 20)))
+                tlut[8 * t + 4 * el + k] = len(tables)
+                vals[(t, el, k)] = v
+                tables.append(([0.0], [0.0], [[v]], -1))
+    hdr, pool = _pool(*tables)
+    sense = np.asarray(senses if senses is not None else rng.integers(0, 3, T), np.int32)
+    edge = np.asarray(edges if edges is not None else np.zeros(T), np.int32)
+    lib = {"lut_hdr": hdr, "lut_val": pool, "timing_sense": sense, "timing_edge": edge,
+           "timing_lut": tlut, "timing_has": np.ones(2 * T, np.int32)}
+    return lib, vals
+
+
+def _allowed(sense, edge, frf, trf):
+    if edge == 1 and frf != 0:
+        return False
+    if edge == 2 and frf != 1:
+        return False
+    if sense == 0 and frf != trf:
+        return False
+    if sense == 1 and frf == trf:
+        return False
+    return True
+
+
+# ---------------------------------------------------- path brute force --
+@pytest.mark.parametrize("seed", range(6))
+def test_fprop_equals_min_max_over_enumerated_paths(seed):
+    rng = np.random.default_rng(100 + seed)
+    n = 24
+    perm = rng.permutation(n)
+    arcs, kind, timing = [], [], []
+    for a in range(n):
+        for b in range(a + 1, n):
+            if rng.random() < 0.18:
+                arcs.append((int(perm[a]), int(perm[b])))
+                kind.append(int(rng.choice(4, p=[0.35, 0.1, 0.45, 0.1])))
+                timing.append(int(rng.integers(0, 5)))
+    m = len(arcs)
+    rcd = rng.uniform(0, 5, (m, 4)).astype(np.float32)
+    rci = rng.uniform(0, 30, (m, 4)).astype(np.float32)
+    indeg = np.zeros(n, int)
+    for _, v in arcs:
+        indeg[v] += 1
+    pis = np.full((n, 4), NAN, np.float32)
+    pia = np.full((n, 4), NAN, np.float32)
+    for v in range(n):
+        if indeg[v] == 0 or rng.random() < 0.1:  # sources, and a few inner pins
+            pis[v] = rng.uniform(3, 15, 4)
+            pia[v] = rng.uniform(0, 8, 4)
+    lib, vals = _scalar_lib(5, rng, edges=rng.choice([0, 0, 0, 1, 2], 5))
+    g, order = _graph(n, arcs, kind, timing, rcd, rci, None, pis, pia)
+    fr, ends = _levels(g)
+    slew, at, ad = O.fprop(g, lib, fr, ends)
+
+    # brute force: every path from every pin with a PI value, one by one
+    fo = [[] for _ in range(n)]
+    for j, (u, v) in enumerate(arcs):
+        fo[u].append(j)
+    best_at = np.full((n, 4), NAN)
+    best_sl = np.full((n, 4), NAN)
+
+    def relax(arr, v, c, val, el):
+        cur = arr[v, c]
+        if math.isnan(cur) or (val < cur if el == 0 else val > cur):
+            arr[v, c] = val
+
+    def walk(v, el, rf, a, s):
+        c = el * 2 + rf
+        if a is not None:
+            relax(best_at, v, c, a, el)
+        if s is not None:
+            relax(best_sl, v, c, s, el)
+        for j in fo[v]:
+            w, k = arcs[j][1], kind[j]
+            if k == 3:
+                continue
+            for trf in range(2):
+                cc = el * 2 + trf
+                if k in (0, 1):
+                    if trf != rf:
+                        continue
+                    if k == 0:
+                        d = rcd[j, cc]
+                        s2 = None if s is None else float(np.sqrt(np.float32(
+                            np.float32(s) * np.float32(s) + rci[j, cc])))
+                    else:
+                        d, s2 = np.float32(0.0), s
+                else:
+                    t = timing[j]
+                    if s is None or not _allowed(lib["timing_sense"][t], lib["timing_edge"][t], rf, trf):
+                        continue
+                    d = np.float32(vals[(t, el, trf)])
+                    s2 = vals[(t, el, 2 + trf)]
+                a2 = None if a is None else float(np.float32(d) + np.float32(a))
+                walk(w, el, trf, a2, s2)
+
+    for v in range(n):
+        for el in range(2):
+            for rf in range(2):
+                c = el * 2 + rf
+                a = None if np.isnan(pia[v, c]) else float(pia[v, c])
+                s = None if np.isnan(pis[v, c]) else float(pis[v, c])
+                if a is not None or s is not None:
+                    walk(v, el, rf, a, s)
+    np.testing.assert_array_equal(at, best_at.astype(np.float32))
+    np.testing.assert_array_equal(slew, best_sl.astype(np.float32))
+    assert np.isfinite(at).sum() > 20
+
+
+# --------------------------------------------------------- arc rules ----
+def test_net_arcs_carry_each_transition_to_itself_only():
+    # 0 --rc--> 1 --empty--> 2
+    g, _ = _graph(3, [(0, 1), (1, 2)], [0, 1],
+                  rcd=[[1, 2, 3, 4], [9, 9, 9, 9]], rci=[[16, 9, 0, 33], [0, 0, 0, 0]],
+                  pis=[[3, -4, 0, 4], [NAN] * 4, [NAN] * 4], pia=[[10, 20, 30, 40], [NAN] * 4, [NAN] * 4])
+    lib, _ = _scalar_lib(1, np.random.default_rng(0))
+    slew, at, ad = O.fprop(g, lib, *_levels(g))
+    np.testing.assert_array_equal(slew[1], np.float32([5, -5, 0, np.sqrt(np.float32(49))]))
+    np.testing.assert_array_equal(at[1], np.float32([11, 22, 33, 44]))
+    np.testing.assert_array_equal(slew[2], slew[1])
+    np.testing.assert_array_equal(at[2], at[1])
+    # arc_delay[8j + 4el + 2frf + trf]: only frf == trf is set on a net arc
+    d0 = ad[0].reshape(2, 2, 2)
+    assert np.isnan(d0[:, 0, 1]).all() and np.isnan(d0[:, 1, 0]).all()
+    np.testing.assert_array_equal(d0[:, 0, 0], [1, 3])
+    np.testing.assert_array_equal(d0[:, 1, 1], [2, 4])
+    np.testing.assert_array_equal(ad[1].reshape(2, 2, 2)[:, [0, 1], [0, 1]], 0)
+
+
+def test_missing_sink_no_view_and_loop_breaker_propagate_nothing():
+    # 0 -rc(NaN: sink outside the tree)-> 1;  0 -cell(no late view)-> 2;  0 -skip-> 3
+    pis = [[5] * 4, [NAN] * 4, [NAN] * 4, [NAN] * 4]
+    pia = [[1] * 4, [NAN] * 4, [NAN] * 4, [NAN] * 4]
+    g, _ = _graph(4, [(0, 1), (0, 2), (0, 3)], [0, 2, 3], timing=[0, 0, 0],
+                  rcd=[[NAN] * 4, [0] * 4, [0] * 4], rci=[[NAN] * 4, [0] * 4, [0] * 4],
+                  pis=pis, pia=pia)
+    lib, vals = _scalar_lib(1, np.random.default_rng(3), senses=[2])
+    lib["timing_has"] = np.array([1, 0], np.int32)
+    slew, at, ad = O.fprop(g, lib, *_levels(g))
+    assert np.isnan(slew[1]).all() and np.isnan(at[1]).all() and np.isnan(ad[0]).all()
+    assert np.isnan(slew[3]).all() and np.isnan(at[3]).all() and np.isnan(ad[2]).all()
+    # non-unate early view: both inputs reach both outputs; late: nothing
+    for trf in range(2):
+        assert at[2, trf] == np.float32(vals[(0, 0, trf)]) + np.float32(1)
+        assert slew[2, trf] == np.float32(vals[(0, 0, 2 + trf)])
+    assert np.isnan(at[2, 2:]).all() and np.isnan(ad[1, 4:]).all()
+
+
+@pytest.mark.parametrize("sense,edge,live", [
+    (0, 0, {(0, 0), (1, 1)}), (1, 0, {(0, 1), (1, 0)}), (2, 0, {(0, 0), (0, 1), (1, 0), (1, 1)}),
+    (2, 1, {(0, 0), (0, 1)}), (2, 2, {(1, 0), (1, 1)})])
+def test_cell_arc_transition_rules(sense, edge, live):
+    g, _ = _graph(2, [(0, 1)], [2], timing=[0], pis=[[5] * 4, [NAN] * 4],
+                  pia=[[0, 100, 0, 100], [NAN] * 4])
+    lib, vals = _scalar_lib(1, np.random.default_rng(4), senses=[sense], edges=[edge])
+    _, at, ad = O.fprop(g, lib, *_levels(g))
+    for el in range(2):
+        for frf in range(2):
+            for trf in range(2):
+                d = ad[0, 4 * el + 2 * frf + trf]
+                if (frf, trf) in live:
+                    assert d == np.float32(vals[(0, el, trf)])
+                else:
+                    assert np.isnan(d)
+
+
+def test_load_first_template_equals_transposed_slew_first():
+    rng = np.random.default_rng(5)
+    i1 = np.array([5, 30, 50, 80, 140, 200, 350], np.float32)
+    i2 = np.array([1, 5, 10, 15, 20, 50, 100, 200], np.float32)
+    tab = np.cumsum(np.cumsum(rng.uniform(0.1, 1, (7, 8)), 0), 1).astype(np.float32)
+    for var1 in (0, 1):
+        tables = [(i1, i2, tab, 0) if var1 == 0 else (i2, i1, tab.T, 1)] * 4
+        hdr, pool = _pool(*tables)
+        lib = {"lut_hdr": hdr, "lut_val": pool, "timing_sense": np.array([2], np.int32),
+               "timing_edge": np.array([0], np.int32),
+               "timing_lut": np.array([0, 1, 2, 3, 0, 1, 2, 3], np.int32),
+               "timing_has": np.array([1, 1], np.int32)}
+        g, _ = _graph(2, [(0, 1)], [2], timing=[0], load=[[1] * 4, [17, 33, 60, 150]],
+                      pis=[[12, 47, 90, 260], [NAN] * 4], pia=[[0] * 4, [NAN] * 4])
+        res = O.fprop(g, lib, *_levels(g))
+        if var1 == 0:
+            first = res
+        else:
+            # the same bilinear value, interpolated in the other order
+            for a, b in zip(first, res):
+                np.testing.assert_allclose(a, b, rtol=1e-6, atol=0)
+    # and the value itself: bilinear at (slew, load) of the late fall corner
+    want = O.lut_eval(hdr, pool, 0, 150.0, 260.0)  # load-first pool: (load, slew)
+    assert res[1][1, 3] == np.float32(max(want, O.lut_eval(hdr, pool, 0, 150.0, 90.0)))
+
+
+def test_empty_graph_and_levels():
+    g, _ = _graph(0, [], [])
+    lib, _ = _scalar_lib(1, np.random.default_rng(6))
+    s, a, d = O.fprop(g, lib, np.zeros(0, np.int32), np.zeros(1, np.int32))
+    assert s.shape == (0, 4) and a.shape == (0, 4) and d.shape == (0, 8)
