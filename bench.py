@@ -561,16 +561,26 @@ def run_fprop(ctx, stream, sh, reps=10, cpu=True):
     L.run(stream=sh)
     levels = L.num_ends - 1
     P = Propagator(g, lib, ctx)
-    for _ in range(2):
-        P.run(L.frontiers, L.ends, levels, stream=sh)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        P.run(L.frontiers, L.ends, levels, stream=sh)
-    e1.record(stream)
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+    # pin order, one launch per level (otg_fprop)
+    direct_ms = timed(lambda: P.run(L.frontiers, L.ends, levels, stream=sh))
+    # the level-ordered plan, built once per levelization (otg_fprop_plan_*)
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+    t0 = time.perf_counter()
+    P.build_plan(L.frontiers, L.ends, levels, stream=sh)
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    ms = timed(lambda: P.run_plan(stream=sh))
     n, m = csr.n, len(csr.fanin)
     # per pin: fanin_start, frontier entry, load, PI slew and arrival, slew
     # and arrival out (4 + 4 + 5 x 16 B); per arc: source, kind, timing,
@@ -582,10 +592,15 @@ def run_fprop(ctx, stream, sh, reps=10, cpu=True):
            "pins": n, "arcs": m, "levels": levels, "ms": ms, "pins_per_s": n / (ms / 1e3),
            "arcs_per_s": m / (ms / 1e3), "us_per_level": ms * 1e3 / levels,
            "algorithmic_bytes": alg, "achieved_GBps": alg / (ms / 1e3) / 1e9,
-           "frac": alg / (ms / 1e3) / 1e9 / PEAK_HBM_GBS}
+           "frac": alg / (ms / 1e3) / 1e9 / PEAK_HBM_GBS, "path": "level-ordered plan",
+           "plan_build_ms": plan_ms, "pin_order_ms": direct_ms}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
+        # the outputs of one plan run alone (the pin-order runs wrote them too)
+        for t in (P.slew, P.at, P.arc_delay):
+            t.fill_(float("nan"))
+        P.run_plan(stream=sh)
         slew, at, ad = P.host()
         fr, ends, _ = L.host()
         gk = ("fanin_start", "fanin", "arc_kind", "arc_timing", "arc_rc_delay", "arc_rc_impulse",

@@ -117,6 +117,33 @@ otg_status otg_fprop(otg_context* ctx, const otg_prop_graph* graph, const otg_pr
                      const int32_t* frontiers, const int32_t* frontier_ends,
                      int32_t num_levels, const otg_prop_result* out, otg_stream stream);
 
+/* A level-ordered plan of a levelized graph for repeated propagations (no
+ * counterpart in the reference, which walks its pins in place: the plan is
+ * this library's layout for the same per-level tasks, timer.cpp:1079-1086).
+ * The pins of the frontiers are renumbered to their position in the
+ * frontiers, so that the pins of a level are consecutive, and their fanin
+ * arcs to consecutive slots in that order.  Built once per levelization
+ * (allocates, synchronises `stream`); the arrays it was built from may be
+ * freed afterwards. */
+typedef struct otg_fprop_plan otg_fprop_plan;
+
+otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, int32_t num_arcs,
+                                 const int32_t* fanin_start, const int32_t* fanin,
+                                 const int32_t* frontiers, const int32_t* frontier_ends,
+                                 int32_t num_levels, otg_stream stream, otg_fprop_plan** plan);
+otg_status otg_fprop_plan_destroy(otg_fprop_plan* plan);
+/* Pins in the plan's levels and their fanin arcs. */
+otg_status otg_fprop_plan_sizes(const otg_fprop_plan* plan, int32_t* positions, int32_t* slots);
+
+/* otg_fprop over a plan: the same results, bit for bit.  The graph's
+ * per-arc and per-pin inputs are gathered into plan order, the levels are
+ * propagated over contiguous data, and slew and arrival are scattered back
+ * to pin order (arc delays are written in arc order directly).  graph must
+ * have the plan's CSR (its fanin_start / fanin are not read again).
+ * Asynchronous on `stream`; no allocation. */
+otg_status otg_fprop_run(otg_context* ctx, otg_fprop_plan* plan, const otg_prop_graph* graph,
+                         const otg_prop_lib* lib, const otg_prop_result* out, otg_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

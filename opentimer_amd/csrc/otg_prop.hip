@@ -12,9 +12,11 @@
 // in FP32 with contraction off (division and square root correctly
 // rounded), so results equal the restatement in the oracle bit for bit.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdint>
 
 #include "otg_internal.hpp"
 #include "otg_prop.h"
@@ -100,6 +102,65 @@ __device__ __forceinline__ float table_value(const LibView& L, int l, float slew
   return L.hdr[4 * l + 2] == 1 ? lut_eval(L, l, load, slew) : lut_eval(L, l, slew, load);
 }
 
+// table_value of table l at the slews s0 and s1 (the two input transitions)
+// and one load, where ok0 / ok1: the header and the indices are read once
+// for both.
+__device__ __forceinline__ void table_pair(const LibView& L, int l, float s0, float s1,
+                                           float load, bool ok0, bool ok1, float& r0, float& r1) {
+  r0 = r1 = __builtin_nanf("");
+  if (l < 0 || !(ok0 || ok1)) return;
+  const int n1 = L.hdr[4 * l], n2 = L.hdr[4 * l + 1], var1 = L.hdr[4 * l + 2],
+            off = L.hdr[4 * l + 3];
+  const float* i1 = L.pool + off;
+  const float* i2 = i1 + n1;
+  const float* tab = i2 + n2;
+  if (n1 == 1 && n2 == 1) {
+    if (ok0) r0 = tab[0];
+    if (ok1) r1 = tab[0];
+    return;
+  }
+  if (n1 > kUnrolledIdx || n2 > kUnrolledIdx) {
+    if (ok0) r0 = table_value(L, l, s0, load);
+    if (ok1) r1 = table_value(L, l, s1, load);
+    return;
+  }
+  float x1[kUnrolledIdx], x2[kUnrolledIdx];
+#pragma unroll
+  for (int k = 0; k < kUnrolledIdx; ++k) {
+    x1[k] = k < n1 ? i1[k] : 0.0f;
+    x2[k] = k < n2 ? i2[k] : 0.0f;
+  }
+  const bool lf = var1 == 1;
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    if (!(f ? ok1 : ok0)) continue;
+    const float sv = f ? s1 : s0;
+    const float val1 = lf ? load : sv, val2 = lf ? sv : load;
+    int c1 = 0, c2 = 0;
+#pragma unroll
+    for (int k = 0; k < kUnrolledIdx; ++k) {
+      if (k < n1) c1 += x1[k] < val1 ? 1 : 0;
+      if (k < n2) c2 += x2[k] < val2 ? 1 : 0;
+    }
+    const int a1 = max(1, min(c1, n1 - 1)), b1 = max(1, min(c2, n2 - 1));
+    const int a0 = a1 - 1, b0 = b1 - 1;
+    float r;
+    if (n1 == 1) {
+      r = lut_interpolate(val2, i2[b0], i2[b1], tab[b0], tab[b1]);
+    } else if (n2 == 1) {
+      r = lut_interpolate(val1, i1[a0], i1[a1], tab[a0 * n2], tab[a1 * n2]);
+    } else {
+      const float v0 =
+          lut_interpolate(val1, i1[a0], i1[a1], tab[a0 * n2 + b0], tab[a1 * n2 + b0]);
+      const float v1 =
+          lut_interpolate(val1, i1[a0], i1[a1], tab[a0 * n2 + b1], tab[a1 * n2 + b1]);
+      r = lut_interpolate(val2, i2[b0], i2[b1], v0, v1);
+    }
+    if (f) r1 = r;
+    else r0 = r;
+  }
+}
+
 // Timing::is_transition_defined (ot/liberty/timing.cpp:197-225).
 __device__ __forceinline__ bool transition_defined(int sense, int edge, int frf, int trf) {
   if (edge == OTG_EDGE_RISING && frf != 0) return false;
@@ -116,116 +177,515 @@ __device__ __forceinline__ float relax(float cur, float v, int el) {
   return el == 0 ? (v < cur ? v : cur) : (v > cur ? v : cur);
 }
 
+// Slew and delay of one arc for corner (el, trf) from both input
+// transitions f (so[f], d[f]; NaN where nothing propagates): Arc::_fprop_slew
+// and Arc::_fprop_delay (arc.cpp:82-135) for one arc.
+__device__ __forceinline__ void arc_candidates(const LibView& L, int kind, int t, float rd,
+                                               float ri, const float su[2], float lc, int el,
+                                               int trf, float so[2], float d[2]) {
+  // every delay of the pin's fanin arcs is reset first (timer.cpp:757-759)
+  so[0] = so[1] = d[0] = d[1] = __builtin_nanf("");
+  if (kind == OTG_ARC_NET_RC) {
+    // net arcs carry rf -> rf only (arc.cpp:91-98, 120-124)
+    const float x = su[trf];
+    if (!__builtin_isnan(x)) so[trf] = x < 0.0f ? -sqrtf(x * x + ri) : sqrtf(x * x + ri);
+    d[trf] = rd;
+  } else if (kind == OTG_ARC_NET_EMPTY) {
+    so[trf] = su[trf];
+    d[trf] = 0.0f;
+  } else if (kind == OTG_ARC_CELL) {
+    if (L.has[2 * t + el]) {
+      const int sense = L.sense[t], edge = L.edge[t];
+      const bool ok0 = !__builtin_isnan(su[0]) && transition_defined(sense, edge, 0, trf);
+      const bool ok1 = !__builtin_isnan(su[1]) && transition_defined(sense, edge, 1, trf);
+      const int ld = L.tlut[8 * t + 4 * el + trf], ls = L.tlut[8 * t + 4 * el + 2 + trf];
+      table_pair(L, ls, su[0], su[1], lc, ok0, ok1, so[0], so[1]);
+      table_pair(L, ld, su[0], su[1], lc, ok0, ok1, d[0], d[1]);
+    }
+  }
+}
+
+// Where a pin's fanin arcs and its sources' values are found: the caller's
+// arrays in pin / arc order (Direct), or a plan's level-ordered copies
+// (Planned).
+struct Direct {
+  const FpropParams& P;
+  __device__ int first(int v) const { return P.g.fanin_start[v]; }
+  __device__ int last(int v) const { return P.g.fanin_start[v + 1]; }
+  __device__ void arc(int j, int c, int& kind, int& t, int& src, float& rd, float& ri) const {
+    kind = P.g.arc_kind[j];
+    t = P.g.arc_timing[j];
+    src = P.g.fanin[j];
+    rd = P.g.arc_rc_delay[4LL * j + c];
+    ri = P.g.arc_rc_impulse[4LL * j + c];
+  }
+  __device__ void source(int u, int el, float su[2], float au[2]) const {
+    su[0] = P.out.slew[4 * u + el * 2];
+    su[1] = P.out.slew[4 * u + el * 2 + 1];
+    au[0] = P.out.at[4 * u + el * 2];
+    au[1] = P.out.at[4 * u + el * 2 + 1];
+  }
+  __device__ int64_t arc_id(int j) const { return j; }
+};
+
 // One (pin, corner) of a level: relax its slew and arrival over every fanin
 // arc and input transition in arc order (the oracle's order), writing the
 // arcs' delays.  Arcs are taken kChunk at a time: their own data, the
 // source pins' values and the library entries are loaded for the whole
 // chunk before the values are combined.
-__device__ __forceinline__ void fprop_pin(const FpropParams& P, const LibView& L, int v, int c) {
-  const otg_prop_graph& G = P.g;
+template <class Src>
+__device__ __forceinline__ void fprop_one(const Src& S, const LibView& L, float* arc_delay,
+                                          int v, int c, float& s, float& a, float lc) {
   const int el = c >> 1, trf = c & 1;
-  float s = G.pi_slew[4 * v + c], a = G.pi_at[4 * v + c];
-  const float lc = G.pin_load[4 * v + c];
-  const int j0 = G.fanin_start[v], j1 = G.fanin_start[v + 1];
+  const int j0 = S.first(v), j1 = S.last(v);
   for (int jb = j0; jb < j1; jb += kChunk) {
     int kind[kChunk], t[kChunk], u[kChunk];
     float rd[kChunk], ri[kChunk];
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
       const bool live = jb + q < j1;
-      const int j = live ? jb + q : j0;
-      kind[q] = live ? G.arc_kind[j] : OTG_ARC_SKIP;
-      u[q] = G.fanin[j];
-      t[q] = G.arc_timing[j];
-      rd[q] = G.arc_rc_delay[4LL * j + c];
-      ri[q] = G.arc_rc_impulse[4LL * j + c];
+      S.arc(live ? jb + q : j0, c, kind[q], t[q], u[q], rd[q], ri[q]);
+      if (!live) kind[q] = OTG_ARC_SKIP;
     }
     float su[kChunk][2], au[kChunk][2];
 #pragma unroll
-    for (int q = 0; q < kChunk; ++q)
-#pragma unroll
-      for (int f = 0; f < 2; ++f) {
-        su[q][f] = P.out.slew[4 * u[q] + el * 2 + f];
-        au[q][f] = P.out.at[4 * u[q] + el * 2 + f];
-      }
+    for (int q = 0; q < kChunk; ++q) S.source(u[q], el, su[q], au[q]);
     float so[kChunk][2], d[kChunk][2];
 #pragma unroll
-    for (int q = 0; q < kChunk; ++q) {
-      // every delay of the pin's fanin arcs is reset first (timer.cpp:757-759)
-#pragma unroll
-      for (int f = 0; f < 2; ++f) so[q][f] = d[q][f] = __builtin_nanf("");
-      if (kind[q] == OTG_ARC_NET_RC) {
-        // net arcs carry rf -> rf only (arc.cpp:91-98, 120-124)
-        const float x = su[q][trf];
-        if (!__builtin_isnan(x)) so[q][trf] = x < 0.0f ? -sqrtf(x * x + ri[q]) : sqrtf(x * x + ri[q]);
-        d[q][trf] = rd[q];
-      } else if (kind[q] == OTG_ARC_NET_EMPTY) {
-        so[q][trf] = su[q][trf];
-        d[q][trf] = 0.0f;
-      } else if (kind[q] == OTG_ARC_CELL) {
-        const int tt = t[q];
-        if (L.has[2 * tt + el]) {
-          const int sense = L.sense[tt], edge = L.edge[tt];
-          const int ld = L.tlut[8 * tt + 4 * el + trf], ls = L.tlut[8 * tt + 4 * el + 2 + trf];
-#pragma unroll
-          for (int f = 0; f < 2; ++f) {
-            if (!__builtin_isnan(su[q][f]) && transition_defined(sense, edge, f, trf)) {
-              so[q][f] = table_value(L, ls, su[q][f], lc);
-              d[q][f] = table_value(L, ld, su[q][f], lc);
-            }
-          }
-        }
-      }
-    }
+    for (int q = 0; q < kChunk; ++q)
+      arc_candidates(L, kind[q], t[q], rd[q], ri[q], su[q], lc, el, trf, so[q], d[q]);
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
       if (jb + q >= j1) break;
+      const int64_t j = S.arc_id(jb + q);
 #pragma unroll
       for (int f = 0; f < 2; ++f) {
         s = relax(s, so[q][f], el);
-        if (P.out.arc_delay) P.out.arc_delay[8LL * (jb + q) + 4 * el + 2 * f + trf] = d[q][f];
-        if (!__builtin_isnan(d[q][f]) && !__builtin_isnan(au[q][f])) a = relax(a, d[q][f] + au[q][f], el);
+        if (arc_delay) arc_delay[8 * j + 4 * el + 2 * f + trf] = d[q][f];
+        if (!__builtin_isnan(d[q][f]) && !__builtin_isnan(au[q][f]))
+          a = relax(a, d[q][f] + au[q][f], el);
       }
     }
   }
-  P.out.slew[4 * v + c] = s;
-  P.out.at[4 * v + c] = a;
 }
 
-// One level: thread per (pin, corner), grid-stride over the level's pins.
-// kLds: every workgroup first copies the library into LDS (a workgroup with
-// no pin of the level returns before that).
+// Copies the library into LDS (the caller checked that it fits) and returns
+// the view of it; every thread of the workgroup takes part.
+__device__ __forceinline__ LibView stage_library(const otg_prop_lib& lib, int pool_words,
+                                                 int32_t* lds) {
+  const int nl = lib.num_luts, nt = lib.num_timings;
+  const int o_tlut = 4 * nl, o_sense = o_tlut + 8 * nt, o_edge = o_sense + nt,
+            o_has = o_edge + nt, o_pool = o_has + 2 * nt;
+  for (int k = threadIdx.x; k < 4 * nl; k += blockDim.x) lds[k] = lib.lut_hdr[k];
+  for (int k = threadIdx.x; k < 8 * nt; k += blockDim.x) lds[o_tlut + k] = lib.timing_lut[k];
+  for (int k = threadIdx.x; k < nt; k += blockDim.x) {
+    lds[o_sense + k] = lib.timing_sense[k];
+    lds[o_edge + k] = lib.timing_edge[k];
+  }
+  for (int k = threadIdx.x; k < 2 * nt; k += blockDim.x) lds[o_has + k] = lib.timing_has[k];
+  const int32_t* pool = reinterpret_cast<const int32_t*>(lib.lut_val);
+  for (int k = threadIdx.x; k < pool_words; k += blockDim.x) lds[o_pool + k] = pool[k];
+  __syncthreads();
+  return LibView{lds, reinterpret_cast<const float*>(lds + o_pool), lds + o_tlut,
+                 lds + o_sense, lds + o_edge, lds + o_has};
+}
+
+__device__ __forceinline__ LibView device_library(const otg_prop_lib& lib) {
+  return LibView{lib.lut_hdr, lib.lut_val, lib.timing_lut, lib.timing_sense, lib.timing_edge,
+                 lib.timing_has};
+}
+
+// One level in pin order: thread per (pin, corner), grid-stride over the
+// level's pins.  kLds: every workgroup first copies the library into LDS (a
+// workgroup with no pin of the level returns before that).
 template <bool kLds>
 __global__ __launch_bounds__(1024) void fprop_level_kernel(FpropParams P, int lv) {
   const int b = P.ends[lv], e = P.ends[lv + 1];
   const int64_t total = 4LL * (e - b);
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  LibView L{P.lib.lut_hdr, P.lib.lut_val, P.lib.timing_lut, P.lib.timing_sense,
-            P.lib.timing_edge, P.lib.timing_has};
+  LibView L = device_library(P.lib);
   if constexpr (kLds) {
     __shared__ int32_t lds[kLdsWords];
     if ((int64_t)blockIdx.x * blockDim.x >= total) return;
-    const int nl = P.lib.num_luts, nt = P.lib.num_timings;
-    const int o_tlut = 4 * nl, o_sense = o_tlut + 8 * nt, o_edge = o_sense + nt,
-              o_has = o_edge + nt, o_pool = o_has + 2 * nt;
-    for (int k = threadIdx.x; k < 4 * nl; k += blockDim.x) lds[k] = P.lib.lut_hdr[k];
-    for (int k = threadIdx.x; k < 8 * nt; k += blockDim.x) lds[o_tlut + k] = P.lib.timing_lut[k];
-    for (int k = threadIdx.x; k < nt; k += blockDim.x) {
-      lds[o_sense + k] = P.lib.timing_sense[k];
-      lds[o_edge + k] = P.lib.timing_edge[k];
-    }
-    for (int k = threadIdx.x; k < 2 * nt; k += blockDim.x) lds[o_has + k] = P.lib.timing_has[k];
-    const int32_t* pool = reinterpret_cast<const int32_t*>(P.lib.lut_val);
-    for (int k = threadIdx.x; k < P.pool_words; k += blockDim.x) lds[o_pool + k] = pool[k];
-    __syncthreads();
-    L = LibView{lds, reinterpret_cast<const float*>(lds + o_pool), lds + o_tlut, lds + o_sense,
-                lds + o_edge, lds + o_has};
+    L = stage_library(P.lib, P.pool_words, lds);
   }
-  for (; idx < total; idx += (int64_t)gridDim.x * blockDim.x)
-    fprop_pin(P, L, P.frontiers[b + (int)(idx >> 2)], (int)(idx & 3));
+  const Direct S{P};
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int v = P.frontiers[b + (int)(idx >> 2)], c = (int)(idx & 3);
+    float s = P.g.pi_slew[4 * v + c], a = P.g.pi_at[4 * v + c];
+    fprop_one(S, L, P.out.arc_delay, v, c, s, a, P.g.pin_load[4 * v + c]);
+    P.out.slew[4 * v + c] = s;
+    P.out.at[4 * v + c] = a;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Level-ordered plan: pins renumbered to their position in the frontiers
+// (the pins of a level are consecutive), fanin arcs to slots in that order,
+// so that a level reads its pins' and arcs' data as contiguous ranges.
+// ---------------------------------------------------------------------------
+
+struct PlanView {
+  const int32_t* frontiers;  // [n_lv] position -> pin
+  const int32_t* ends;       // [levels+1]
+  const int32_t* aofs;       // [n_lv+1] first slot of a position
+  const int32_t* slot_src;   // [m_lv] source position, or -(pin+1) for a source outside the plan
+  const int32_t* slot_arc;   // [m_lv] the arc's index in the caller's CSR
+  int32_t* slot_kt;          // [2 m_lv] kind, timing
+  float* slot_rc;            // [8 m_lv] RC delay[4], impulse[4]
+  float* pos_in;             // [12 n_lv] load[4], PI slew[4], PI arrival[4]
+  float* pos_sa;             // [8 n_lv] slew[4], arrival[4]
+};
+
+struct PlanParams {
+  PlanView pv;
+  otg_prop_lib lib;
+  otg_prop_result out;
+  int32_t pool_words;
+};
+
+struct Planned {
+  const PlanParams& P;
+  __device__ int first(int q) const { return P.pv.aofs[q]; }
+  __device__ int last(int q) const { return P.pv.aofs[q + 1]; }
+  __device__ void arc(int k, int c, int& kind, int& t, int& src, float& rd, float& ri) const {
+    kind = P.pv.slot_kt[2LL * k];
+    t = P.pv.slot_kt[2LL * k + 1];
+    src = P.pv.slot_src[k];
+    rd = P.pv.slot_rc[8LL * k + c];
+    ri = P.pv.slot_rc[8LL * k + 4 + c];
+  }
+  __device__ void source(int src, int el, float su[2], float au[2]) const {
+    if (src >= 0) {
+      const float2* x = reinterpret_cast<const float2*>(P.pv.pos_sa + 8LL * src + el * 2);
+      const float2 sv = x[0], av = x[2];
+      su[0] = sv.x;
+      su[1] = sv.y;
+      au[0] = av.x;
+      au[1] = av.y;
+    } else {
+      const int u = -src - 1;
+      su[0] = P.out.slew[4 * u + el * 2];
+      su[1] = P.out.slew[4 * u + el * 2 + 1];
+      au[0] = P.out.at[4 * u + el * 2];
+      au[1] = P.out.at[4 * u + el * 2 + 1];
+    }
+  }
+  __device__ int64_t arc_id(int k) const { return P.pv.slot_arc[k]; }
+};
+
+template <bool kLds>
+__global__ __launch_bounds__(1024) void fprop_plan_level_kernel(PlanParams P, int lv) {
+  const int b = P.pv.ends[lv], e = P.pv.ends[lv + 1];
+  const int64_t total = 4LL * (e - b);
+  LibView L = device_library(P.lib);
+  if constexpr (kLds) {
+    __shared__ int32_t lds[kLdsWords];
+    const int64_t share0 = ((total + gridDim.x - 1) / gridDim.x + 63) & ~63LL;
+    if ((int64_t)blockIdx.x * share0 >= total) return;
+    L = stage_library(P.lib, P.pool_words, lds);
+  }
+  const Planned S{P};
+  // the level's lanes in equal wavefront-aligned shares, so that every
+  // workgroup (one per CU) has work
+  const int64_t share = ((total + gridDim.x - 1) / gridDim.x + 63) & ~63LL;
+  const int64_t lo = (int64_t)blockIdx.x * share, hi = min(total, lo + share);
+  for (int64_t idx = lo + threadIdx.x; idx < hi; idx += blockDim.x) {
+    const int q = b + (int)(idx >> 2), c = (int)(idx & 3);
+    const float* in = P.pv.pos_in + 12LL * q;
+    float s = in[4 + c], a = in[8 + c];
+    fprop_one(S, L, P.out.arc_delay, q, c, s, a, in[c]);
+    P.pv.pos_sa[8LL * q + c] = s;
+    P.pv.pos_sa[8LL * q + 4 + c] = a;
+  }
+}
+
+// plan build: pin -> position (-1 outside the plan) and the fanin count of
+// every position
+__global__ void plan_fill_kernel(int32_t* a, int64_t n, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+__global__ void plan_pos_kernel(const int32_t* frontiers, int32_t n_lv, const int32_t* fanin_start,
+                                int32_t* pin_pos, int32_t* deg) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int v = frontiers[q];
+    pin_pos[v] = (int)q;
+    deg[q] = fanin_start[v + 1] - fanin_start[v];
+  }
+}
+
+__global__ void plan_slot_kernel(const int32_t* frontiers, int32_t n_lv, const int32_t* fanin_start,
+                                 const int32_t* fanin, const int32_t* pin_pos,
+                                 const int32_t* aofs, int32_t* slot_src, int32_t* slot_arc) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int v = frontiers[q];
+    const int j0 = fanin_start[v], k0 = aofs[q], dn = aofs[q + 1] - k0;
+    for (int r = 0; r < dn; ++r) {
+      const int j = j0 + r, u = fanin[j], pu = pin_pos[u];
+      slot_src[k0 + r] = pu >= 0 ? pu : -u - 1;
+      slot_arc[k0 + r] = j;
+    }
+  }
+}
+
+// run: the caller's per-arc and per-pin inputs into plan order ...
+// 4 floats of a pin or arc: one 16-byte access when the caller's array is
+// 16-byte aligned (kVec, checked on the host), else four
+template <bool kVec>
+__device__ __forceinline__ float4 load4(const float* p) {
+  if constexpr (kVec) return *reinterpret_cast<const float4*>(p);
+  else return make_float4(p[0], p[1], p[2], p[3]);
+}
+template <bool kVec>
+__device__ __forceinline__ void store4(float* p, float4 x) {
+  if constexpr (kVec) {
+    *reinterpret_cast<float4*>(p) = x;
+  } else {
+    p[0] = x.x;
+    p[1] = x.y;
+    p[2] = x.z;
+    p[3] = x.w;
+  }
+}
+
+template <bool kVec>
+__global__ void plan_gather_arcs_kernel(PlanView pv, otg_prop_graph g, int32_t m_lv) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m_lv;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int j = pv.slot_arc[k];
+    reinterpret_cast<int2*>(pv.slot_kt)[k] = make_int2(g.arc_kind[j], g.arc_timing[j]);
+    float4* o = reinterpret_cast<float4*>(pv.slot_rc + 8 * k);
+    o[0] = load4<kVec>(g.arc_rc_delay + 4LL * j);
+    o[1] = load4<kVec>(g.arc_rc_impulse + 4LL * j);
+  }
+}
+
+template <bool kVec>
+__global__ void plan_gather_pins_kernel(PlanView pv, otg_prop_graph g, int32_t n_lv) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int v = pv.frontiers[q];
+    float4* o = reinterpret_cast<float4*>(pv.pos_in + 12 * q);
+    o[0] = load4<kVec>(g.pin_load + 4LL * v);
+    o[1] = load4<kVec>(g.pi_slew + 4LL * v);
+    o[2] = load4<kVec>(g.pi_at + 4LL * v);
+  }
+}
+
+// ... and the results back to pin order
+template <bool kVec>
+__global__ void plan_scatter_kernel(PlanView pv, otg_prop_result out, int32_t n_lv) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int v = pv.frontiers[q];
+    const float4* x = reinterpret_cast<const float4*>(pv.pos_sa + 8 * q);
+    store4<kVec>(out.slew + 4LL * v, x[0]);
+    store4<kVec>(out.at + 4LL * v, x[1]);
+  }
 }
 
 }  // namespace
+
+struct otg_fprop_plan {
+  otg_context* ctx = nullptr;
+  int32_t num_pins = 0, num_arcs = 0, n_lv = 0, m_lv = 0, num_levels = 0;
+  int32_t* frontiers = nullptr;
+  int32_t* ends = nullptr;
+  int32_t* aofs = nullptr;
+  int32_t* slot_src = nullptr;
+  int32_t* slot_arc = nullptr;
+  int32_t* slot_kt = nullptr;
+  float* slot_rc = nullptr;
+  float* pos_in = nullptr;
+  float* pos_sa = nullptr;
+};
+
+namespace {
+
+void plan_free(otg_fprop_plan* p) {
+  for (void* x : {(void*)p->frontiers, (void*)p->ends, (void*)p->aofs, (void*)p->slot_src,
+                  (void*)p->slot_arc, (void*)p->slot_kt, (void*)p->slot_rc, (void*)p->pos_in,
+                  (void*)p->pos_sa})
+    if (x) (void)hipFree(x);
+  delete p;
+}
+
+int grid_for(const otg_context* ctx, int64_t n, int block) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, ctx->num_cus * 16LL));
+}
+
+bool lib_ok(const otg_prop_lib* lib) {
+  return lib->num_luts >= 0 && lib->num_timings >= 0 && lib->num_lut_vals >= 0 &&
+         !(lib->num_timings > 0 && (!lib->timing_sense || !lib->timing_edge ||
+                                    !lib->timing_lut || !lib->timing_has)) &&
+         !(lib->num_luts > 0 && (!lib->lut_hdr || !lib->lut_val || lib->num_lut_vals < 1));
+}
+
+bool lib_in_lds(const otg_prop_lib* lib) {
+  return 4LL * lib->num_luts + 12LL * lib->num_timings + lib->num_lut_vals <= kLdsWords;
+}
+
+}  // namespace
+
+extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, int32_t num_arcs,
+                                            const int32_t* fanin_start, const int32_t* fanin,
+                                            const int32_t* frontiers,
+                                            const int32_t* frontier_ends, int32_t num_levels,
+                                            otg_stream stream, otg_fprop_plan** plan) {
+  if (!ctx || !plan || num_pins < 0 || num_arcs < 0 || num_levels < 0 ||
+      (num_pins > 0 && !fanin_start) || (num_arcs > 0 && !fanin) ||
+      (num_levels > 0 && (!frontiers || !frontier_ends)))
+    return OTG_ERR_INVALID_ARGUMENT;
+  *plan = nullptr;
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  auto* p = new otg_fprop_plan();
+  p->ctx = ctx;
+  p->num_pins = num_pins;
+  p->num_arcs = num_arcs;
+  p->num_levels = num_levels;
+  int32_t* pin_pos = nullptr;
+  int32_t* deg = nullptr;
+  void* tmp = nullptr;
+  auto fail = [&](hipError_t e) {
+    (void)hipStreamSynchronize(s);
+    if (pin_pos) (void)hipFree(pin_pos);
+    if (deg) (void)hipFree(deg);
+    if (tmp) (void)hipFree(tmp);
+    plan_free(p);
+    otg_set_error(ctx, "otg_fprop_plan_create: %s", hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? OTG_ERR_OUT_OF_MEMORY : OTG_ERR_DEVICE;
+  };
+  hipError_t e;
+  int32_t e0 = 0, n_lv = 0;
+  if (num_levels > 0) {
+    if ((e = hipMemcpyAsync(&e0, frontier_ends, sizeof(int32_t), hipMemcpyDeviceToHost, s)) ||
+        (e = hipMemcpyAsync(&n_lv, frontier_ends + num_levels, sizeof(int32_t),
+                            hipMemcpyDeviceToHost, s)) ||
+        (e = hipStreamSynchronize(s)))
+      return fail(e);
+    if (e0 != 0 || n_lv < 0 || n_lv > num_pins) {
+      plan_free(p);
+      otg_set_error(ctx, "otg_fprop_plan_create: level ends run from %d to %d over %d pins", e0,
+                    n_lv, num_pins);
+      return OTG_ERR_INVALID_ARGUMENT;
+    }
+  }
+  p->n_lv = n_lv;
+  const size_t I = sizeof(int32_t);
+  if ((e = hipMalloc(&p->frontiers, I * std::max(n_lv, 1))) ||
+      (e = hipMalloc(&p->ends, I * (num_levels + 1))) ||
+      (e = hipMalloc(&p->aofs, I * (n_lv + 1))) ||
+      (e = hipMalloc(&pin_pos, I * std::max(num_pins, 1))) ||
+      (e = hipMalloc(&deg, I * std::max(n_lv, 1))))
+    return fail(e);
+  if (n_lv > 0 &&
+      (e = hipMemcpyAsync(p->frontiers, frontiers, I * n_lv, hipMemcpyDeviceToDevice, s)))
+    return fail(e);
+  if (num_levels > 0 &&
+      (e = hipMemcpyAsync(p->ends, frontier_ends, I * (num_levels + 1), hipMemcpyDeviceToDevice, s)))
+    return fail(e);
+  hipLaunchKernelGGL(plan_fill_kernel, dim3(grid_for(ctx, num_pins, 256)), dim3(256), 0, s,
+                     pin_pos, (int64_t)num_pins, -1);
+  hipLaunchKernelGGL(plan_fill_kernel, dim3(1), dim3(64), 0, s, p->aofs, (int64_t)1, 0);
+  if (n_lv > 0) {
+    hipLaunchKernelGGL(plan_pos_kernel, dim3(grid_for(ctx, n_lv, 256)), dim3(256), 0, s,
+                       p->frontiers, n_lv, fanin_start, pin_pos, deg);
+    size_t tb = 0;
+    if ((e = hipcub::DeviceScan::InclusiveSum(nullptr, tb, deg, p->aofs + 1, n_lv, s)) ||
+        (e = hipMalloc(&tmp, std::max<size_t>(tb, 1))) ||
+        (e = hipcub::DeviceScan::InclusiveSum(tmp, tb, deg, p->aofs + 1, n_lv, s)))
+      return fail(e);
+  }
+  int32_t m_lv = 0;
+  if ((e = hipMemcpyAsync(&m_lv, p->aofs + n_lv, I, hipMemcpyDeviceToHost, s)) ||
+      (e = hipStreamSynchronize(s)) || (e = hipGetLastError()))
+    return fail(e);
+  p->m_lv = m_lv;
+  if ((e = hipMalloc(&p->slot_src, I * std::max(m_lv, 1))) ||
+      (e = hipMalloc(&p->slot_arc, I * std::max(m_lv, 1))) ||
+      (e = hipMalloc(&p->slot_kt, 2 * I * std::max(m_lv, 1))) ||
+      (e = hipMalloc(&p->slot_rc, 8 * sizeof(float) * std::max(m_lv, 1))) ||
+      (e = hipMalloc(&p->pos_in, 12 * sizeof(float) * std::max(n_lv, 1))) ||
+      (e = hipMalloc(&p->pos_sa, 8 * sizeof(float) * std::max(n_lv, 1))))
+    return fail(e);
+  if (n_lv > 0)
+    hipLaunchKernelGGL(plan_slot_kernel, dim3(grid_for(ctx, n_lv, 256)), dim3(256), 0, s,
+                       p->frontiers, n_lv, fanin_start, fanin, pin_pos, p->aofs, p->slot_src,
+                       p->slot_arc);
+  if ((e = hipStreamSynchronize(s)) || (e = hipGetLastError())) return fail(e);
+  (void)hipFree(pin_pos);
+  (void)hipFree(deg);
+  if (tmp) (void)hipFree(tmp);
+  *plan = p;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_fprop_plan_destroy(otg_fprop_plan* plan) {
+  if (!plan) return OTG_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(plan->ctx->device);
+  plan_free(plan);
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_fprop_plan_sizes(const otg_fprop_plan* plan, int32_t* positions,
+                                           int32_t* slots) {
+  if (!plan) return OTG_ERR_INVALID_ARGUMENT;
+  if (positions) *positions = plan->n_lv;
+  if (slots) *slots = plan->m_lv;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_fprop_run(otg_context* ctx, otg_fprop_plan* plan,
+                                    const otg_prop_graph* graph, const otg_prop_lib* lib,
+                                    const otg_prop_result* out, otg_stream stream) {
+  if (!ctx || !plan || !graph || !lib || !out || !lib_ok(lib) || plan->ctx != ctx)
+    return OTG_ERR_INVALID_ARGUMENT;
+  if (graph->num_pins != plan->num_pins || graph->num_arcs != plan->num_arcs) {
+    otg_set_error(ctx, "otg_fprop_run: graph of %d pins / %d arcs, plan of %d / %d",
+                  graph->num_pins, graph->num_arcs, plan->num_pins, plan->num_arcs);
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  if (plan->n_lv == 0) return OTG_OK;
+  if (!out->slew || !out->at || !graph->pin_load || !graph->pi_slew || !graph->pi_at ||
+      (plan->m_lv > 0 && (!graph->arc_kind || !graph->arc_timing || !graph->arc_rc_delay ||
+                          !graph->arc_rc_impulse)))
+    return OTG_ERR_INVALID_ARGUMENT;
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  PlanView pv{plan->frontiers, plan->ends,    plan->aofs,   plan->slot_src, plan->slot_arc,
+              plan->slot_kt,   plan->slot_rc, plan->pos_in, plan->pos_sa};
+  auto a16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  const bool vec_arcs = a16(graph->arc_rc_delay) && a16(graph->arc_rc_impulse);
+  const bool vec_pins = a16(graph->pin_load) && a16(graph->pi_slew) && a16(graph->pi_at);
+  const bool vec_out = a16(out->slew) && a16(out->at);
+  const dim3 ga(grid_for(ctx, plan->m_lv, 256)), gp(grid_for(ctx, plan->n_lv, 256));
+  if (plan->m_lv > 0) {
+    if (vec_arcs) hipLaunchKernelGGL(plan_gather_arcs_kernel<true>, ga, dim3(256), 0, s, pv, *graph, plan->m_lv);
+    else hipLaunchKernelGGL(plan_gather_arcs_kernel<false>, ga, dim3(256), 0, s, pv, *graph, plan->m_lv);
+  }
+  if (vec_pins) hipLaunchKernelGGL(plan_gather_pins_kernel<true>, gp, dim3(256), 0, s, pv, *graph, plan->n_lv);
+  else hipLaunchKernelGGL(plan_gather_pins_kernel<false>, gp, dim3(256), 0, s, pv, *graph, plan->n_lv);
+  PlanParams P{pv, *lib, *out, lib->num_lut_vals};
+  if (lib_in_lds(lib)) {
+    for (int lv = plan->num_levels - 1; lv >= 0; --lv)
+      hipLaunchKernelGGL(fprop_plan_level_kernel<true>, dim3(std::max(1, ctx->num_cus)),
+                         dim3(1024), 0, s, P, lv);
+  } else {
+    for (int lv = plan->num_levels - 1; lv >= 0; --lv)
+      hipLaunchKernelGGL(fprop_plan_level_kernel<false>, dim3(std::max(1, ctx->num_cus * 2)),
+                         dim3(1024), 0, s, P, lv);
+  }
+  if (vec_out) hipLaunchKernelGGL(plan_scatter_kernel<true>, gp, dim3(256), 0, s, pv, *out, plan->n_lv);
+  else hipLaunchKernelGGL(plan_scatter_kernel<false>, gp, dim3(256), 0, s, pv, *out, plan->n_lv);
+  OTG_HIP(ctx, hipGetLastError());
+  return OTG_OK;
+}
 
 extern "C" otg_status otg_fprop(otg_context* ctx, const otg_prop_graph* graph,
                                 const otg_prop_lib* lib, const int32_t* frontiers,
@@ -246,9 +706,8 @@ extern "C" otg_status otg_fprop(otg_context* ctx, const otg_prop_graph* graph,
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = otg_pick_stream(ctx, stream);
   FpropParams P{*graph, *lib, frontiers, frontier_ends, *out, (int32_t)lib->num_lut_vals};
-  const int64_t lds_words = 4LL * lib->num_luts + 12LL * lib->num_timings + lib->num_lut_vals;
   // a level's pins are not known on the host: a fixed grid strides over them
-  if (lds_words <= kLdsWords) {
+  if (lib_in_lds(lib)) {
     const int grid = std::max(1, ctx->num_cus);
     for (int lv = num_levels - 1; lv >= 0; --lv)
       hipLaunchKernelGGL(fprop_level_kernel<true>, dim3(grid), dim3(1024), 0, s, P, lv);

@@ -194,6 +194,12 @@ _sig("otg_incr_query", C.c_int, C.c_void_p, vp, i32p, i32p, C.POINTER(C.c_void_p
      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p))
 _sig("otg_fprop", C.c_int, ctx_p, C.POINTER(PropGraph), C.POINTER(PropLib), vp, vp, C.c_int32,
      C.POINTER(PropResult), vp)
+_sig("otg_fprop_plan_create", C.c_int, ctx_p, C.c_int32, C.c_int32, vp, vp, vp, vp, C.c_int32, vp,
+     C.POINTER(C.c_void_p))
+_sig("otg_fprop_plan_destroy", C.c_int, C.c_void_p)
+_sig("otg_fprop_plan_sizes", C.c_int, C.c_void_p, i32p, i32p)
+_sig("otg_fprop_run", C.c_int, ctx_p, C.c_void_p, C.POINTER(PropGraph), C.POINTER(PropLib),
+     C.POINTER(PropResult), vp)
 _sig("otg_graph_begin", C.c_int, ctx_p, vp)
 _sig("otg_graph_end", C.c_int, ctx_p, vp, C.POINTER(C.c_void_p))
 _sig("otg_graph_launch", C.c_int, ctx_p, C.c_void_p, vp)
@@ -225,7 +231,8 @@ EXPORTED = [
     "otg_rct_compute", "otg_rct_query",
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize",
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",
-    "otg_fprop", "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
+    "otg_fprop", "otg_fprop_plan_create", "otg_fprop_plan_destroy", "otg_fprop_plan_sizes",
+    "otg_fprop_run", "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
     "otg_gen_rc_sizes", "otg_gen_rc_fill", "otg_gen_dag", "otg_build_fanin_csr",
     "otg_spef_read", "otg_spef_read_buffer", "otg_spef_last_error", "otg_spef_forest_get",
     "otg_spef_net_name", "otg_spef_node_name", "otg_spef_find_net", "otg_spef_find_node",

@@ -6,6 +6,10 @@ level from the sources to the sinks, Timer::_fprop_slew / _fprop_delay /
 _fprop_at (timer.cpp:733-783) on each pin of the level.  The graph, the
 library and the results stay in device memory; the level order comes from
 the levelizer's device arrays (opentimer_amd.levelize.Levelizer).
+
+`run` propagates in pin order (otg_fprop); `build_plan` + `run_plan`
+renumber the levels' pins and arcs once into level order (otg_fprop_plan_*)
+and then propagate over contiguous data, with the same results.
 """
 from __future__ import annotations
 
@@ -58,6 +62,38 @@ class Propagator:
         self.ctx.check(N.lib.otg_fprop(self.ctx.ptr, C.byref(self.graph), C.byref(self.lib),
                                        frontiers.data_ptr(), ends.data_ptr(), int(num_levels),
                                        C.byref(out), stream))
+
+    def build_plan(self, frontiers, ends, num_levels: int, stream=None):
+        stream = torch.cuda.current_stream(self.device).cuda_stream if stream is None else stream
+        self.close_plan()
+        h = C.c_void_p()
+        self.ctx.check(N.lib.otg_fprop_plan_create(
+            self.ctx.ptr, self.n, self.m, self._t["fanin_start"].data_ptr(),
+            self._t["fanin"].data_ptr(), frontiers.data_ptr(), ends.data_ptr(), int(num_levels),
+            stream, C.byref(h)))
+        self._plan = h
+        pos, slots = C.c_int32(), C.c_int32()
+        self.ctx.check(N.lib.otg_fprop_plan_sizes(h, C.byref(pos), C.byref(slots)))
+        return pos.value, slots.value
+
+    def run_plan(self, stream=None, arc_delay: bool = True):
+        stream = torch.cuda.current_stream(self.device).cuda_stream if stream is None else stream
+        out = N.PropResult(self.slew.data_ptr(), self.at.data_ptr(),
+                           self.arc_delay.data_ptr() if arc_delay else None)
+        self.ctx.check(N.lib.otg_fprop_run(self.ctx.ptr, self._plan, C.byref(self.graph),
+                                           C.byref(self.lib), C.byref(out), stream))
+
+    def close_plan(self):
+        if getattr(self, "_plan", None):
+            torch.cuda.synchronize(self.device)
+            N.lib.otg_fprop_plan_destroy(self._plan)
+        self._plan = None
+
+    def __del__(self):
+        try:
+            self.close_plan()
+        except Exception:
+            pass
 
     def host(self):
         torch.cuda.synchronize(self.device)

@@ -58,6 +58,16 @@ def _run(ctx, dcfg, lib_seed=11, graph_seed=12, num_timings=64, **kw):
     _same(slew, oslew)
     _same(at, oat)
     _same(ad, oad)
+    # the level-ordered plan: the same values
+    pos, slots = P.build_plan(L.frontiers, L.ends, L.num_ends - 1)
+    assert (pos, slots) == (csr.n, len(csr.fanin))
+    for t in (P.slew, P.at, P.arc_delay):
+        t.fill_(float("nan"))
+    P.run_plan()
+    ps, pa, pd = P.host()
+    _same(ps, oslew)
+    _same(pa, oat)
+    _same(pd, oad)
     return slew, at, ad, P, L
 
 
@@ -100,6 +110,61 @@ def test_fprop_rerun_and_no_arc_delay(ctx):
     _same(s2, slew)
     _same(a2, at)
     assert (ad2 == 1234.0).all()
+
+
+def test_fprop_plan_with_sources_outside_it(ctx):
+    # a plan over all levels but the last (the sources): their values are
+    # read from the caller's arrays, which hold a previous run's results
+    cfg = W.DagConfig("dag-40k", 40_000, 100_000, 20, seed=23)
+    slew, at, ad, P, L = _run(ctx, cfg)
+    lv = L.num_ends - 1
+    fr, ends, _ = L.host()
+    pos, slots = P.build_plan(L.frontiers, L.ends, lv - 1)
+    assert pos == ends[lv - 1]
+    P.at[torch.from_numpy(fr[: ends[lv - 1]]).long().cuda()] = 0.0  # overwritten by the run
+    P.run_plan()
+    s2, a2, d2 = P.host()
+    _same(s2, slew)
+    _same(a2, at)
+    _same(d2, ad)
+    # rerunning a plan gives the same values; a graph of another size is refused
+    P.run_plan()
+    _same(P.host()[1], at)
+    bad = N.PropGraph(P.graph.num_pins + 1, P.graph.num_arcs, *([None] * 9))
+    out = N.PropResult(P.slew.data_ptr(), P.at.data_ptr(), None)
+    assert N.lib.otg_fprop_run(ctx.ptr, P._plan, C.byref(bad), C.byref(P.lib), C.byref(out),
+                               None) == 1
+    P.close_plan()
+
+
+def test_fprop_plan_unaligned_arrays(ctx):
+    # per-pin and per-arc arrays 4 bytes off a 16-byte boundary take the
+    # plan's scalar gather and scatter
+    cfg = W.config0_dag(seed=14)
+    slew, at, ad, P, L = _run(ctx, cfg)
+
+    def shifted(t):
+        buf = torch.empty(t.numel() + 1, dtype=t.dtype, device=t.device)
+        buf[1:] = t.reshape(-1)
+        return buf, buf.data_ptr() + 4
+    keep = []
+    g = N.PropGraph.from_buffer_copy(P.graph)
+    for name in ("arc_rc_delay", "arc_rc_impulse", "pin_load", "pi_slew", "pi_at"):
+        buf, ptr = shifted(P._t[name])
+        keep.append(buf)
+        setattr(g, name, ptr)
+    so, sp = shifted(torch.full_like(P.slew, float("nan")))
+    ao, ap = shifted(torch.full_like(P.at, float("nan")))
+    P.arc_delay.fill_(float("nan"))
+    out = N.PropResult(sp, ap, P.arc_delay.data_ptr())
+    assert sp % 16 == 4 and P._t["pin_load"].data_ptr() % 16 == 0
+    ctx.check(N.lib.otg_fprop_run(ctx.ptr, P._plan, C.byref(g), C.byref(P.lib), C.byref(out),
+                                  None))
+    torch.cuda.synchronize()
+    _same(so[1:].reshape(-1, 4)[: P.n].cpu().numpy(), slew)
+    _same(ao[1:].reshape(-1, 4)[: P.n].cpu().numpy(), at)
+    _same(P.arc_delay[: P.m].cpu().numpy(), ad)
+    P.close_plan()
 
 
 def test_fprop_empty_and_invalid(ctx):
