@@ -167,6 +167,39 @@ def test_fprop_plan_unaligned_arrays(ctx):
     P.close_plan()
 
 
+def test_c17_tau15_whole_chain_on_gpu(ctx):
+    # SPEF -> GPU BFS and compensated RC -> GPU levelization -> GPU
+    # propagation over a plan, against the reference's printed arrival
+    # times (3 decimals) and, on the same RC inputs, bit-exact vs the oracle
+    import tau15_case as T
+    from opentimer_amd.rct import RcSolver
+
+    def gpu_rc(forest):
+        s = RcSolver.from_edges(forest, ctx)
+        s.solve("compensated")
+        h = s.host()
+        return s.host_topology()["node2bfs"], h["load"], h["delay"], h["impulse"]
+    g, lib, pins, gold = T.build(gpu_rc)
+    n = len(pins)
+    graph = W.PropGraphData(*(g[k] for k in GKEYS))
+    library = W.PropLibrary(*(lib[k] for k in LKEYS))
+    csr = W.FaninCsr(n, g["fanin_start"], g["fanin"],
+                     np.bincount(g["fanin"], minlength=n).astype(np.int32), None)
+    csr.frontier = np.flatnonzero(csr.out_degree == 0).astype(np.int32)
+    L = Levelizer(csr, ctx)
+    L.run()
+    P = Propagator(graph, library, ctx)
+    P.build_plan(L.frontiers, L.ends, L.num_ends - 1)
+    P.run_plan()
+    slew, at, ad = P.host()
+    assert np.abs(at.astype(np.float64) - gold).max() <= 0.0005 + 1e-5, pins
+    fr, ends, _ = L.host()
+    os_, oa, od = O.fprop(g, lib, fr, ends)
+    _same(slew, os_)
+    _same(at, oa)
+    _same(ad, od)
+
+
 def test_fprop_empty_and_invalid(ctx):
     lib = W.prop_library(num_timings=2)
     g = W.PropGraphData(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32),
