@@ -1,17 +1,21 @@
-"""The TAU 2015 c17 design as a forward-propagation case (test helper).
+"""TAU 2015 designs as forward-propagation cases (test helper).
 
-Builds, from tests/golden/prop/c17.json and tests/golden/spef/c17.spef, the
-graph and library arrays of include/otg_prop.h the way the reference's
-Timer sees the design:
+Builds, from tests/golden/prop/<design>.json and tests/golden/spef/
+<design>.spef, the graph and library arrays of include/otg_prop.h the way
+the reference's Timer sees the design:
 
 * RC trees from the SPEF (otg_spef_read), with the input capacitance of
   every sink pin added to its node (RctNode::cap, net.cpp:24-27; a cell
   input's rise/fall capacitance, a primary output's load; the driver's pin
-  is the root and adds nothing), solved per corner;
+  is the root and adds nothing), solved per corner; a net without
+  parasitics is an empty RC tree (load = the sinks' capacitances, arc slew
+  = input slew, delay 0; net.cpp:546-553, 652-680);
 * net arcs driver -> sink with the RC delay and impulse at the sink's node
-  (Net::_delay / _slew, net.cpp:646-699), cell arcs A1/A2 -> ZN of
-  NAND2_X1 with the early and late library's tables, and each ZN's load =
-  the root load of the net it drives (Net::_load, net.cpp:626-642);
+  (Net::_delay / _slew, net.cpp:646-699); a cell arc for every timing of a
+  cell pin that propagates (Timer::_insert_gate_arcs, timer.cpp:238-268; the
+  constraint timings have no delay or transition table), with both
+  libraries' tables; the load of a cell output = the root load of the net it
+  drives (Net::_load, net.cpp:626-642), 0 when it drives none;
 * primary-input arrival and slew from the .timing file (tau15.cpp:99-118).
 """
 import json
@@ -22,122 +26,143 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = os.path.join(ROOT, "tests", "golden")
 
+DESIGNS = ("c17", "s27_spef", "s400")
 SENSE = {"positive_unate": 0, "negative_unate": 1, "non_unate": 2, None: 2}
 EDGE = {"rising_edge": 1, "falling_edge": 2}
 TABLES = ("cell_rise", "cell_fall", "rise_transition", "fall_transition")
 
 
-def load():
-    return json.load(open(os.path.join(GOLD, "prop", "c17.json")))
+def load(design="c17"):
+    return json.load(open(os.path.join(GOLD, "prop", f"{design}.json")))
 
 
-def spef_forest():
+def spef_forest(design="c17"):
     from opentimer_amd.spef import SpefForest
-    return SpefForest.read(os.path.join(GOLD, "spef", "c17.spef"))
+    return SpefForest.read(os.path.join(GOLD, "spef", f"{design}.spef"))
 
 
-def build(rc_solve):
+def build(rc_solve, design="c17"):
     """rc_solve(forest) -> (node2bfs, load[4V], delay[4V], impulse[4V]) in the
     BFS order of every net.  Returns (graph dict, lib dict, pin names,
-    golden arrival [n,4])."""
-    J = load()
-    cell = J["cell"]
+    golden arrival [n,4] with NaN for pins the reference did not report)."""
+    J = load(design)
+    cells_e, cells_l = J["cells"]["early"], J["cells"]["late"]
     pins = list(J["inputs"]) + list(J["outputs"])
+    direction = {p: "output" for p in J["inputs"]}  # ports drive / sink their nets
+    direction.update({p: "input" for p in J["outputs"]})
     for inst in J["instances"]:
-        assert inst["cell"] == cell
-        pins += [f"{inst['name']}:{p}" for p in ("A1", "A2", "ZN")]
+        for cp, info in cells_e[inst["cell"]]["pins"].items():
+            pins.append(f"{inst['name']}:{cp}")
+            direction[pins[-1]] = info["direction"]
     pid = {p: i for i, p in enumerate(pins)}
     n = len(pins)
+    cell_of = {inst["name"]: inst["cell"] for inst in J["instances"]}
 
-    # net of every pin (ports are named after their net)
     net_pins = {}
     for p in J["inputs"] + J["outputs"]:
         net_pins.setdefault(p, []).append(p)
     for inst in J["instances"]:
-        for p, net in inst["pins"].items():
-            net_pins.setdefault(net, []).append(f"{inst['name']}:{p}")
+        for cp, net in inst["pins"].items():
+            net_pins.setdefault(net, []).append(f"{inst['name']}:{cp}")
 
     def pin_cap(name):
         if name in J["inputs"]:
             return [0.0] * 4
         if name in J["outputs"]:
             return [J["load"].get(name, 0.0)] * 4
-        lp = name.split(":")[1]
-        e, l = J["pins"]["early"][lp], J["pins"]["late"][lp]
+        g, cp = name.split(":")
+        e, l = cells_e[cell_of[g]]["pins"][cp], cells_l[cell_of[g]]["pins"][cp]
         return [e["cap_rise"], e["cap_fall"], l["cap_rise"], l["cap_fall"]]
 
-    forest = spef_forest()
-    drivers = {}
+    def driver(net):
+        d = [p for p in net_pins[net] if direction[p] == "output"]
+        assert len(d) == 1, (net, d)
+        return d[0]
+
+    forest = spef_forest(design)
+    rc_nets = {}
     for k in range(forest.num_nets):
         net = forest.net_name(k)
         root = int(forest.roots[k])
+        drv = driver(net)
+        assert forest.find_node(k, drv) == root, (net, drv)
         for p in net_pins[net]:
             loc = forest.find_node(k, p)
             assert loc >= 0, (net, p)
-            if loc == root:
-                drivers[net] = (k, p)
-            else:
+            if p != drv:
                 forest.add_pin_cap(k, loc, np.asarray(pin_cap(p), np.float32))
+        rc_nets[net] = k
     n2b, ld, dl, im = rc_solve(forest)
     ns = forest.nodes_start
 
     def at_node(k, loc, arr):
-        return arr[4 * (ns[k] + n2b[ns[k] + loc]): 4 * (ns[k] + n2b[ns[k] + loc]) + 4]
+        i = 4 * (ns[k] + n2b[ns[k] + loc])
+        return arr[i: i + 4]
 
-    # library: timing t = index of the arc in the cell's list, per split
-    arcs_e, arcs_l = J["arcs"]["early"], J["arcs"]["late"]
-    assert [(a["from"], a["to"]) for a in arcs_e] == [(a["from"], a["to"]) for a in arcs_l]
-    hdr, pool = [], []
-    T = len(arcs_e)
-    tlut = np.full(8 * T, -1, np.int32)
-    sense = np.zeros(T, np.int32)
-    edge = np.zeros(T, np.int32)
-    for t in range(T):
-        sense[t] = SENSE[arcs_e[t]["timing_sense"]]
-        edge[t] = EDGE.get(arcs_e[t]["timing_type"], 0)
-        for el, a in enumerate((arcs_e[t], arcs_l[t])):
-            for k, tn in enumerate(TABLES):
-                tab = a["tables"].get(tn)
-                if tab is None:
-                    continue
-                var1 = {"input_net_transition": 0, "total_output_net_capacitance": 1}.get(
-                    tab["variable_1"], -1)
-                i1, i2 = tab["index_1"] or [0.0], tab["index_2"] or [0.0]
-                assert len(tab["values"]) == len(i1) * len(i2)
-                tlut[8 * t + 4 * el + k] = len(hdr) // 4
-                hdr += [len(i1), len(i2), var1, sum(len(x) for x in pool)]
-                pool.append(np.asarray(i1 + i2 + tab["values"], np.float32))
+    # library: one timing per (cell, propagating timing), both splits
+    hdr, pool, sense, edge, tluts = [], [], [], [], []
+    toff = {}
+    for cname in sorted(cells_e):
+        toff[cname] = len(sense)
+        for a_e, a_l in zip(cells_e[cname]["arcs"], cells_l[cname]["arcs"]):
+            sense.append(SENSE[a_e["timing_sense"]])
+            edge.append(EDGE.get(a_e["timing_type"], 0))
+            tl = [-1] * 8
+            for el, a in enumerate((a_e, a_l)):
+                for k, tn in enumerate(TABLES):
+                    tab = a["tables"].get(tn)
+                    if tab is None:
+                        continue
+                    var1 = {"input_net_transition": 0, "total_output_net_capacitance": 1}.get(
+                        tab["variable_1"], -1)
+                    i1, i2 = tab["index_1"] or [0.0], tab["index_2"] or [0.0]
+                    assert len(tab["values"]) == len(i1) * len(i2)
+                    tl[4 * el + k] = len(hdr) // 4
+                    hdr += [len(i1), len(i2), var1, sum(len(x) for x in pool)]
+                    pool.append(np.asarray(i1 + i2 + tab["values"], np.float32))
+            tluts += tl
+    T = len(sense)
     lib = {"lut_hdr": np.asarray(hdr, np.int32), "lut_val": np.concatenate(pool),
-           "timing_sense": sense, "timing_edge": edge, "timing_lut": tlut,
-           "timing_has": np.ones(2 * T, np.int32)}
+           "timing_sense": np.asarray(sense, np.int32), "timing_edge": np.asarray(edge, np.int32),
+           "timing_lut": np.asarray(tluts, np.int32), "timing_has": np.ones(2 * T, np.int32)}
 
-    # arcs, grouped by their sink pin into a fanin CSR
     arcs = []  # (from, to, kind, timing, rc delay[4], rc impulse[4])
-    for net, (k, drv) in drivers.items():
-        for p in net_pins[net]:
-            if p == drv:
-                continue
-            loc = forest.find_node(k, p)
-            arcs.append((pid[drv], pid[p], 0, 0, at_node(k, loc, dl), at_node(k, loc, im)))
     nan4 = np.full(4, np.nan, np.float32)
+    zero4 = np.zeros(4, np.float32)
+    load4 = np.zeros((n, 4), np.float32)
+    for net, members in net_pins.items():
+        drv = driver(net)
+        if net in rc_nets:
+            k = rc_nets[net]
+            for p in members:
+                if p != drv:
+                    loc = forest.find_node(k, p)
+                    arcs.append((pid[drv], pid[p], 0, 0, at_node(k, loc, dl), at_node(k, loc, im)))
+            load4[pid[drv]] = at_node(k, int(forest.roots[k]), ld)
+        else:
+            for p in members:
+                if p != drv:
+                    arcs.append((pid[drv], pid[p], 1, 0, zero4, zero4))
+            load4[pid[drv]] = np.sum([pin_cap(p) for p in members if p != drv], axis=0)
     for inst in J["instances"]:
-        for t, a in enumerate(arcs_e):
+        c = inst["cell"]
+        for t, a in enumerate(cells_e[c]["arcs"]):
             arcs.append((pid[f"{inst['name']}:{a['from']}"], pid[f"{inst['name']}:{a['to']}"], 2,
-                         t, nan4, nan4))
+                         toff[c] + t, nan4, nan4))
+    for p in J["inputs"]:
+        load4[pid[p]] = 0.0  # a primary input drives through no cell arc
     arcs.sort(key=lambda x: x[1])
     fs = np.zeros(n + 1, np.int32)
     for a in arcs:
         fs[a[1] + 1] += 1
     fs = np.cumsum(fs).astype(np.int32)
-    load4 = np.zeros((n, 4), np.float32)
-    for net, (k, drv) in drivers.items():
-        if drv not in J["inputs"]:
-            load4[pid[drv]] = at_node(k, int(forest.roots[k]), ld)
     pis = np.full((n, 4), np.nan, np.float32)
     pia = np.full((n, 4), np.nan, np.float32)
     for p in J["inputs"]:
-        pis[pid[p]] = J["slew"][p]
-        pia[pid[p]] = J["at"][p]
+        if p in J["slew"]:
+            pis[pid[p]] = J["slew"][p]
+        if p in J["at"]:
+            pia[pid[p]] = J["at"][p]
     graph = {
         "fanin_start": fs,
         "fanin": np.array([a[0] for a in arcs], np.int32),
@@ -147,7 +172,9 @@ def build(rc_solve):
         "arc_rc_impulse": np.array([a[5] for a in arcs], np.float32).reshape(-1, 4),
         "pin_load": load4, "pi_slew": pis, "pi_at": pia,
     }
-    gold = np.array([J["golden_at"][p] for p in pins], np.float64)
+    gold = np.full((n, 4), np.nan)
+    for p, v in J["golden_at"].items():
+        gold[pid[p]] = v
     forest.close()
     return graph, lib, pins, gold
 

@@ -167,19 +167,22 @@ def test_fprop_plan_unaligned_arrays(ctx):
     P.close_plan()
 
 
-def test_c17_tau15_whole_chain_on_gpu(ctx):
+@pytest.mark.parametrize("design", ("c17", "s27_spef", "s400"))
+def test_tau15_whole_chain_on_gpu(ctx, design):
     # SPEF -> GPU BFS and compensated RC -> GPU levelization -> GPU
     # propagation over a plan, against the reference's printed arrival
-    # times (3 decimals) and, on the same RC inputs, bit-exact vs the oracle
+    # times (3 decimals; tolerance of tests/test_prop_tau15.py) and, on the
+    # same RC inputs, bit-exact vs the oracle
     import tau15_case as T
     from opentimer_amd.rct import RcSolver
+    print(f"tau15 design {design}", flush=True)
 
     def gpu_rc(forest):
         s = RcSolver.from_edges(forest, ctx)
         s.solve("compensated")
         h = s.host()
         return s.host_topology()["node2bfs"], h["load"], h["delay"], h["impulse"]
-    g, lib, pins, gold = T.build(gpu_rc)
+    g, lib, pins, gold = T.build(gpu_rc, design)
     n = len(pins)
     graph = W.PropGraphData(*(g[k] for k in GKEYS))
     library = W.PropLibrary(*(lib[k] for k in LKEYS))
@@ -192,7 +195,9 @@ def test_c17_tau15_whole_chain_on_gpu(ctx):
     P.build_plan(L.frontiers, L.ends, L.num_ends - 1)
     P.run_plan()
     slew, at, ad = P.host()
-    assert np.abs(at.astype(np.float64) - gold).max() <= 0.0005 + 1e-5, pins
+    m = ~np.isnan(gold)
+    err = np.abs(at.astype(np.float64)[m] - gold[m])
+    assert (err <= 0.0005 + 1e-5 + 1e-6 * np.abs(gold[m])).all(), float(err.max())
     fr, ends, _ = L.host()
     os_, oa, od = O.fprop(g, lib, fr, ends)
     _same(slew, os_)
