@@ -582,6 +582,14 @@ def run_fprop(ctx, stream, sh, reps=10, cpu=True):
     plan_ms = (time.perf_counter() - t0) * 1e3
     ms = timed(lambda: P.run_plan(stream=sh))
     n, m = csr.n, len(csr.fanin)
+    # required times and slacks (otg_bprop) from primary-output values on the
+    # sinks, over the forward pass's results
+    rng = np.random.default_rng(5)
+    po = np.full((n, 4), np.nan, np.float32)
+    sinks = csr.out_degree == 0
+    po[sinks] = rng.uniform(3000.0, 6000.0, (int(sinks.sum()), 4)).astype(np.float32)
+    P.required(po, L.frontiers, L.ends, levels, stream=sh)
+    bprop_ms = timed(lambda: P.rerun_required(L.frontiers, L.ends, levels, stream=sh))
     # per pin: fanin_start, frontier entry, load, PI slew and arrival, slew
     # and arrival out (4 + 4 + 5 x 16 B); per arc: source, kind, timing,
     # RC delay and impulse, the source's slew and arrival, 8 arc delays out
@@ -593,7 +601,9 @@ def run_fprop(ctx, stream, sh, reps=10, cpu=True):
            "arcs_per_s": m / (ms / 1e3), "us_per_level": ms * 1e3 / levels,
            "algorithmic_bytes": alg, "achieved_GBps": alg / (ms / 1e3) / 1e9,
            "frac": alg / (ms / 1e3) / 1e9 / PEAK_HBM_GBS, "path": "level-ordered plan",
-           "plan_build_ms": plan_ms, "pin_order_ms": direct_ms}
+           "plan_build_ms": plan_ms, "pin_order_ms": direct_ms,
+           "bprop": {"ms": bprop_ms, "pins_per_s": n / (bprop_ms / 1e3),
+                     "what": "required times and slacks, primary-output values on the sinks"}}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -601,6 +611,8 @@ def run_fprop(ctx, stream, sh, reps=10, cpu=True):
         for t in (P.slew, P.at, P.arc_delay):
             t.fill_(float("nan"))
         P.run_plan(stream=sh)
+        rat, slack = P.required(po, L.frontiers, L.ends, levels, stream=sh)
+        rat, slack = rat.cpu().numpy(), slack.cpu().numpy()
         slew, at, ad = P.host()
         fr, ends, _ = L.host()
         gk = ("fanin_start", "fanin", "arc_kind", "arc_timing", "arc_rc_delay", "arc_rc_impulse",
@@ -615,8 +627,12 @@ def run_fprop(ctx, stream, sh, reps=10, cpu=True):
             return bool(np.array_equal(np.isnan(a), np.isnan(b))
                         and np.array_equal(a[~np.isnan(a)].view(np.int32),
                                            b[~np.isnan(b)].view(np.int32)))
+        t0 = time.perf_counter()
+        orat, oslack = O.bprop(g.fanin_start, g.fanin, od, po, oa, fr, ends)
+        out["bprop"]["cpu_ms"] = (time.perf_counter() - t0) * 1e3
         out["bitexact_vs_oracle"] = {"slew": same(slew, os_), "at": same(at, oa),
-                                     "arc_delay": same(ad, od)}
+                                     "arc_delay": same(ad, od), "rat": same(rat, orat),
+                                     "slack": same(slack, oslack)}
         out["cpu_baseline"] = {"value": n / (cms / 1e3), "unit": "pins/s", "cores": 1,
                                "kind": "port", "cpu": cpu_model(), "ms": cms,
                                "sample": "the whole graph, one propagation"}

@@ -144,6 +144,40 @@ otg_status otg_fprop_plan_sizes(const otg_fprop_plan* plan, int32_t* positions, 
 otg_status otg_fprop_run(otg_context* ctx, otg_fprop_plan* plan, const otg_prop_graph* graph,
                          const otg_prop_lib* lib, const otg_prop_result* out, otg_stream stream);
 
+/* Backward propagation of required times over the same levels, level 0
+ * (the sinks) first: the forward tasks' counterpart (timer.cpp:1094-1103 over
+ * Timer::_bprop_rat, timer.cpp:812-841, and Arc::_bprop_rat for propagation
+ * arcs, arc.cpp:150-175).  A pin's required time starts at its primary-
+ * output value (po_rat, NaN elsewhere) and is relaxed over its fanout arcs
+ * with rat(sink, el, trf) - delay(el, frf, trf), the delays being the
+ * forward pass's arc_delay; MIN keeps the larger value, MAX the smaller
+ * (Pin::_relax_rat, pin.cpp:362-380).  Required times from timing checks
+ * (setup / hold tests of sequential cells, timer.cpp:823-835) are not
+ * computed: po_rat is the only source.  slack (optional, needs at):
+ * MIN at - rat, MAX rat - at (Pin::slack, pin.cpp:251-256).  arc_delay and
+ * rat must be 8-byte aligned.  Asynchronous on `stream`. */
+typedef struct otg_bprop_graph {
+  int32_t num_pins;
+  int32_t num_arcs;
+  const int32_t* fanout_start; /* [n+1] arcs grouped by source pin          */
+  const int32_t* fanout_arc;   /* [m]   arc ids (fanin-CSR positions)       */
+  const int32_t* arc_to;       /* [m]   sink pin of every arc               */
+  const float* arc_delay;      /* [8m]  otg_prop_result.arc_delay           */
+  const float* po_rat;         /* [4n]  primary-output required time or NaN */
+  const float* at;             /* [4n]  arrival (for slack) or NULL         */
+} otg_bprop_graph;
+
+otg_status otg_bprop(otg_context* ctx, const otg_bprop_graph* graph, const int32_t* frontiers,
+                     const int32_t* frontier_ends, int32_t num_levels, float* rat, float* slack,
+                     otg_stream stream);
+
+/* Host helper: the fanout CSR and arc sinks of a fanin CSR (host arrays;
+ * arcs of one source in arc order), as the reference's host builds its
+ * per-pin fanout lists. */
+otg_status otg_build_fanout_csr(int32_t num_pins, const int32_t* fanin_start,
+                                const int32_t* fanin, int32_t* fanout_start,
+                                int32_t* fanout_arc, int32_t* arc_to);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/otg_gen.h"
+#include "../../include/otg_prop.h"
 
 namespace {
 
@@ -260,5 +261,27 @@ extern "C" otg_status otg_build_fanin_csr(int32_t n, int64_t m, const int32_t* a
     if ((!cand || cand[i]) && out_degree[i] == 0) frontier[f++] = i;
   *first_size = f;
   if (num_kept) *num_kept = kept;
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_build_fanout_csr(int32_t num_pins, const int32_t* fanin_start,
+                                           const int32_t* fanin, int32_t* fanout_start,
+                                           int32_t* fanout_arc, int32_t* arc_to) {
+  if (num_pins < 0 || !fanin_start || !fanout_start) return OTG_ERR_INVALID_ARGUMENT;
+  const int64_t m = fanin_start[num_pins];
+  if (m > 0 && (!fanin || !fanout_arc || !arc_to)) return OTG_ERR_INVALID_ARGUMENT;
+  for (int32_t v = 0; v <= num_pins; ++v) fanout_start[v] = 0;
+  for (int32_t v = 0; v < num_pins; ++v) {
+    if (fanin_start[v + 1] < fanin_start[v]) return OTG_ERR_INVALID_ARGUMENT;
+    for (int32_t j = fanin_start[v]; j < fanin_start[v + 1]; ++j) {
+      const int32_t u = fanin[j];
+      if (u < 0 || u >= num_pins) return OTG_ERR_INVALID_ARGUMENT;
+      arc_to[j] = v;
+      ++fanout_start[u + 1];
+    }
+  }
+  for (int32_t v = 0; v < num_pins; ++v) fanout_start[v + 1] += fanout_start[v];
+  std::vector<int32_t> next(fanout_start, fanout_start + num_pins);
+  for (int64_t j = 0; j < m; ++j) fanout_arc[next[fanin[j]]++] = (int32_t)j;
   return OTG_OK;
 }

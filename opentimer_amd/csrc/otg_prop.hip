@@ -489,6 +489,123 @@ __global__ void plan_scatter_kernel(PlanView pv, otg_prop_result out, int32_t n_
   }
 }
 
+// ---------------------------------------------------------------------------
+// Backward propagation of required times (Timer::_bprop_rat,
+// ot/timer/timer.cpp:812-841; Arc::_bprop_rat for propagation arcs,
+// arc.cpp:150-175), levels first to last: thread per (pin, corner c =
+// el*2 + frf), relaxing the primary-output value over the pin's fanout arcs
+// in fanout order with rat(sink, el, trf) - delay(el, frf, trf) from the
+// forward pass's arc delays (NaN where unset, so net arcs and loop breakers
+// need no case of their own); Pin::_relax_rat (pin.cpp:362-380) keeps the
+// larger value for MIN and the smaller for MAX.  Slack (pin.cpp:251-256):
+// MIN at - rat, MAX rat - at.
+// ---------------------------------------------------------------------------
+
+struct BpropParams {
+  otg_bprop_graph g;
+  const int32_t* frontiers;
+  const int32_t* ends;
+  float* rat;
+  float* slack;
+};
+
+__device__ __forceinline__ float relax_rat(float cur, float v, int el) {
+  if (__builtin_isnan(v)) return cur;
+  if (__builtin_isnan(cur)) return v;
+  return el == 0 ? (v > cur ? v : cur) : (v < cur ? v : cur);
+}
+
+// A candidate of a pin's relaxation with its place in the sequential order
+// (the primary-output value -1, then 2 * fanout position + trf): combining
+// candidates keeps the better value and, between equal values (+0 and -0
+// included), the earlier one, which is what the sequential relaxation keeps.
+__device__ __forceinline__ void rat_pick(float& v, int& i, float v2, int i2, int el) {
+  if (__builtin_isnan(v2)) return;
+  const bool take = __builtin_isnan(v) || (el == 0 ? v2 > v : v2 < v) || (v2 == v && i2 < i);
+  if (take) {
+    v = v2;
+    i = i2;
+  }
+}
+
+constexpr int kBpropSmall = 8;  // fanout arcs a pin's own thread walks; more go to a wavefront
+
+// Levels first to last, one launch per level: workgroups of 256 threads take
+// the level's (pin, corner) items in chunks of 256; a pin with more than
+// kBpropSmall fanout arcs is queued in LDS by its corner-0 thread and taken
+// by a whole wavefront (16 arc lanes x 4 corners) once the chunk is done.
+__global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv) {
+  __shared__ int32_t queue[64];
+  __shared__ int32_t qn;
+  const int b = P.ends[lv], e = P.ends[lv + 1];
+  const int64_t total = 4LL * (e - b);
+  const otg_bprop_graph& G = P.g;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    if (threadIdx.x == 0) qn = 0;
+    __syncthreads();
+    const int64_t idx = base + threadIdx.x;
+    if (idx < total) {
+      const int u = P.frontiers[b + (int)(idx >> 2)], c = (int)(idx & 3);
+      const int el = c >> 1, frf = c & 1;
+      const int k0 = G.fanout_start[u], k1 = G.fanout_start[u + 1];
+      if (k1 - k0 > kBpropSmall) {
+        if (c == 0) queue[atomicAdd(&qn, 1)] = u;
+      } else {
+        float r = G.po_rat[4 * u + c];
+        for (int k = k0; k < k1; ++k) {
+          const int j = G.fanout_arc[k], v = G.arc_to[j];
+          const float2 d =
+              *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j + 4 * el + 2 * frf);
+          const float2 rv = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
+          if (!__builtin_isnan(d.x) && !__builtin_isnan(rv.x)) r = relax_rat(r, rv.x - d.x, el);
+          if (!__builtin_isnan(d.y) && !__builtin_isnan(rv.y)) r = relax_rat(r, rv.y - d.y, el);
+        }
+        P.rat[4 * u + c] = r;
+        if (P.slack) {
+          const float a = G.at[4 * u + c];
+          P.slack[4 * u + c] = (__builtin_isnan(a) || __builtin_isnan(r))
+                                   ? __builtin_nanf("")
+                                   : (el == 0 ? a - r : r - a);
+        }
+      }
+    }
+    __syncthreads();
+    const int nq = qn;
+    for (int q = wave; q < nq; q += blockDim.x >> 6) {
+      const int u = queue[q], c = lane & 3, a0 = lane >> 2;
+      const int el = c >> 1, frf = c & 1;
+      const int k0 = G.fanout_start[u], k1 = G.fanout_start[u + 1];
+      float r = a0 == 0 ? G.po_rat[4 * u + c] : __builtin_nanf("");
+      int ri = -1;
+      for (int k = k0 + a0; k < k1; k += 16) {
+        const int j = G.fanout_arc[k], v = G.arc_to[j];
+        const float2 d = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j + 4 * el + 2 * frf);
+        const float2 rv = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
+        if (!__builtin_isnan(d.x) && !__builtin_isnan(rv.x)) rat_pick(r, ri, rv.x - d.x, 2 * (k - k0), el);
+        if (!__builtin_isnan(d.y) && !__builtin_isnan(rv.y)) rat_pick(r, ri, rv.y - d.y, 2 * (k - k0) + 1, el);
+      }
+#pragma unroll
+      for (int m = 4; m < 64; m <<= 1) {
+        const float r2 = __shfl_xor(r, m);
+        const int i2 = __shfl_xor(ri, m);
+        rat_pick(r, ri, r2, i2, el);
+      }
+      if (a0 == 0) {
+        P.rat[4 * u + c] = r;
+        if (P.slack) {
+          const float a = G.at[4 * u + c];
+          P.slack[4 * u + c] = (__builtin_isnan(a) || __builtin_isnan(r))
+                                   ? __builtin_nanf("")
+                                   : (el == 0 ? a - r : r - a);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 struct otg_fprop_plan {
@@ -716,6 +833,27 @@ extern "C" otg_status otg_fprop(otg_context* ctx, const otg_prop_graph* graph,
     for (int lv = num_levels - 1; lv >= 0; --lv)
       hipLaunchKernelGGL(fprop_level_kernel<false>, dim3(grid), dim3(1024), 0, s, P, lv);
   }
+  OTG_HIP(ctx, hipGetLastError());
+  return OTG_OK;
+}
+
+extern "C" otg_status otg_bprop(otg_context* ctx, const otg_bprop_graph* graph,
+                                const int32_t* frontiers, const int32_t* frontier_ends,
+                                int32_t num_levels, float* rat, float* slack, otg_stream stream) {
+  if (!ctx || !graph || num_levels < 0 || graph->num_pins < 0 || graph->num_arcs < 0)
+    return OTG_ERR_INVALID_ARGUMENT;
+  if (num_levels == 0 || graph->num_pins == 0) return OTG_OK;
+  if (!frontiers || !frontier_ends || !rat || !graph->fanout_start || !graph->po_rat ||
+      (slack && !graph->at) ||
+      (graph->num_arcs > 0 && (!graph->fanout_arc || !graph->arc_to || !graph->arc_delay)) ||
+      ((uintptr_t)graph->arc_delay & 7) || ((uintptr_t)rat & 7))
+    return OTG_ERR_INVALID_ARGUMENT;
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  BpropParams P{*graph, frontiers, frontier_ends, rat, slack};
+  const int grid = std::max(1, ctx->num_cus * 8);
+  for (int lv = 0; lv < num_levels; ++lv)
+    hipLaunchKernelGGL(bprop_level_kernel, dim3(grid), dim3(256), 0, s, P, lv);
   OTG_HIP(ctx, hipGetLastError());
   return OTG_OK;
 }

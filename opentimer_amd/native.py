@@ -114,6 +114,12 @@ class PropResult(C.Structure):
     _fields_ = [("slew", vp), ("at", vp), ("arc_delay", vp)]
 
 
+class BpropGraph(C.Structure):
+    _fields_ = [("num_pins", C.c_int32), ("num_arcs", C.c_int32), ("fanout_start", vp),
+                ("fanout_arc", vp), ("arc_to", vp), ("arc_delay", vp), ("po_rat", vp),
+                ("at", vp)]
+
+
 class GenSizeClass(C.Structure):
     _fields_ = [("weight", C.c_double), ("min_nodes", C.c_int32), ("max_nodes", C.c_int32),
                 ("shape", C.c_int32), ("pad", C.c_int32), ("branch_prob", C.c_double)]
@@ -200,6 +206,8 @@ _sig("otg_fprop_plan_destroy", C.c_int, C.c_void_p)
 _sig("otg_fprop_plan_sizes", C.c_int, C.c_void_p, i32p, i32p)
 _sig("otg_fprop_run", C.c_int, ctx_p, C.c_void_p, C.POINTER(PropGraph), C.POINTER(PropLib),
      C.POINTER(PropResult), vp)
+_sig("otg_bprop", C.c_int, ctx_p, C.POINTER(BpropGraph), vp, vp, C.c_int32, vp, vp, vp)
+_sig("otg_build_fanout_csr", C.c_int, C.c_int32, vp, vp, vp, vp, vp)
 _sig("otg_graph_begin", C.c_int, ctx_p, vp)
 _sig("otg_graph_end", C.c_int, ctx_p, vp, C.POINTER(C.c_void_p))
 _sig("otg_graph_launch", C.c_int, ctx_p, C.c_void_p, vp)
@@ -232,7 +240,7 @@ EXPORTED = [
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize",
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",
     "otg_fprop", "otg_fprop_plan_create", "otg_fprop_plan_destroy", "otg_fprop_plan_sizes",
-    "otg_fprop_run", "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
+    "otg_fprop_run", "otg_bprop", "otg_build_fanout_csr", "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
     "otg_gen_rc_sizes", "otg_gen_rc_fill", "otg_gen_dag", "otg_build_fanin_csr",
     "otg_spef_read", "otg_spef_read_buffer", "otg_spef_last_error", "otg_spef_forest_get",
     "otg_spef_net_name", "otg_spef_node_name", "otg_spef_find_net", "otg_spef_find_node",

@@ -99,3 +99,37 @@ class Propagator:
         torch.cuda.synchronize(self.device)
         return (self.slew[: self.n].cpu().numpy(), self.at[: self.n].cpu().numpy(),
                 self.arc_delay[: self.m].cpu().numpy())
+
+    def required(self, po_rat, frontiers, ends, num_levels: int, stream=None):
+        """Required times and slacks over the same levels (otg_bprop), from
+        the primary-output required times po_rat [n,4] (NaN elsewhere) and
+        this propagator's arrival times and arc delays.  Returns device
+        tensors (rat, slack), [n,4] each."""
+        stream = torch.cuda.current_stream(self.device).cuda_stream if stream is None else stream
+        if getattr(self, "_fanout", None) is None:
+            fs = self._t["fanin_start"].cpu().numpy()
+            fi = self._t["fanin"].cpu().numpy()
+            fo_start = np.zeros(self.n + 1, np.int32)
+            fo_arc = np.zeros(max(self.m, 1), np.int32)
+            arc_to = np.zeros(max(self.m, 1), np.int32)
+            N.check(N.lib.otg_build_fanout_csr(self.n, N.addr(fs), N.addr(fi), N.addr(fo_start),
+                                               N.addr(fo_arc), N.addr(arc_to)))
+            self._fanout = tuple(torch.from_numpy(a).to(self.device)
+                                 for a in (fo_start, fo_arc, arc_to))
+        pr = torch.from_numpy(np.ascontiguousarray(po_rat, np.float32).reshape(-1, 4)).to(self.device)
+        rat = torch.full((max(self.n, 1), 4), float("nan"), dtype=torch.float32, device=self.device)
+        slack = torch.full_like(rat, float("nan"))
+        g = N.BpropGraph(self.n, self.m, *(t.data_ptr() for t in self._fanout),
+                         self.arc_delay.data_ptr(), pr.data_ptr(), self.at.data_ptr())
+        self._bp = (g, pr, rat, slack)
+        self.rerun_required(frontiers, ends, num_levels, stream)
+        return rat[: self.n], slack[: self.n]
+
+    def rerun_required(self, frontiers, ends, num_levels: int, stream=None):
+        """otg_bprop again over the buffers of the last `required` call (for
+        timing and after a new forward pass)."""
+        stream = torch.cuda.current_stream(self.device).cuda_stream if stream is None else stream
+        g, _, rat, slack = self._bp
+        self.ctx.check(N.lib.otg_bprop(self.ctx.ptr, C.byref(g), frontiers.data_ptr(),
+                                       ends.data_ptr(), int(num_levels), rat.data_ptr(),
+                                       slack.data_ptr(), stream))

@@ -43,6 +43,7 @@ for name, args, res in [
     ("ora_counting_argsort", [C.c_int, vp, C.c_int, C.c_int, vp, vp, vp], None),
     ("ora_lut_eval", [vp, vp, C.c_int, C.c_float, C.c_float], C.c_float),
     ("ora_fprop", [C.c_int] + [vp] * 17 + [C.c_int, vp, vp, vp], None),
+    ("ora_bprop", [vp] * 8 + [C.c_int, vp, vp], None),
 ]:
     f = getattr(_lib, name)
     f.argtypes = args
@@ -215,3 +216,33 @@ def fprop(graph, lib, frontiers, ends):
                    _p(pia), _p(sense), _p(edge), _p(tlut), _p(thas), _p(hdr), _p(pool), _p(fr),
                    _p(en), len(en) - 1, _p(slew), _p(at), _p(ad))
     return slew[: 4 * n].reshape(n, 4), at[: 4 * n].reshape(n, 4), ad[: 8 * m].reshape(m, 8)
+
+
+def fanout_csr(fanin_start, fanin):
+    """Arcs grouped by their source pin (stable in arc order): (fanout_start
+    [n+1], fanout_arc [m], arc_to [m])."""
+    fs = _i32(fanin_start)
+    n, m = len(fs) - 1, int(fs[-1])
+    arc_to = np.repeat(np.arange(n, dtype=np.int32), np.diff(fs)).astype(np.int32)
+    src = _i32(fanin)[:m]
+    order = np.argsort(src, kind="stable").astype(np.int32)
+    start = np.zeros(n + 1, np.int32)
+    np.cumsum(np.bincount(src, minlength=n), out=start[1:])
+    return start, order, arc_to
+
+
+def bprop(fanin_start, fanin, arc_delay, po_rat, at, frontiers, ends):
+    """Required times and slacks, levels first to last (timer.cpp:812-841,
+    arc.cpp:150-175, pin.cpp:251-256, 362-380).  Returns (rat [n,4], slack [n,4])."""
+    fo_start, fo_arc, arc_to = fanout_csr(fanin_start, fanin)
+    n = len(fo_start) - 1
+    one_i = lambda a: _i32(a) if len(a) else np.zeros(1, np.int32)
+    ad = _f32(arc_delay).reshape(-1) if np.size(arc_delay) else np.zeros(1, np.float32)
+    pr = _f32(po_rat).reshape(-1) if n else np.zeros(1, np.float32)
+    a = _f32(at).reshape(-1) if n else np.zeros(1, np.float32)
+    rat = np.full(max(4 * n, 1), np.nan, np.float32)
+    slack = np.full(max(4 * n, 1), np.nan, np.float32)
+    en = _i32(ends)
+    _lib.ora_bprop(_p(fo_start), _p(one_i(fo_arc)), _p(one_i(arc_to)), _p(ad), _p(pr), _p(a),
+                   _p(one_i(frontiers)), _p(en), len(en) - 1, _p(rat), _p(slack))
+    return rat[: 4 * n].reshape(n, 4), slack[: 4 * n].reshape(n, 4)

@@ -624,3 +624,47 @@ void ora_fprop(int n, const int32_t* fs, const int32_t* fi, const int32_t* kind,
     }
   }
 }
+
+/* Timer::_bprop_rat (ot/timer/timer.cpp:812-841) with Arc::_bprop_rat    */
+/* (ot/timer/arc.cpp:150-175) for propagation arcs, levels first to last  */
+/* (timer.cpp:1094-1103): a pin's required time starts at its primary-    */
+/* output value and is relaxed over its fanout arcs (in arc order) with   */
+/* rat(sink, el, trf) - delay(el, frf, trf); Pin::_relax_rat (pin.cpp:    */
+/* 362-380) keeps the larger value for MIN, the smaller for MAX.  The     */
+/* arc delays are those of the forward pass (NaN where unset), so net     */
+/* arcs (rf -> rf only) and loop breakers (all NaN) need no case of their */
+/* own.  Slack (pin.cpp:251-256): MIN at - rat, MAX rat - at.             */
+static float relax_rat(float cur, float v, int el) {
+  if (isnan(v)) return cur;
+  if (isnan(cur)) return v;
+  return el == 0 ? (v > cur ? v : cur) : (v < cur ? v : cur);
+}
+
+void ora_bprop(const int32_t* fo_start, const int32_t* fo_arc, const int32_t* arc_to,
+               const float* arc_delay, const float* po_rat, const float* at,
+               const int32_t* frontiers, const int32_t* ends, int num_levels, float* rat,
+               float* slack) {
+  for (int lv = 0; lv < num_levels; ++lv) {
+    for (int q = ends[lv]; q < ends[lv + 1]; ++q) {
+      const int u = frontiers[q];
+      for (int el = 0; el < 2; ++el)
+        for (int frf = 0; frf < 2; ++frf) {
+          const int c = el * 2 + frf;
+          float r = po_rat[4 * u + c];
+          for (int k = fo_start[u]; k < fo_start[u + 1]; ++k) {
+            const int j = fo_arc[k], v = arc_to[j];
+            for (int trf = 0; trf < 2; ++trf) {
+              const float d = arc_delay[8 * j + 4 * el + 2 * frf + trf];
+              const float rv = rat[4 * v + el * 2 + trf];
+              if (!isnan(d) && !isnan(rv)) r = relax_rat(r, rv - d, el);
+            }
+          }
+          rat[4 * u + c] = r;
+          if (slack) {
+            const float a = at[4 * u + c];
+            slack[4 * u + c] = (isnan(a) || isnan(r)) ? NAN : (el == 0 ? a - r : r - a);
+          }
+        }
+    }
+  }
+}

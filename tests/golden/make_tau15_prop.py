@@ -4,9 +4,9 @@ the assertions (<design>.timing), every cell the netlist uses from both
 libraries (<design>_Early.lib, <design>_Late.lib: pin directions and
 capacitances, the timings that propagate -- constraint timings (setup, hold,
 recovery, removal) carry no delay or transition table and are left out --
-with their tables), and the arrival times the reference reports before the
-first design change (the leading report_* lines of <design>.ops with the
-matching <design>.output lines).
+with their tables), and the arrival times, required times and slacks the
+reference reports before the first design change (the leading report_*
+lines of <design>.ops with the matching <design>.output lines).
 
 The parasitics are the SPEF fixtures tests/golden/spef/<design>.spef (the
 same files, copied unchanged).  Run from the repo root:
@@ -147,7 +147,7 @@ def netlist(text):
 
 
 def timing_file(text):
-    at, slew, load = {}, {}, {}
+    at, slew, load, rat = {}, {}, {}, {}
     for line in text.splitlines():
         f = line.split()
         if not f:
@@ -158,10 +158,15 @@ def timing_file(text):
             slew[f[1]] = [float(x) for x in f[2:6]]
         elif f[0] == "load":
             load[f[1]] = float(f[2])
-    return at, slew, load
+        elif f[0] == "rat":
+            rat[f[1]] = [float(x) for x in f[2:6]]
+    return at, slew, load, rat
 
 
-def golden_at(ops, output):
+def golden(ops, output, kind):
+    """{pin: [early rise, early fall, late rise, late fall]} of the `kind`
+    reports (report_at / report_rat / report_slack) that precede the first
+    design change."""
     ops = ops.splitlines()
     out = output.splitlines()
     gold = {}
@@ -171,7 +176,7 @@ def golden_at(ops, output):
             continue
         if not f[0].startswith("report_"):
             break
-        if f[0] != "report_at":
+        if f[0] != kind:
             continue
         pin = f[f.index("-pin") + 1]
         el = 1 if "-late" in f else 0
@@ -191,20 +196,22 @@ def make(design):
         ke = [(a["from"], a["to"], a["timing_type"]) for a in early[c]["arcs"]]
         kl = [(a["from"], a["to"], a["timing_type"]) for a in late[c]["arcs"]]
         assert ke == kl, (c, ke, kl)
-    at, slew, load = timing_file(rd(f"{design}.timing"))
-    gold = golden_at(rd(f"{design}.ops"), rd(f"{design}.output"))
+    at, slew, load, rat = timing_file(rd(f"{design}.timing"))
+    gold = golden(rd(f"{design}.ops"), rd(f"{design}.output"), "report_at")
+    gold_rat = golden(rd(f"{design}.ops"), rd(f"{design}.output"), "report_rat")
+    gold_slack = golden(rd(f"{design}.ops"), rd(f"{design}.output"), "report_slack")
     assert gold and all(None not in v for v in gold.values())
     doc = {
         "source": f"TAU 2015 benchmark {design} of the reference (benchmark/{design}): "
                   f"{design}.v, {design}.timing, the cells it uses from {design}_Early.lib / "
-                  f"{design}_Late.lib, and the report_at values of the leading reports of "
+                  f"{design}_Late.lib, and the report_at / report_rat / report_slack values of the leading reports of "
                   f"{design}.ops with the matching {design}.output lines; parasitics in "
                   f"../spef/{design}.spef",
         "units": {"time": "ps", "capacitance": "fF", "resistance": "kohm"},
         "inputs": inputs, "outputs": outputs, "instances": insts,
-        "at": at, "slew": slew, "load": load,
+        "at": at, "slew": slew, "load": load, "rat": rat,
         "cells": {"early": early, "late": late},
-        "golden_at": gold,
+        "golden_at": gold, "golden_rat": gold_rat, "golden_slack": gold_slack,
     }
     os.makedirs(os.path.join(HERE, "prop"), exist_ok=True)
     path = os.path.join(HERE, "prop", f"{design}.json")

@@ -196,3 +196,22 @@ def levels(graph):
     fr0 = np.flatnonzero(out == 0).astype(np.int32)
     fr, ends, _ = O.levelize(n, fs, fi, out, fr0)
     return fr, ends
+
+
+def required(design, pins):
+    """(primary-output required times [n,4] from the .timing file, NaN
+    elsewhere; the reference's reported rat and slack [n,4], NaN for pins it
+    did not report)."""
+    J = load(design)
+    pid = {p: i for i, p in enumerate(pins)}
+    n = len(pins)
+    po_rat = np.full((n, 4), np.nan, np.float32)
+    for p, v in J["rat"].items():
+        po_rat[pid[p]] = v
+    gr = np.full((n, 4), np.nan)
+    gs = np.full((n, 4), np.nan)
+    for p, v in J["golden_rat"].items():
+        gr[pid[p]] = v
+    for p, v in J["golden_slack"].items():
+        gs[pid[p]] = v
+    return po_rat, gr, gs

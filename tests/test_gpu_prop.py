@@ -203,6 +203,71 @@ def test_tau15_whole_chain_on_gpu(ctx, design):
     _same(slew, os_)
     _same(at, oa)
     _same(ad, od)
+    # required times and slacks (c17: every required time comes from a
+    # primary output; the sequential designs' come from timing checks too)
+    po_rat, gr, gs = T.required(design, pins)
+    rat, slack = P.required(po_rat, L.frontiers, L.ends, L.num_ends - 1)
+    orat, oslack = O.bprop(g["fanin_start"], g["fanin"], ad, po_rat, at, fr, ends)
+    _same(rat.cpu().numpy(), orat)
+    _same(slack.cpu().numpy(), oslack)
+    if design == "c17":
+        for got, want in ((orat, gr), (oslack, gs)):
+            err = np.abs(got.astype(np.float64) - want)
+            assert (err <= 0.0005 + 1e-5 + 1e-6 * np.abs(want)).all()
+
+
+@pytest.mark.parametrize("cfg", [W.config0_dag(seed=15),
+                                 W.DagConfig("dag-200k", 200_000, 500_000, 40, seed=24)],
+                         ids=lambda c: c.name)
+def test_bprop_bitexact(ctx, cfg):
+    # required times and slacks (otg_bprop) from primary-output values on the
+    # sinks and 5% of the other pins, over the forward pass's results
+    slew, at, ad, P, L = _run(ctx, cfg)
+    fr, ends, _ = L.host()
+    g = {k: P._t[k].cpu().numpy() for k in ("fanin_start", "fanin")}
+    rng = np.random.default_rng(3)
+    n = len(g["fanin_start"]) - 1
+    out_deg = np.bincount(g["fanin"], minlength=n)
+    po = np.full((n, 4), np.nan, np.float32)
+    pick = (out_deg == 0) | (rng.random(n) < 0.05)
+    po[pick] = rng.uniform(500, 3000, (int(pick.sum()), 4))
+    rat, slack = P.required(po, L.frontiers, L.ends, L.num_ends - 1)
+    orat, oslack = O.bprop(g["fanin_start"], g["fanin"], ad, po, at, fr, ends)
+    _same(rat.cpu().numpy(), orat)
+    _same(slack.cpu().numpy(), oslack)
+    assert np.isfinite(orat).mean() > 0.5
+
+
+@pytest.mark.parametrize("fanout", [5, 40])
+def test_bprop_signed_zero_ties_follow_fanout_order(ctx, fanout):
+    # a hub pin drives `fanout` sinks through net arcs without parasitics
+    # (delay 0) whose required times are +0 and -0 mixed: the sequential
+    # relaxation keeps the first of the equal candidates in fanout order, and
+    # so must the thread path (5 arcs) and the wavefront path (40 arcs)
+    from test_prop_oracle import _graph, _levels, _scalar_lib
+    n = fanout + 1
+    arcs = [(0, k + 1) for k in range(fanout)]
+    pis = np.full((n, 4), np.nan, np.float32)
+    pis[0] = 5.0
+    pia = np.full((n, 4), np.nan, np.float32)
+    pia[0] = 0.0
+    g, _ = _graph(n, arcs, [1] * fanout, pis=pis, pia=pia)
+    lib, _ = _scalar_lib(1, np.random.default_rng(0))
+    fr, ends = _levels(g)
+    P = Propagator(W.PropGraphData(**g), W.PropLibrary(**lib), ctx)
+    frt = torch.from_numpy(fr).cuda()
+    ent = torch.from_numpy(ends).cuda()
+    P.run(frt, ent, len(ends) - 1)
+    _, at, ad = P.host()
+    rng = np.random.default_rng(fanout)
+    for trial in range(4):
+        po = np.full((n, 4), np.nan, np.float32)
+        po[1:] = np.where(rng.random((fanout, 4)) < 0.5, np.float32(-0.0), np.float32(0.0))
+        rat, slack = P.required(po, frt, ent, len(ends) - 1)
+        orat, oslack = O.bprop(g["fanin_start"], g["fanin"], ad, po, at, fr, ends)
+        _same(rat.cpu().numpy(), orat)
+        _same(slack.cpu().numpy(), oslack)
+        assert np.signbit(orat[0]).tolist() == np.signbit(po[1, [0, 1, 2, 3]]).tolist()
 
 
 def test_fprop_empty_and_invalid(ctx):

@@ -336,3 +336,78 @@ def test_empty_graph_and_levels():
     lib, _ = _scalar_lib(1, np.random.default_rng(6))
     s, a, d = O.fprop(g, lib, np.zeros(0, np.int32), np.zeros(1, np.int32))
     assert s.shape == (0, 4) and a.shape == (0, 4) and d.shape == (0, 8)
+
+
+# ------------------------------------------------------- backward pass --
+@pytest.mark.parametrize("seed", range(4))
+def test_bprop_equals_max_min_over_enumerated_paths(seed):
+    # required time of a pin = max (early) / min (late) over every path to a
+    # pin with a primary-output value of that value minus the path's arc
+    # delays, subtracted from the far end back (the reference's order); the
+    # delays are the forward pass's, so only defined transitions count
+    rng = np.random.default_rng(200 + seed)
+    n = 22
+    perm = rng.permutation(n)
+    arcs, kind, timing = [], [], []
+    for a in range(n):
+        for b in range(a + 1, n):
+            if rng.random() < 0.2:
+                arcs.append((int(perm[a]), int(perm[b])))
+                kind.append(int(rng.choice(4, p=[0.35, 0.1, 0.45, 0.1])))
+                timing.append(int(rng.integers(0, 4)))
+    m = len(arcs)
+    indeg = np.zeros(n, int)
+    outdeg = np.zeros(n, int)
+    for u, v in arcs:
+        indeg[v] += 1
+        outdeg[u] += 1
+    pis = np.full((n, 4), NAN, np.float32)
+    pia = np.full((n, 4), NAN, np.float32)
+    src = indeg == 0
+    pis[src] = rng.uniform(3, 15, (int(src.sum()), 4))
+    pia[src] = rng.uniform(0, 8, (int(src.sum()), 4))
+    lib, _ = _scalar_lib(4, rng)
+    g, order = _graph(n, arcs, kind, timing, rng.uniform(0, 5, (m, 4)), rng.uniform(0, 30, (m, 4)),
+                      None, pis, pia)
+    fr, ends = _levels(g)
+    slew, at, ad = O.fprop(g, lib, fr, ends)
+    po = np.full((n, 4), NAN, np.float32)
+    for v in range(n):
+        if outdeg[v] == 0 or rng.random() < 0.15:
+            po[v] = rng.uniform(20, 60, 4)
+    rat, slack = O.bprop(g["fanin_start"], g["fanin"], ad, po, at, fr, ends)
+
+    fs, fi = g["fanin_start"], g["fanin"]
+    fo = [[] for _ in range(n)]
+    for v in range(n):
+        for j in range(fs[v], fs[v + 1]):
+            fo[fi[j]].append((j, v))
+
+    memo = {}
+
+    def values(u, el, rf):
+        key = (u, el, rf)
+        if key in memo:
+            return memo[key]
+        out = [] if np.isnan(po[u, el * 2 + rf]) else [np.float32(po[u, el * 2 + rf])]
+        for j, v in fo[u]:
+            for trf in range(2):
+                d = ad[j, 4 * el + 2 * rf + trf]
+                if np.isnan(d):
+                    continue
+                out += [np.float32(r - d) for r in values(v, el, trf)]
+        memo[key] = out
+        return out
+
+    for u in range(n):
+        for el in range(2):
+            for rf in range(2):
+                vals = values(u, el, rf)
+                want = NAN if not vals else (max(vals) if el == 0 else min(vals))
+                got = rat[u, el * 2 + rf]
+                assert (np.isnan(got) and np.isnan(want)) or got == np.float32(want), (u, el, rf)
+                a = at[u, el * 2 + rf]
+                s = NAN if np.isnan(a) or np.isnan(got) else (a - got if el == 0 else got - a)
+                assert (np.isnan(s) and np.isnan(slack[u, el * 2 + rf])) or \
+                    np.float32(s) == slack[u, el * 2 + rf]
+    assert np.isfinite(rat).sum() > 20

@@ -54,3 +54,17 @@ def test_fixture_shape(design):
                     assert np.all(np.diff(tab["index_2"]) > 0)
     for inst in J["instances"]:
         assert inst["cell"] in J["cells"]["early"]
+
+
+def test_c17_required_times_and_slacks_match_reference_output():
+    # backward propagation from the primary outputs' required times (c17 has
+    # no timing checks, so every required time comes from a primary output)
+    g, lib, pins, gold = T.build(T.oracle_rc, "c17")
+    fr, ends = T.levels(g)
+    slew, at, ad = O.fprop(g, lib, fr, ends)
+    po_rat, gr, gs = T.required("c17", pins)
+    rat, slack = O.bprop(g["fanin_start"], g["fanin"], ad, po_rat, at, fr, ends)
+    for got, want in ((rat, gr), (slack, gs)):
+        assert (~np.isnan(want)).sum() == 100
+        err = np.abs(got.astype(np.float64) - want)
+        assert (err <= tolerance(want)).all(), float(np.nanmax(err))
