@@ -169,3 +169,28 @@ def test_fanin_csr_candidates_and_loop_breakers():
     assert np.array_equal(csr.out_degree[cand == 1], outdeg[cand == 1])
     front = np.flatnonzero((cand == 1) & (outdeg == 0))
     assert np.array_equal(np.sort(csr.frontier), front)
+
+
+def test_fanout_csr_matches_a_stable_sort_of_the_fanin_csr():
+    from opentimer_amd import native as N
+    from opentimer_amd import workloads as W
+    import oracle as O
+    csr = W.fanin_csr(W.generate_dag(W.config0_dag(seed=31)))
+    n, m = csr.n, len(csr.fanin)
+    fo_start = np.zeros(n + 1, np.int32)
+    fo_arc = np.zeros(m, np.int32)
+    arc_to = np.zeros(m, np.int32)
+    N.check(N.lib.otg_build_fanout_csr(n, N.addr(csr.start), N.addr(csr.fanin), N.addr(fo_start),
+                                       N.addr(fo_arc), N.addr(arc_to)))
+    s, a, t = O.fanout_csr(csr.start, csr.fanin)
+    assert np.array_equal(fo_start, s) and np.array_equal(fo_arc, a) and np.array_equal(arc_to, t)
+    # every arc once, under its source, in arc order
+    assert np.array_equal(np.sort(fo_arc), np.arange(m))
+    for u in (0, n // 2, n - 1):
+        arcs = fo_arc[fo_start[u]:fo_start[u + 1]]
+        assert np.all(np.diff(arcs) > 0) and np.all(csr.fanin[arcs] == u)
+    # a source outside the pins is refused
+    bad = csr.fanin.copy()
+    bad[0] = n
+    assert N.lib.otg_build_fanout_csr(n, N.addr(csr.start), N.addr(bad), N.addr(fo_start),
+                                      N.addr(fo_arc), N.addr(arc_to)) == 1
