@@ -335,6 +335,9 @@ struct PlanView {
   float* slot_rc;            // [8 m_lv] RC delay[4], impulse[4]
   float* pos_in;             // [12 n_lv] load[4], PI slew[4], PI arrival[4]
   float* pos_sa;             // [8 n_lv] slew[4], arrival[4]
+  const int32_t* slot_pos;   // [m_lv] position whose fanin arc the slot is
+  const int2* wchunk;        // wavefront chunks: positions [x, y)
+  const int32_t* wstart;     // [levels+1] first chunk of every level
 };
 
 struct PlanParams {
@@ -374,29 +377,77 @@ struct Planned {
   __device__ int64_t arc_id(int k) const { return P.pv.slot_arc[k]; }
 };
 
+// One level over the plan by wavefront chunks: a chunk is consecutive
+// positions (at most 16) whose fanin arcs fill at most 16 slots (a position
+// with more arcs is a chunk of its own, taken 16 slots at a time).  Lane
+// 4*i + c computes the candidates of slot i for corner c (both input
+// transitions) -- one pass of table work for the chunk's arcs -- and lane
+// 4*p + c, the owner of position p, relaxes its slew and arrival over its
+// slots in arc order, fetching the candidates by wavefront shuffles: the
+// same operations in the same order as fprop_one.
 template <bool kLds>
-__global__ __launch_bounds__(1024) void fprop_plan_level_kernel(PlanParams P, int lv) {
-  const int b = P.pv.ends[lv], e = P.pv.ends[lv + 1];
-  const int64_t total = 4LL * (e - b);
+__global__ __launch_bounds__(1024) void fprop_wave_level_kernel(PlanParams P, int lv) {
+  const int c0 = P.pv.wstart[lv], nch = P.pv.wstart[lv + 1] - c0;
   LibView L = device_library(P.lib);
   if constexpr (kLds) {
     __shared__ int32_t lds[kLdsWords];
-    const int64_t share0 = ((total + gridDim.x - 1) / gridDim.x + 63) & ~63LL;
-    if ((int64_t)blockIdx.x * share0 >= total) return;
+    if ((int)blockIdx.x >= nch) return;
     L = stage_library(P.lib, P.pool_words, lds);
   }
   const Planned S{P};
-  // the level's lanes in equal wavefront-aligned shares, so that every
-  // workgroup (one per CU) has work
-  const int64_t share = ((total + gridDim.x - 1) / gridDim.x + 63) & ~63LL;
-  const int64_t lo = (int64_t)blockIdx.x * share, hi = min(total, lo + share);
-  for (int64_t idx = lo + threadIdx.x; idx < hi; idx += blockDim.x) {
-    const int q = b + (int)(idx >> 2), c = (int)(idx & 3);
-    const float* in = P.pv.pos_in + 12LL * q;
-    float s = in[4 + c], a = in[8 + c];
-    fprop_one(S, L, P.out.arc_delay, q, c, s, a, in[c]);
-    P.pv.pos_sa[8LL * q + c] = s;
-    P.pv.pos_sa[8LL * q + 4 + c] = a;
+  const int lane = threadIdx.x & 63, c = lane & 3, el = c >> 1, trf = c & 1;
+  const int twaves = gridDim.x * (blockDim.x >> 6);
+  for (int w = blockIdx.x + gridDim.x * (threadIdx.x >> 6); w < nch; w += twaves) {
+    const int2 qq = P.pv.wchunk[c0 + w];
+    const int q0 = qq.x, q1 = qq.y;
+    const int k0 = P.pv.aofs[q0], ns = P.pv.aofs[q1] - k0;
+    const int pq = q0 + (lane >> 2);
+    const bool owner = pq < q1;
+    float s = __builtin_nanf(""), a = __builtin_nanf("");
+    int ks = 0, ke = 0;
+    if (owner) {
+      const float* in = P.pv.pos_in + 12LL * pq;
+      s = in[4 + c];
+      a = in[8 + c];
+      ks = P.pv.aofs[pq] - k0;
+      ke = P.pv.aofs[pq + 1] - k0;
+    }
+    for (int sb = 0; sb < ns; sb += 16) {
+      float so[2] = {__builtin_nanf(""), __builtin_nanf("")};
+      float ac[2] = {__builtin_nanf(""), __builtin_nanf("")};
+      const int sl = sb + (lane >> 2);
+      if (sl < ns) {
+        const int k = k0 + sl;
+        int kind, t, src;
+        float rd, ri, su[2], au[2], d[2];
+        S.arc(k, c, kind, t, src, rd, ri);
+        S.source(src, el, su, au);
+        const float lc = P.pv.pos_in[12LL * P.pv.slot_pos[k] + c];
+        arc_candidates(L, kind, t, rd, ri, su, lc, el, trf, so, d);
+        const int64_t j = S.arc_id(k);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+          if (P.out.arc_delay) P.out.arc_delay[8 * j + 4 * el + 2 * f + trf] = d[f];
+          if (!__builtin_isnan(d[f]) && !__builtin_isnan(au[f])) ac[f] = d[f] + au[f];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int from = 4 * i + c;
+        const float s0 = __shfl(so[0], from), s1 = __shfl(so[1], from);
+        const float a0 = __shfl(ac[0], from), a1 = __shfl(ac[1], from);
+        if (owner && sb + i >= ks && sb + i < ke) {
+          s = relax(s, s0, el);
+          a = relax(a, a0, el);
+          s = relax(s, s1, el);
+          a = relax(a, a1, el);
+        }
+      }
+    }
+    if (owner) {
+      P.pv.pos_sa[8LL * pq + c] = s;
+      P.pv.pos_sa[8LL * pq + 4 + c] = a;
+    }
   }
 }
 
@@ -420,7 +471,8 @@ __global__ void plan_pos_kernel(const int32_t* frontiers, int32_t n_lv, const in
 
 __global__ void plan_slot_kernel(const int32_t* frontiers, int32_t n_lv, const int32_t* fanin_start,
                                  const int32_t* fanin, const int32_t* pin_pos,
-                                 const int32_t* aofs, int32_t* slot_src, int32_t* slot_arc) {
+                                 const int32_t* aofs, int32_t* slot_src, int32_t* slot_arc,
+                                 int32_t* slot_pos) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
        q += (int64_t)gridDim.x * blockDim.x) {
     const int v = frontiers[q];
@@ -429,6 +481,7 @@ __global__ void plan_slot_kernel(const int32_t* frontiers, int32_t n_lv, const i
       const int j = j0 + r, u = fanin[j], pu = pin_pos[u];
       slot_src[k0 + r] = pu >= 0 ? pu : -u - 1;
       slot_arc[k0 + r] = j;
+      slot_pos[k0 + r] = (int)q;
     }
   }
 }
@@ -620,6 +673,10 @@ struct otg_fprop_plan {
   float* slot_rc = nullptr;
   float* pos_in = nullptr;
   float* pos_sa = nullptr;
+  int32_t* slot_pos = nullptr;
+  int2* wchunk = nullptr;
+  int32_t* wstart = nullptr;
+  int32_t num_chunks = 0;
 };
 
 namespace {
@@ -627,7 +684,7 @@ namespace {
 void plan_free(otg_fprop_plan* p) {
   for (void* x : {(void*)p->frontiers, (void*)p->ends, (void*)p->aofs, (void*)p->slot_src,
                   (void*)p->slot_arc, (void*)p->slot_kt, (void*)p->slot_rc, (void*)p->pos_in,
-                  (void*)p->pos_sa})
+                  (void*)p->pos_sa, (void*)p->slot_pos, (void*)p->wchunk, (void*)p->wstart})
     if (x) (void)hipFree(x);
   delete p;
 }
@@ -729,12 +786,46 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
       (e = hipMalloc(&p->slot_kt, 2 * I * std::max(m_lv, 1))) ||
       (e = hipMalloc(&p->slot_rc, 8 * sizeof(float) * std::max(m_lv, 1))) ||
       (e = hipMalloc(&p->pos_in, 12 * sizeof(float) * std::max(n_lv, 1))) ||
-      (e = hipMalloc(&p->pos_sa, 8 * sizeof(float) * std::max(n_lv, 1))))
+      (e = hipMalloc(&p->pos_sa, 8 * sizeof(float) * std::max(n_lv, 1))) ||
+      (e = hipMalloc(&p->slot_pos, I * std::max(m_lv, 1))))
     return fail(e);
   if (n_lv > 0)
     hipLaunchKernelGGL(plan_slot_kernel, dim3(grid_for(ctx, n_lv, 256)), dim3(256), 0, s,
                        p->frontiers, n_lv, fanin_start, fanin, pin_pos, p->aofs, p->slot_src,
-                       p->slot_arc);
+                       p->slot_arc, p->slot_pos);
+  // wavefront chunks, level by level: consecutive positions, at most 16 of
+  // them and at most 16 arc slots (a position with more slots alone)
+  {
+    std::vector<int32_t> aofs(n_lv + 1), ends(num_levels + 1, 0);
+    if ((e = hipMemcpyAsync(aofs.data(), p->aofs, I * (n_lv + 1), hipMemcpyDeviceToHost, s)) ||
+        (num_levels > 0 && (e = hipMemcpyAsync(ends.data(), p->ends, I * (num_levels + 1),
+                                               hipMemcpyDeviceToHost, s))) ||
+        (e = hipStreamSynchronize(s)))
+      return fail(e);
+    std::vector<int2> chunks;
+    std::vector<int32_t> wstart(num_levels + 1, 0);
+    chunks.reserve(n_lv / 4 + num_levels + 1);
+    for (int32_t lv = 0; lv < num_levels; ++lv) {
+      wstart[lv] = (int32_t)chunks.size();
+      int32_t q = ends[lv];
+      while (q < ends[lv + 1]) {
+        int32_t q1 = q + 1;
+        while (q1 < ends[lv + 1] && q1 - q < 16 && aofs[q1 + 1] - aofs[q] <= 16) ++q1;
+        chunks.push_back(make_int2(q, q1));
+        q = q1;
+      }
+    }
+    wstart[num_levels] = (int32_t)chunks.size();
+    p->num_chunks = (int32_t)chunks.size();
+    if ((e = hipMalloc(&p->wchunk, sizeof(int2) * std::max<size_t>(chunks.size(), 1))) ||
+        (e = hipMalloc(&p->wstart, I * (num_levels + 1))) ||
+        (!chunks.empty() && (e = hipMemcpyAsync(p->wchunk, chunks.data(),
+                                                sizeof(int2) * chunks.size(),
+                                                hipMemcpyHostToDevice, s))) ||
+        (e = hipMemcpyAsync(p->wstart, wstart.data(), I * (num_levels + 1),
+                            hipMemcpyHostToDevice, s)))
+      return fail(e);
+  }
   if ((e = hipStreamSynchronize(s)) || (e = hipGetLastError())) return fail(e);
   (void)hipFree(pin_pos);
   (void)hipFree(deg);
@@ -775,8 +866,9 @@ extern "C" otg_status otg_fprop_run(otg_context* ctx, otg_fprop_plan* plan,
     return OTG_ERR_INVALID_ARGUMENT;
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = otg_pick_stream(ctx, stream);
-  PlanView pv{plan->frontiers, plan->ends,    plan->aofs,   plan->slot_src, plan->slot_arc,
-              plan->slot_kt,   plan->slot_rc, plan->pos_in, plan->pos_sa};
+  PlanView pv{plan->frontiers, plan->ends,     plan->aofs,     plan->slot_src,
+              plan->slot_arc,  plan->slot_kt,  plan->slot_rc,  plan->pos_in,
+              plan->pos_sa,    plan->slot_pos, plan->wchunk,   plan->wstart};
   auto a16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
   const bool vec_arcs = a16(graph->arc_rc_delay) && a16(graph->arc_rc_impulse);
   const bool vec_pins = a16(graph->pin_load) && a16(graph->pi_slew) && a16(graph->pi_at);
@@ -791,11 +883,11 @@ extern "C" otg_status otg_fprop_run(otg_context* ctx, otg_fprop_plan* plan,
   PlanParams P{pv, *lib, *out, lib->num_lut_vals};
   if (lib_in_lds(lib)) {
     for (int lv = plan->num_levels - 1; lv >= 0; --lv)
-      hipLaunchKernelGGL(fprop_plan_level_kernel<true>, dim3(std::max(1, ctx->num_cus)),
+      hipLaunchKernelGGL(fprop_wave_level_kernel<true>, dim3(std::max(1, ctx->num_cus)),
                          dim3(1024), 0, s, P, lv);
   } else {
     for (int lv = plan->num_levels - 1; lv >= 0; --lv)
-      hipLaunchKernelGGL(fprop_plan_level_kernel<false>, dim3(std::max(1, ctx->num_cus * 2)),
+      hipLaunchKernelGGL(fprop_wave_level_kernel<false>, dim3(std::max(1, ctx->num_cus * 2)),
                          dim3(1024), 0, s, P, lv);
   }
   if (vec_out) hipLaunchKernelGGL(plan_scatter_kernel<true>, gp, dim3(256), 0, s, pv, *out, plan->n_lv);
