@@ -607,13 +607,27 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
         if (c == 0) queue[atomicAdd(&qn, 1)] = u;
       } else {
         float r = G.po_rat[4 * u + c];
-        for (int k = k0; k < k1; ++k) {
-          const int j = G.fanout_arc[k], v = G.arc_to[j];
-          const float2 d =
-              *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j + 4 * el + 2 * frf);
-          const float2 rv = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
-          if (!__builtin_isnan(d.x) && !__builtin_isnan(rv.x)) r = relax_rat(r, rv.x - d.x, el);
-          if (!__builtin_isnan(d.y) && !__builtin_isnan(rv.y)) r = relax_rat(r, rv.y - d.y, el);
+        // 4 arcs per step: their ids, then sinks and delays, then the sinks'
+        // required times are loaded for all 4 before the relaxation
+        for (int k = k0; k < k1; k += 4) {
+          int j[4];
+          float2 d[4], rv[4];
+#pragma unroll
+          for (int z = 0; z < 4; ++z) j[z] = G.fanout_arc[k + z < k1 ? k + z : k0];
+#pragma unroll
+          for (int z = 0; z < 4; ++z) {
+            const int v = G.arc_to[j[z]];
+            d[z] = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j[z] + 4 * el + 2 * frf);
+            rv[z] = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
+          }
+#pragma unroll
+          for (int z = 0; z < 4; ++z) {
+            if (k + z >= k1) break;
+            if (!__builtin_isnan(d[z].x) && !__builtin_isnan(rv[z].x))
+              r = relax_rat(r, rv[z].x - d[z].x, el);
+            if (!__builtin_isnan(d[z].y) && !__builtin_isnan(rv[z].y))
+              r = relax_rat(r, rv[z].y - d[z].y, el);
+          }
         }
         P.rat[4 * u + c] = r;
         if (P.slack) {
@@ -632,12 +646,26 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
       const int k0 = G.fanout_start[u], k1 = G.fanout_start[u + 1];
       float r = a0 == 0 ? G.po_rat[4 * u + c] : __builtin_nanf("");
       int ri = -1;
-      for (int k = k0 + a0; k < k1; k += 16) {
-        const int j = G.fanout_arc[k], v = G.arc_to[j];
-        const float2 d = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j + 4 * el + 2 * frf);
-        const float2 rv = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
-        if (!__builtin_isnan(d.x) && !__builtin_isnan(rv.x)) rat_pick(r, ri, rv.x - d.x, 2 * (k - k0), el);
-        if (!__builtin_isnan(d.y) && !__builtin_isnan(rv.y)) rat_pick(r, ri, rv.y - d.y, 2 * (k - k0) + 1, el);
+      for (int kb = k0 + a0; kb < k1; kb += 64) {
+        int j[4];
+        float2 d[4], rv[4];
+#pragma unroll
+        for (int z = 0; z < 4; ++z) j[z] = G.fanout_arc[kb + 16 * z < k1 ? kb + 16 * z : k0];
+#pragma unroll
+        for (int z = 0; z < 4; ++z) {
+          const int v = G.arc_to[j[z]];
+          d[z] = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j[z] + 4 * el + 2 * frf);
+          rv[z] = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
+        }
+#pragma unroll
+        for (int z = 0; z < 4; ++z) {
+          const int k = kb + 16 * z;
+          if (k >= k1) break;
+          if (!__builtin_isnan(d[z].x) && !__builtin_isnan(rv[z].x))
+            rat_pick(r, ri, rv[z].x - d[z].x, 2 * (k - k0), el);
+          if (!__builtin_isnan(d[z].y) && !__builtin_isnan(rv[z].y))
+            rat_pick(r, ri, rv[z].y - d[z].y, 2 * (k - k0) + 1, el);
+        }
       }
 #pragma unroll
       for (int m = 4; m < 64; m <<= 1) {
