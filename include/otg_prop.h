@@ -122,9 +122,11 @@ otg_status otg_fprop(otg_context* ctx, const otg_prop_graph* graph, const otg_pr
  * this library's layout for the same per-level tasks, timer.cpp:1079-1086).
  * The pins of the frontiers are renumbered to their position in the
  * frontiers, so that the pins of a level are consecutive, and their fanin
- * arcs to consecutive slots in that order.  Built once per levelization
- * (allocates, synchronises `stream`); the arrays it was built from may be
- * freed afterwards. */
+ * arcs to consecutive slots in that order; the positions of every level are
+ * packed into wavefront chunks (at most 16 positions and 16 arc slots, a
+ * position with more slots alone).  Built once per levelization (allocates,
+ * copies the slot offsets to the host for the packing, synchronises
+ * `stream`); the arrays it was built from may be freed afterwards. */
 typedef struct otg_fprop_plan otg_fprop_plan;
 
 otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, int32_t num_arcs,

@@ -238,6 +238,51 @@ def test_bprop_bitexact(ctx, cfg):
     assert np.isfinite(orat).mean() > 0.5
 
 
+def test_fprop_plan_high_fanin_positions(ctx):
+    # positions with more than 16 fanin arcs are wavefront chunks of their
+    # own, taken 16 arc slots at a time: fanins of 100, 17..40 and 11 (the
+    # generated DAGs stay below 16)
+    from test_prop_oracle import _graph, _levels
+    rng = np.random.default_rng(41)
+    n_src = 100
+    arcs, kind, timing = [], [], []
+
+    def add(u, v):
+        arcs.append((u, v))
+        kind.append(int(rng.choice(4, p=[0.4, 0.1, 0.45, 0.05])))
+        timing.append(int(rng.integers(0, 64)))
+    for u in range(n_src):
+        add(u, n_src)                               # pin 100: fanin 100
+    for k in range(10):                             # pins 101..110: fanin 17..40
+        for u in rng.choice(n_src, int(rng.integers(17, 41)), replace=False):
+            add(int(u), n_src + 1 + k)
+    for k in range(11):                             # pin 111: fanin 11
+        add(n_src + k, n_src + 11)
+    n = n_src + 12
+    m = len(arcs)
+    pis = np.full((n, 4), np.nan, np.float32)
+    pia = np.full((n, 4), np.nan, np.float32)
+    pis[:n_src] = rng.uniform(5, 20, (n_src, 4))
+    pia[:n_src] = rng.uniform(0, 10, (n_src, 4))
+    rcd = rng.uniform(0, 5, (m, 4)).astype(np.float32)
+    rci = rng.uniform(0, 40, (m, 4)).astype(np.float32)
+    g, _ = _graph(n, arcs, kind, timing, rcd, rci, rng.uniform(1, 200, (n, 4)), pis, pia)
+    lib = W.prop_library(seed=11)
+    fr, ends = _levels(g)
+    P = Propagator(W.PropGraphData(**g), lib, ctx)
+    frt = torch.from_numpy(fr).cuda()
+    ent = torch.from_numpy(ends).cuda()
+    P.build_plan(frt, ent, len(ends) - 1)
+    P.run_plan()
+    slew, at, ad = P.host()
+    os_, oa, od = O.fprop(g, {k: getattr(lib, k) for k in LKEYS}, fr, ends)
+    _same(slew, os_)
+    _same(at, oa)
+    _same(ad, od)
+    assert np.isfinite(oa[n_src:]).all()
+    P.close_plan()
+
+
 @pytest.mark.parametrize("fanout", [5, 40])
 def test_bprop_signed_zero_ties_follow_fanout_order(ctx, fanout):
     # a hub pin drives `fanout` sinks through net arcs without parasitics
