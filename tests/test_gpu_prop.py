@@ -315,6 +315,49 @@ def test_bprop_signed_zero_ties_follow_fanout_order(ctx, fanout):
         assert np.signbit(orat[0]).tolist() == np.signbit(po[1, [0, 1, 2, 3]]).tolist()
 
 
+def test_fprop_and_bprop_replayed_in_a_hip_graph(ctx):
+    # a plan run and a backward pass allocate nothing and clear nothing with
+    # memset: captured once into a hipGraph, every replay (with eager work and
+    # synchronisation between replays) gives the eager results bit for bit
+    from opentimer_amd.incremental import Graph
+    cfg = W.config0_dag(seed=16)
+    slew, at, ad, P, L = _run(ctx, cfg)
+    lv = L.num_ends - 1
+    n = P.n
+    out_deg = np.bincount(P._t["fanin"].cpu().numpy()[: P.m], minlength=n)
+    po = np.full((n, 4), np.nan, np.float32)
+    po[out_deg == 0] = 900.0
+    rat0, slack0 = (t.cpu().numpy() for t in P.required(po, L.frontiers, L.ends, lv))
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    h = stream.cuda_stream
+    ctx.check(N.lib.otg_graph_begin(ctx.ptr, h))
+    try:
+        P.run_plan(stream=h)
+        P.rerun_required(L.frontiers, L.ends, lv, stream=h)
+    finally:
+        gh = C.c_void_p()
+        st = N.lib.otg_graph_end(ctx.ptr, h, C.byref(gh))
+    ctx.check(st)
+    G = Graph(ctx, gh)
+    try:
+        for rep in range(3):
+            for t in (P.slew, P.at, P.arc_delay, P._bp[2], P._bp[3]):
+                t.fill_(float("nan"))
+            torch.cuda.synchronize()
+            G.launch(stream)
+            stream.synchronize()
+            s2, a2, d2 = P.host()
+            _same(s2, slew)
+            _same(a2, at)
+            _same(d2, ad)
+            _same(P._bp[2][:n].cpu().numpy(), rat0)
+            _same(P._bp[3][:n].cpu().numpy(), slack0)
+    finally:
+        G.close()
+        P.close_plan()
+
+
 def test_fprop_empty_and_invalid(ctx):
     lib = W.prop_library(num_timings=2)
     g = W.PropGraphData(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32),
