@@ -643,8 +643,9 @@ def run_fprop(ctx, stream, sh, reps=10, cpu=True):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
-              "(nets shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: nets shard "
+              "into independent contiguous ranges, one per GPU, and the figure of one GPU "
+              "is what is measured", file=sys.stderr)
         sys.exit(2)
     import torch
 
