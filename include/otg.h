@@ -309,6 +309,23 @@ otg_status otg_levelize(otg_context* ctx, int32_t n, int32_t num_edges,
                         int32_t first_size, int32_t* frontiers, int32_t* frontiers_ends,
                         int32_t* level_of_pin, int32_t* num_ends, otg_stream stream);
 
+/* otg_levelize without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * semantics and results as otg_levelize; its working arrays (the consumed
+ * fanout counts, the per-level append counters and the end flag, 2n+3 ints)
+ * live in a fixed array of the library on each device (about 16 MB, held
+ * until the process ends), so n may not exceed
+ * OTG_LEVELIZE_NOALLOC_MAX_PINS (OTG_ERR_INVALID_ARGUMENT otherwise).  Every
+ * call on one GPU uses that GPU's one array: calls that overlap on a GPU
+ * (from any context or stream) race.  Synchronous (it returns *num_ends). */
+#define OTG_LEVELIZE_NOALLOC_MAX_PINS (1 << 21)
+otg_status otg_levelize_noalloc(otg_context* ctx, int32_t n, int32_t num_edges,
+                                const int32_t* fanin_start, const int32_t* fanin,
+                                const int32_t* out_degree, const int32_t* first_frontier,
+                                int32_t first_size, int32_t* frontiers, int32_t* frontiers_ends,
+                                int32_t* level_of_pin, int32_t* num_ends, otg_stream stream);
+
 /* ------------------------------------------------------------------------ */
 /* Incremental re-levelization and hipGraph capture                          */
 /* ------------------------------------------------------------------------ */

@@ -20,8 +20,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "otg_internal.hpp"
+
+// Working memory of otg_levelize_noalloc: the consumed fanout counts [n], the
+// per-level append counters [n+2] and the end flag [1] of one levelization,
+// for at most OTG_LEVELIZE_NOALLOC_MAX_PINS pins.  Each GPU holds its own copy
+// for the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (the array's name is not a device
+// address in host code).
+constexpr int64_t kNoAllocInts = 2LL * OTG_LEVELIZE_NOALLOC_MAX_PINS + 8;
+static __device__ __align__(256) int32_t scratch[kNoAllocInts];
 
 namespace {
 
@@ -286,6 +296,90 @@ otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_si
     return OTG_ERR_CYCLE;
   }
   *num_ends_host = ne;
+  return OTG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Levelization without allocation: otg_levelize on the library's fixed
+// per-device array.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+int32_t* g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on ctx's device, looked up once per device with
+// that device current.
+otg_status noalloc_scratch(otg_context* ctx, int32_t** out) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) {
+    OTG_SET_ERR(ctx, "device %d out of range for the fixed levelization array", ctx->device);
+    return OTG_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (!g_noalloc_ptr[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      OTG_SET_ERR(ctx, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return OTG_ERR_DEVICE;
+    }
+    g_noalloc_ptr[ctx->device] = static_cast<int32_t*>(p);
+  }
+  *out = g_noalloc_ptr[ctx->device];
+  return OTG_OK;
+}
+}  // namespace
+
+extern "C" otg_status otg_levelize_noalloc(otg_context* ctx, int32_t n, int32_t num_edges,
+                                           const int32_t* fanin_start, const int32_t* fanin,
+                                           const int32_t* out_degree,
+                                           const int32_t* first_frontier, int32_t first_size,
+                                           int32_t* frontiers, int32_t* frontiers_ends,
+                                           int32_t* level_of_pin, int32_t* num_ends,
+                                           otg_stream stream) {
+  if (!ctx || n < 0 || num_edges < 0 || first_size < 0 || first_size > n || !num_ends)
+    return OTG_ERR_INVALID_ARGUMENT;
+  if (n > OTG_LEVELIZE_NOALLOC_MAX_PINS) {
+    OTG_SET_ERR(ctx, "otg_levelize_noalloc: %d pins exceed the fixed array's %d", n,
+                OTG_LEVELIZE_NOALLOC_MAX_PINS);
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  if (n > 0 && (!fanin_start || !out_degree || !frontiers || !frontiers_ends ||
+                (num_edges > 0 && !fanin) || (first_size > 0 && !first_frontier)))
+    return OTG_ERR_INVALID_ARGUMENT;
+  hipStream_t s = otg_pick_stream(ctx, stream);
+  OTG_HIP(ctx, hipSetDevice(ctx->device));
+  if (n == 0) {
+    // the same answer as otg_levelize (ends = {0, 0}), written by a kernel
+    if (frontiers_ends) {
+      int32_t* dev = nullptr;
+      otg_status st0 = noalloc_scratch(ctx, &dev);
+      if (st0 != OTG_OK) return st0;
+      LevelizeParams P{};
+      P.ends = frontiers_ends;
+      P.cnt = dev;
+      P.num_ends = dev + 2;
+      hipLaunchKernelGGL(levelize_init_kernel, dim3(1), dim3(kBlock), 0, s, P);
+      OTG_HIP(ctx, hipGetLastError());
+    }
+    OTG_HIP(ctx, hipStreamSynchronize(s));
+    *num_ends = 2;
+    return OTG_OK;
+  }
+  int32_t* dev = nullptr;
+  otg_status st = noalloc_scratch(ctx, &dev);
+  if (st != OTG_OK) return st;
+  int32_t* out_copy = dev;        // [n]
+  int32_t* counters = dev + n;    // [n+3]: append counters, then the end flag
+  OTG_HIP(ctx, hipMemcpyAsync(out_copy, out_degree, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  if (first_size > 0 && first_frontier != frontiers)
+    OTG_HIP(ctx, hipMemcpyAsync(frontiers, first_frontier, sizeof(int32_t) * first_size,
+                                hipMemcpyDeviceToDevice, s));
+  int32_t ne = 0;
+  st = otg_levelize_launch_raw(ctx, n, first_size, fanin_start, fanin, out_copy, frontiers,
+                               frontiers_ends, level_of_pin, counters, &ne, s);
+  if (st != OTG_OK) return st;
+  *num_ends = ne;
   return OTG_OK;
 }
 

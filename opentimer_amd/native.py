@@ -193,6 +193,9 @@ _sig("otg_toposort_compute", C.c_int, ctx_p, C.c_int32, C.c_int32, C.c_int32, vp
      i32p, vp)
 _sig("otg_levelize", C.c_int, ctx_p, C.c_int32, C.c_int32, vp, vp, vp, vp, C.c_int32, vp, vp, vp,
      i32p, vp)
+_sig("otg_levelize_noalloc", C.c_int, ctx_p, C.c_int32, C.c_int32, vp, vp, vp, vp, C.c_int32, vp, vp,
+     vp, i32p, vp)
+LEVELIZE_NOALLOC_MAX_PINS = 1 << 21
 _sig("otg_incr_create", C.c_int, ctx_p, C.POINTER(IncrGraph), C.c_int32, C.POINTER(C.c_void_p))
 _sig("otg_incr_destroy", C.c_int, C.c_void_p)
 _sig("otg_incr_relevelize", C.c_int, C.c_void_p, vp, C.c_int32, vp)
@@ -237,7 +240,7 @@ EXPORTED = [
     "otg_version", "otg_rct_plan_create", "otg_rct_plan_destroy", "otg_rct_plan_stats",
     "otg_rct_solve", "otg_rct_reference_f64", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
     "otg_rct_compute", "otg_rct_query",
-    "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize",
+    "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize", "otg_levelize_noalloc",
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",
     "otg_fprop", "otg_fprop_plan_create", "otg_fprop_plan_destroy", "otg_fprop_plan_sizes",
     "otg_fprop_run", "otg_bprop", "otg_build_fanout_csr", "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
