@@ -78,11 +78,33 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
             list(ex.map(run, cmds))
         out_tmp = LIB + ".tmp"
         run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", out_tmp,
-             "-pthread"])
+             "-pthread", "-Wl,-soname,libotg.so"])
         os.replace(out_tmp, LIB)
+        build_bridge(run)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return LIB
+
+
+BRIDGE_SRC = os.path.join(HERE, "bridge", "ot_cuda_bridge.cpp")
+BRIDGE_LIB = os.path.join(HERE, "libot_cuda_bridge.so")
+HARNESS_SRC = os.path.join(os.path.dirname(HERE), "tests", "bridge", "harness.cpp")
+HARNESS_LIB = os.path.join(os.path.dirname(HERE), "tests", "bridge", "libbridge_harness.so")
+
+
+def build_bridge(run) -> None:
+    """The OpenTimer bridge (the reference's GPU entry points with their C++
+    signatures, forwarding to libotg) and, when the test tree is present, the
+    test harness that calls it the way the reference's timer does."""
+    bridge_inc = os.path.join(HERE, "bridge")
+    run(["g++", *CXX_FLAGS, f"-I{bridge_inc}", "-shared", BRIDGE_SRC, f"-L{HERE}", "-lotg",
+         "-Wl,-rpath,$ORIGIN", "-Wl,-soname,libot_cuda_bridge.so", "-o", BRIDGE_LIB + ".tmp"])
+    os.replace(BRIDGE_LIB + ".tmp", BRIDGE_LIB)
+    if os.path.exists(HARNESS_SRC):
+        run(["g++", *CXX_FLAGS, f"-I{bridge_inc}", "-shared", HARNESS_SRC, f"-L{HERE}",
+             "-lot_cuda_bridge", "-Wl,-rpath,$ORIGIN/../../opentimer_amd",
+             "-o", HARNESS_LIB + ".tmp"])
+        os.replace(HARNESS_LIB + ".tmp", HARNESS_LIB)
 
 
 if __name__ == "__main__":
