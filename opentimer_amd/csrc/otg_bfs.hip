@@ -3,11 +3,13 @@
 // block_couting_sort (ot/cuda/sort.cuh:63-142) and prepare_res_cap
 // (rct.cu:314-359).
 //
-// One wavefront per net.  The net's undirected edges are turned into an
-// adjacency list in LDS (nets up to kLdsNodes) or in global scratch (larger
-// nets); the BFS runs frontier by frontier with ballot compaction; the
-// ordering is a stable counting sort by distance in which node ids are
-// visited from the largest down, so that the position of node u is
+// Nets of up to 64 nodes are packed into wavefronts (one lane per node);
+// larger nets take one wavefront each, with their adjacency list, distances
+// and BFS queue in LDS as 16-bit ids (nets up to kMaxLds nodes) or in global
+// scratch (larger nets).  The BFS runs frontier by frontier with ballot
+// compaction; the ordering is a stable counting sort by distance in which
+// node ids are visited from the largest down, so that the position of node u
+// is
 //   #(nodes with smaller distance) + #(nodes with equal distance, larger id)
 // -- the order the reference's bucket-end counting sort produces.  Work is
 // O(nodes) per net (the reference rescans every edge once per BFS level).
@@ -20,13 +22,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "otg_internal.hpp"
 
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kLdsNodes = 512;  // nets up to this size keep their scratch in LDS
 
 struct BfsParams {
   int32_t num_nets;
@@ -99,7 +101,7 @@ __device__ __forceinline__ void bfs_net(const BfsParams& P, int net, int32_t* ba
   M::sync();
   ok = __all(ok);
   if (!ok) {
-    if (lane == 0) atomicMin(P.bad, net);
+    atomicMin(P.bad, net);  // every lane: the exit stays uniform
     return;
   }
   // exclusive offsets: wave-wide inclusive scan in chunks of 64
@@ -159,7 +161,7 @@ __device__ __forceinline__ void bfs_net(const BfsParams& P, int net, int32_t* ba
     ++level;
   }
   if (fe != n) {
-    if (lane == 0) atomicMin(P.bad, net);
+    atomicMin(P.bad, net);  // every lane: the exit stays uniform
     return;
   }
   // level sizes -> level start positions (cnt[d])
@@ -275,7 +277,7 @@ __device__ __forceinline__ int wave_prefix_max(int v, int lane) {
 __device__ __forceinline__ bool small_batch_failed(const BfsParams& P, int a, int badj, int lane) {
   const int b = __shfl(wave_prefix_max(badj >= 0 ? -badj : INT32_MIN, lane), kWave - 1);
   if (b == INT32_MIN) return false;
-  if (lane == 0) atomicMin(P.bad, a - b);
+  atomicMin(P.bad, a - b);  // every lane: the exit stays uniform
   return true;
 }
 
@@ -413,19 +415,304 @@ __global__ __launch_bounds__(64) void rct_bfs_small_kernel(BfsParams P) {
   }
 }
 
-__global__ __launch_bounds__(64) void rct_bfs_kernel(BfsParams P) {
-  __shared__ int32_t lds[6 * kLdsNodes + 2];
-  const int lane = threadIdx.x;
-  for (int net = blockIdx.x; net < P.num_nets; net += gridDim.x) {
-    const int n = P.nodes_start[net + 1] - P.nodes_start[net];
-    if (n <= kSmallNodes) continue;  // rct_bfs_small_kernel
-    if (n <= kLdsNodes) {
-      bfs_net<false>(P, net, lds, lds, 0, lane);
+// ---------------------------------------------------------------------------
+// Large nets (more than 64 nodes, at most kMaxLds): one wavefront per net with
+// its whole working set in LDS as 16-bit node ids, 10 bytes per node in size
+// classes of CAP nodes (CAP = 1024 ... 8192, 10 KB ... 80 KB), so that smaller
+// nets run more wavefronts per CU.  Nets come from a per-class list through
+// an atomic counter.  Phases (each bounded by the net's size):
+//   1. degrees, counted with 32-bit LDS atomics on packed 16-bit counters;
+//   2. exclusive offsets by a wave scan; 3. adjacency filled through packed
+//      16-bit cursors;
+//   4. frontier BFS over a 16-bit queue with ballot compaction (a level
+//      appends at most what the queue holds; at most n levels; a node's
+//      neighbours are scanned at most 2(n-1) steps);
+//   5. level sizes, their exclusive starts, and the positions: ids from the
+//      largest down, lanes of equal distance take consecutive positions from
+//      one cursor claim per distinct distance per chunk;
+//   6. oriented edges, pid, R and C in BFS order.
+// A net that is not a spanning tree (an endpoint out of range, a self loop,
+// a node left unreached or reached twice) reports its index.
+// ---------------------------------------------------------------------------
+constexpr int kMaxLds = 8192;
+constexpr uint32_t kUnseen = 0xffffu;
+
+template <int CAP>
+struct BfsLds16 {
+  uint32_t off[CAP / 2 + 2];  // degrees -> exclusive offsets [n+1]; after the BFS: level cursors
+  uint32_t adj[CAP];          // [2(n-1)] neighbours
+  uint32_t dist[CAP / 2];     // [n] distance, then BFS position
+  uint32_t q[CAP / 2];        // [n] fill cursors, then the BFS queue
+};
+
+__device__ __forceinline__ int ld16(const uint32_t* a, int i) {
+  return (int)reinterpret_cast<const uint16_t*>(a)[i];
+}
+__device__ __forceinline__ void st16(uint32_t* a, int i, int v) {
+  reinterpret_cast<uint16_t*>(a)[i] = (uint16_t)v;
+}
+// Adds v to 16-bit entry i with a 32-bit atomic on its word; returns the
+// entry's old value (entries never exceed 65535, so no carry reaches the
+// neighbouring entry).
+__device__ __forceinline__ int add16(uint32_t* a, int i, int v) {
+  const int sh = (i & 1) * 16;
+  const uint32_t old = atomicAdd(&a[i >> 1], (uint32_t)v << sh);
+  return (int)((old >> sh) & 0xffffu);
+}
+
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return (1ull << lane) - 1ull; }
+
+template <int CAP>
+__device__ void bfs_net16(const BfsParams& P, int net, BfsLds16<CAP>& S, int lane) {
+  const int off0 = P.nodes_start[net];
+  const int n = P.nodes_start[net + 1] - off0;
+  if (n <= 0 || n > CAP) {  // not a net of this class (the lists never hold one)
+    if (n > CAP) atomicMin(P.bad, net);  // every lane: the exit stays uniform
+    return;
+  }
+  const int ne = n - 1;
+  const int32_t* e = P.edges_in + 2LL * (off0 - net);
+  const int root = P.roots[net];
+  bool bad = root < 0 || root >= n;
+  // ---- 1. degrees (entry v+1 counts v's edges) ----
+  for (int k = lane; k < n / 2 + 1; k += kWave) S.off[k] = 0u;
+  for (int k = lane; k < (n + 1) / 2; k += kWave) {
+    S.dist[k] = 0xffffffffu;
+    S.q[k] = 0u;
+  }
+  lds_sync();
+  for (int k = lane; k < ne; k += kWave) {
+    const int s = e[2 * k], t = e[2 * k + 1];
+    if (s < 0 || s >= n || t < 0 || t >= n || s == t) {
+      bad = true;
     } else {
-      bfs_net<true>(P, net, P.gscratch + 6LL * P.nodes_start[net] + 2LL * net, lds,
-                    6 * kLdsNodes + 2, lane);
+      add16(S.off, s + 1, 1);
+      add16(S.off, t + 1, 1);
     }
   }
+  lds_sync();
+  if (__any(bad)) {
+    atomicMin(P.bad, net);  // every lane: the exit stays uniform
+    return;
+  }
+  // ---- 2. offsets: inclusive scan of the shifted degrees ----
+  {
+    int carry = 0;
+    for (int b = 0; b <= n; b += kWave) {
+      const int k = b + lane;
+      int v = k <= n ? ld16(S.off, k) : 0;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(v, d);
+        if (lane >= d) v += t;
+      }
+      if (k <= n) st16(S.off, k, v + carry);
+      carry += __shfl(v, kWave - 1);
+    }
+  }
+  lds_sync();
+  // ---- 3. adjacency ----
+  for (int k = lane; k < ne; k += kWave) {
+    const int s = e[2 * k], t = e[2 * k + 1];
+    const int cs = add16(S.q, s, 1), ct = add16(S.q, t, 1);
+    st16(S.adj, ld16(S.off, s) + cs, t);
+    st16(S.adj, ld16(S.off, t) + ct, s);
+  }
+  lds_sync();
+  // ---- 4. BFS ----
+  if (lane == 0) {
+    st16(S.dist, root, 0);
+    st16(S.q, 0, root);
+  }
+  lds_sync();
+  int fb = 0, fe = 1, levels = 0;
+  bool over = false;
+  while (fb < fe && levels < n) {
+    int tail = fe;
+    for (int b = fb; b < fe; b += kWave) {
+      const int k = b + lane;
+      const int u = k < fe ? ld16(S.q, k) : -1;
+      const int jb = u >= 0 ? ld16(S.off, u) : 0;
+      const int je = u >= 0 ? ld16(S.off, u + 1) : 0;
+      for (int it = 0; it < 2 * ne; ++it) {
+        const int j = jb + it;
+        const bool has = j < je;
+        if (!__any(has)) break;
+        int v = 0;
+        bool take = false;
+        if (has) {
+          v = ld16(S.adj, j);
+          take = (uint32_t)ld16(S.dist, v) == kUnseen;
+        }
+        const uint64_t m = __ballot(take);
+        const int slot = tail + __popcll(m & lanes_below(lane));
+        if (take) {
+          st16(S.dist, v, levels + 1);
+          if (slot < n) st16(S.q, slot, v);
+        }
+        tail += __popcll(m);
+      }
+    }
+    lds_sync();
+    if (tail > n) {
+      over = true;
+      break;
+    }
+    fb = fe;
+    fe = tail;
+    ++levels;
+  }
+  bad = over || fe != n;
+  for (int k = lane; k < n && !bad; k += kWave)
+    if ((uint32_t)ld16(S.dist, k) == kUnseen) bad = true;
+  if (__any(bad)) {
+    atomicMin(P.bad, net);  // every lane: the exit stays uniform
+    return;
+  }
+  // ---- 5. level sizes and starts (in off's space), then positions ----
+  for (int k = lane; k < levels / 2 + 1; k += kWave) S.off[k] = 0u;
+  lds_sync();
+  for (int k = lane; k < n; k += kWave) add16(S.off, ld16(S.dist, k), 1);
+  lds_sync();
+  {
+    int carry = 0;
+    for (int b = 0; b < levels; b += kWave) {
+      const int k = b + lane;
+      const int own = k < levels ? ld16(S.off, k) : 0;
+      int v = own;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(v, d);
+        if (lane >= d) v += t;
+      }
+      if (k < levels) st16(S.off, k, v - own + carry);
+      carry += __shfl(v, kWave - 1);
+    }
+  }
+  lds_sync();
+  for (int top = n - 1; top >= 0; top -= kWave) {
+    const int u = top - lane;  // lane 0 holds the chunk's largest id
+    const bool valid = u >= 0;
+    const int d = valid ? ld16(S.dist, u) : -1;
+    uint64_t remaining = __ballot(valid);
+    int pos = 0;
+    for (int it = 0; it < kWave && remaining; ++it) {
+      const int leader = __ffsll((unsigned long long)remaining) - 1;
+      const int key = __shfl(d, leader);
+      const uint64_t m = __ballot(valid && d == key);
+      int base = 0;
+      if (lane == leader) base = add16(S.off, key, __popcll(m));
+      base = __shfl(base, leader);
+      if (valid && d == key) pos = base + __popcll(m & lanes_below(lane));
+      remaining &= ~m;
+    }
+    if (valid) {
+      P.node2bfs[off0 + u] = pos;
+      st16(S.dist, u, pos);  // this chunk's ids only: later chunks read their own
+    }
+  }
+  lds_sync();
+  // ---- 6. oriented edges, pid, R and C in BFS order ----
+  if (lane == 0) {
+    P.pid[off0] = 0;
+    P.pres[off0] = 0.0f;
+  }
+  int32_t* eo = P.edges_out + 2LL * (off0 - net);
+  const float* res = P.edge_res + (off0 - net);
+  for (int k = lane; k < ne; k += kWave) {
+    int s = e[2 * k], t = e[2 * k + 1];
+    int ps = ld16(S.dist, s), pt = ld16(S.dist, t);
+    if (ps > pt) {
+      const int x = s; s = t; t = x;
+      const int y = ps; ps = pt; pt = y;
+    }
+    eo[2 * k] = s;
+    eo[2 * k + 1] = t;
+    P.pid[off0 + pt] = pt - ps;
+    P.pres[off0 + pt] = res[k];
+  }
+  for (int k = lane; k < n; k += kWave)
+    P.cap_bfs[(int64_t)off0 + ld16(S.dist, k)] = P.node_cap[(int64_t)off0 + k];
+  lds_sync();  // the next net reuses the LDS
+}
+
+// Size classes of the on-chip kernel, and the class of the nets larger than
+// kMaxLds (global scratch).
+constexpr int kClasses = 5;
+constexpr int kClassCap[kClasses] = {1024, 2048, 4096, 8192, 0};
+
+__device__ __forceinline__ int bfs_class(int n) {
+  return n <= 1024 ? 0 : n <= 2048 ? 1 : n <= 4096 ? 2 : n <= kMaxLds ? 3 : 4;
+}
+
+// ctl: [0..4] nets per class, [5..9] list cursors, [10..14] queue heads,
+// [16..17] nodes of class-4 nets (int64), [18] largest class-4 net
+__global__ __launch_bounds__(256) void rct_bfs_count_kernel(BfsParams P, int32_t* ctl) {
+  const int net = blockIdx.x * blockDim.x + threadIdx.x;
+  if (net >= P.num_nets) return;
+  const int n = P.nodes_start[net + 1] - P.nodes_start[net];
+  if (n <= kSmallNodes) return;
+  const int c = bfs_class(n);
+  atomicAdd(&ctl[c], 1);
+  if (c == kClasses - 1) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(ctl + 16), (unsigned long long)n);
+    atomicMax(&ctl[18], n);
+  }
+}
+
+struct BfsLists {
+  int32_t* list;                // all large nets, class by class
+  int32_t first[kClasses];      // start of each class's list
+};
+
+__global__ __launch_bounds__(256) void rct_bfs_list_kernel(BfsParams P, int32_t* ctl, BfsLists L) {
+  const int net = blockIdx.x * blockDim.x + threadIdx.x;
+  if (net >= P.num_nets) return;
+  const int n = P.nodes_start[net + 1] - P.nodes_start[net];
+  if (n <= kSmallNodes) return;
+  const int c = bfs_class(n);
+  L.list[L.first[c] + atomicAdd(&ctl[5 + c], 1)] = net;
+}
+
+template <int CAP>
+__global__ __launch_bounds__(64) void rct_bfs_lds_kernel(BfsParams P, const int32_t* list,
+                                                         int32_t count, int32_t* head) {
+  __shared__ BfsLds16<CAP> S;
+  const int lane = threadIdx.x;
+  for (;;) {
+    int idx = 0;
+    if (lane == 0) idx = atomicAdd(head, 1);
+    idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
+    if (idx >= count) break;
+    bfs_net16<CAP>(P, list[idx], S, lane);
+  }
+}
+
+// Nets larger than kMaxLds: the global-scratch path, one scratch region of
+// 6*maxn+2 ints per persistent wavefront.
+__global__ __launch_bounds__(64) void rct_bfs_global_kernel(BfsParams P, const int32_t* list,
+                                                            int32_t count, int32_t* head,
+                                                            int64_t region) {
+  __shared__ int32_t lds[6 * 512 + 2];
+  const int lane = threadIdx.x;
+  int32_t* mine = P.gscratch + region * blockIdx.x;
+  for (;;) {
+    int idx = 0;
+    if (lane == 0) idx = atomicAdd(head, 1);
+    idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
+    if (idx >= count) break;
+    bfs_net<true>(P, list[idx], mine, lds, 6 * 512 + 2, lane);
+  }
+}
+
+__global__ void rct_bfs_zero_kernel(int32_t* p, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) p[j] = 0;
+}
+
+template <int CAP>
+void* bfs_lds_ptr() {
+  return reinterpret_cast<void*>(&rct_bfs_lds_kernel<CAP>);
 }
 
 }  // namespace
@@ -446,22 +733,78 @@ otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t
   P.pres = pres;
   P.cap_bfs = reinterpret_cast<float4*>(cap_bfs);
   P.bad = d_bad;
-  int32_t* gs = nullptr;
-  OTG_HIP(ctx, hipMallocAsync((void**)&gs,
-                              sizeof(int32_t) * (6LL * in->total_num_nodes + 2LL * in->num_nets + 8),
-                              s));
-  P.gscratch = gs;
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&rct_bfs_kernel),
-                                                   64, 0) != hipSuccess || per_cu <= 0)
-    per_cu = 8;
-  const int grid = std::max(1, std::min(in->num_nets, ctx->num_cus * per_cu));
-  const int sgrid = std::max(1, std::min((in->num_nets + kSmallRange - 1) / kSmallRange,
-                                         ctx->num_cus * 16));
+  const int N = in->num_nets;
+  const int ncu = std::max(1, ctx->num_cus);
+  // small nets: packed into wavefronts, independent of the large-net work
+  const int sgrid = std::max(1, std::min((N + kSmallRange - 1) / kSmallRange, ncu * 16));
   hipLaunchKernelGGL(rct_bfs_small_kernel, dim3(sgrid), dim3(64), 0, s, P);
-  hipLaunchKernelGGL(rct_bfs_kernel, dim3(grid), dim3(64), 0, s, P);
-  hipError_t le = hipGetLastError();
-  OTG_HIP(ctx, hipFreeAsync(gs, s));
+  OTG_HIP(ctx, hipGetLastError());
+  // large nets: count by class, list, then one launch per class
+  int32_t* ctl = nullptr;
+  OTG_HIP(ctx, hipMallocAsync((void**)&ctl, sizeof(int32_t) * 32, s));
+  hipLaunchKernelGGL(rct_bfs_zero_kernel, dim3(1), dim3(64), 0, s, ctl, 32);
+  hipLaunchKernelGGL(rct_bfs_count_kernel, dim3((N + 255) / 256), dim3(256), 0, s, P, ctl);
+  int32_t h[32] = {};
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, ctl, sizeof(h), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFreeAsync(ctl, s);
+    OTG_HIP(ctx, e);
+  }
+  int64_t total = 0;
+  BfsLists L{};
+  for (int c = 0; c < kClasses; ++c) {
+    L.first[c] = (int32_t)total;
+    total += h[c];
+  }
+  int64_t nodes4 = 0;
+  std::memcpy(&nodes4, h + 16, sizeof(nodes4));
+  const int maxn4 = h[18];
+  int32_t* list = nullptr;
+  int32_t* gs = nullptr;
+  hipError_t le = hipSuccess;
+  if (total > 0) {
+    le = hipMallocAsync((void**)&list, sizeof(int32_t) * total, s);
+    L.list = list;
+    if (le == hipSuccess)
+      hipLaunchKernelGGL(rct_bfs_list_kernel, dim3((N + 255) / 256), dim3(256), 0, s, P, ctl, L);
+    // largest class first
+    for (int c = kClasses - 2; c >= 0 && le == hipSuccess; --c) {
+      if (h[c] <= 0) continue;
+      void* fn = nullptr;
+      switch (kClassCap[c]) {
+        case 1024: fn = bfs_lds_ptr<1024>(); break;
+        case 2048: fn = bfs_lds_ptr<2048>(); break;
+        case 4096: fn = bfs_lds_ptr<4096>(); break;
+        default: fn = bfs_lds_ptr<kMaxLds>(); break;
+      }
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+      const int grid = std::max(1, std::min(h[c], per_cu * ncu));
+      const int32_t* lc = list + L.first[c];
+      int32_t cnt = h[c];
+      int32_t* head = ctl + 10 + c;
+      void* args[] = {&P, &lc, &cnt, &head};
+      le = hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, s);
+    }
+    const int c4 = kClasses - 1;
+    if (h[c4] > 0 && le == hipSuccess) {
+      const int grid = std::max(1, std::min(h[c4], ncu * 2));
+      const int64_t region = 6LL * maxn4 + 8;
+      le = hipMallocAsync((void**)&gs, sizeof(int32_t) * region * grid, s);
+      P.gscratch = gs;
+      if (le == hipSuccess)
+        hipLaunchKernelGGL(rct_bfs_global_kernel, dim3(grid), dim3(64), 0, s, P,
+                           list + L.first[c4], h[c4], ctl + 10 + c4, region);
+    }
+    (void)nodes4;
+  }
+  if (le == hipSuccess) le = hipGetLastError();
+  if (gs) (void)hipFreeAsync(gs, s);
+  if (list) (void)hipFreeAsync(list, s);
+  (void)hipFreeAsync(ctl, s);
   OTG_HIP(ctx, le);
   return OTG_OK;
 }

@@ -104,6 +104,38 @@ def test_not_a_tree_rejected_in_each_bfs_path(ctx):
         assert f"net {k}" in str(ei.value), (pick, k, str(ei.value))
 
 
+@pytest.mark.parametrize("lo,hi", [(65, 1024), (1025, 2048), (2049, 4096), (4097, 8192),
+                                   (8193, 12000)])
+def test_not_a_tree_rejected_in_each_size_class(ctx, lo, hi):
+    # one broken net (a duplicated edge, or a cycle with a node cut off) in
+    # each size class of the large-net kernels, among valid nets of all sizes
+    cfg = W.RcConfig("bfsclass", 120, [W.SizeClass(0.5, 2, 60), W.SizeClass(0.5, lo, hi, N.SHAPE_CHAIN, 0.1)],
+                     seed=lo)
+    for mistake in ("duplicate", "cycle"):
+        f = W.generate_rc(cfg)
+        sizes = np.diff(f.nodes_start)
+        k = int(np.flatnonzero(sizes >= lo)[1])
+        e0 = int(f.nodes_start[k] - k)
+        bad = f.edges.copy()
+        if mistake == "duplicate":
+            bad[2 * e0], bad[2 * e0 + 1] = bad[2 * e0 + 2], bad[2 * e0 + 3]
+        else:
+            # the net's edge 0 re-attached to one of its own endpoint's other neighbours
+            ed = bad[2 * e0: 2 * (e0 + sizes[k] - 1)].reshape(-1, 2)
+            a = ed[0, 0]
+            other = [int(t if s == a else s) for s, t in ed[1:] if a in (s, t)]
+            if not other:
+                a = ed[0, 1]
+                other = [int(t if s == a else s) for s, t in ed[1:] if a in (s, t)]
+            nb = [int(t if s == other[0] else s) for s, t in ed[1:] if other[0] in (s, t) and a not in (s, t)]
+            ed[0] = (a, nb[0])  # a - other[0] - nb[0] - a: a cycle; ed[0]'s far side cut off
+        f.edges = bad
+        with pytest.raises(N.OtgError) as ei:
+            RcSolver.from_edges(f, ctx)
+        assert ei.value.status == 2
+        assert f"net {k}" in str(ei.value), (lo, mistake, k, str(ei.value))
+
+
 def test_plain_bitexact_vs_reference_arithmetic(cfg0):
     f, s = cfg0
     h = s.host_topology()
