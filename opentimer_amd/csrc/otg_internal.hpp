@@ -125,6 +125,12 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
                                   int32_t count, hipStream_t s);
 int otg_rct_blocks_per_cu(otg_context* ctx, const otg_rct_tuning& t, bool comp);
 
+// otg_plan.hip: the plan's device arrays (groups, schedules, bins, per-net
+// data) from the topology, for a plan whose tuning, method, group cap and
+// bins are set.
+otg_status otg_rct_plan_build_device(otg_context* ctx, otg_rct_plan* p, const otg_rct_tree* topo,
+                                     hipStream_t s);
+
 // otg_bfs.hip
 otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t* node2bfs,
                               int32_t* pid, int32_t* edges_directed, float* pres,

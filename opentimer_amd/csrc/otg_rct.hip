@@ -32,7 +32,6 @@
 
 #include <algorithm>
 #include <cstring>
-#include <thread>
 #include <vector>
 
 #include "otg_internal.hpp"
@@ -1965,40 +1964,6 @@ __global__ __launch_bounds__(64) void rct_group_kernel(RctParams P) {
   if constexpr (GROUP_QUEUE) queue_release(P.queue, lane);
 }
 
-// Per-net validation, largest parent offset and chain flag (plan building).
-// A net is a chain net when at least half of its 64-position windows are
-// paths (every position's parent is the position before it, the first one
-// excepted).
-__global__ void rct_netinfo_kernel(int32_t num_nets, const int32_t* __restrict__ nodes_start,
-                                   const int32_t* __restrict__ pid, int32_t* __restrict__ maxpid,
-                                   int32_t* __restrict__ chain, int32_t* __restrict__ bad) {
-  const int net = blockIdx.x * blockDim.x + threadIdx.x;
-  if (net >= num_nets) return;
-  const int32_t st = nodes_start[net], ed = nodes_start[net + 1];
-  int32_t m = 0, paths = 0, wins = 0;
-  bool ok = ed >= st, path = true;
-  for (int32_t i = st + 1; i < ed; ++i) {
-    const int32_t q = pid[i];
-    if (q < 1 || q > i - st) ok = false;
-    m = q > m ? q : m;
-    const int32_t o = (i - st) & (kWave - 1);
-    if (o == 0) {  // a window ends before i
-      paths += path;
-      ++wins;
-      path = true;
-    } else if (q != 1) {
-      path = false;
-    }
-  }
-  if (ed > st) {
-    paths += path;
-    ++wins;
-  }
-  maxpid[net] = m;
-  chain[net] = 2 * paths >= wins && wins > 1;
-  if (!ok) atomicMin(bad, net);
-}
-
 // Group capacity (positions) of a plan: 128 or 256, from small_target.
 inline int group_cap(int small_target) { return small_target <= 128 ? 128 : 256; }
 
@@ -2103,49 +2068,6 @@ int blocks_per_cu(void* fn) {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess || nb <= 0) nb = 1;
   return nb;
-}
-
-// Static schedule of one packed group (positions [pb, pe) of the host
-// arrays): nodes sorted by (depth, parent, position); one word per schedule
-// index (node | depth << 8 | first child << 16 | children << 24).
-// Returns the group's number of depth steps per sweep (its cost estimate).
-int build_group_schedule(const int32_t* hpid, const int32_t* hstart, const RctItem& g,
-                         uint32_t* out) {
-  const int n = g.pos_end - g.pos_begin;
-  int depth[256], par[256], order[256], cf[256], cn[256];
-  int next_root = g.net_begin;
-  for (int q = 0; q < n; ++q) {
-    const int pos = g.pos_begin + q;
-    if (next_root < g.net_end && hstart[next_root] == pos) {
-      par[q] = -1;
-      depth[q] = 0;
-      ++next_root;
-    } else {
-      par[q] = q - hpid[pos];
-      depth[q] = depth[par[q]] + 1;
-    }
-    order[q] = q;
-    cf[q] = 0;
-    cn[q] = 0;
-  }
-  std::sort(order, order + n, [&](int x, int y) {
-    if (depth[x] != depth[y]) return depth[x] < depth[y];
-    if (par[x] != par[y]) return par[x] < par[y];
-    return x < y;
-  });
-  for (int j = 0; j < n; ++j) {
-    const int p = par[order[j]];
-    if (p >= 0 && cn[p]++ == 0) cf[p] = j;
-  }
-  int steps = 0;
-  for (int j = 0; j < n; ++j) {
-    const int q = order[j];
-    out[j] = (uint32_t)q | ((uint32_t)depth[q] << 8) | ((uint32_t)cf[q] << 16) |
-             ((uint32_t)cn[q] << 24);
-    // a step takes up to 64 schedule entries of one depth
-    if (j == 0 || depth[q] != depth[order[j - 1]] || (j & 63) == 0) ++steps;
-  }
-  return steps;
 }
 
 }  // namespace
@@ -2388,8 +2310,6 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   p->tuning = otg_rct_resolve_tuning(ctx, tuning, method);
   p->num_nets = topo->num_nets;
   p->total_nodes = topo->total_num_nodes;
-  const int N = topo->num_nets;
-  const int64_t V = topo->total_num_nodes;
   const int W = p->tuning.ring_slots;
   const int G = group_cap(p->tuning.small_target);
   p->group_cap = G;
@@ -2406,49 +2326,9 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
       return fail(e_ == hipErrorOutOfMemory ? OTG_ERR_OUT_OF_MEMORY : OTG_ERR_DEVICE);        \
     }                                                                                         \
   } while (0)
-  std::vector<int32_t> hstart(N + 1, 0), hmax(N, 0), hchain(N, 0);
-  if (N > 0) {
-    PLAN_HIP(hipMalloc(&p->d_net_maxpid, sizeof(int32_t) * N));
-    PLAN_HIP(hipMalloc(&p->d_net_chain, sizeof(int32_t) * N));
-    PLAN_HIP(hipMalloc(&p->d_net_scratch, sizeof(int64_t) * N));
-    int32_t* d_bad = nullptr;
-    PLAN_HIP(hipMalloc(&d_bad, sizeof(int32_t)));
-    const int32_t big = INT32_MAX;
-    PLAN_HIP(hipMemcpyAsync(d_bad, &big, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(rct_netinfo_kernel, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, N,
-                       topo->nodes_start, topo->pid, p->d_net_maxpid, p->d_net_chain, d_bad);
-    PLAN_HIP(hipGetLastError());
-    int32_t bad = 0;
-    PLAN_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    PLAN_HIP(hipMemcpyAsync(hstart.data(), topo->nodes_start, sizeof(int32_t) * (N + 1),
-                            hipMemcpyDeviceToHost, ctx->stream));
-    PLAN_HIP(hipMemcpyAsync(hmax.data(), p->d_net_maxpid, sizeof(int32_t) * N,
-                            hipMemcpyDeviceToHost, ctx->stream));
-    PLAN_HIP(hipMemcpyAsync(hchain.data(), p->d_net_chain, sizeof(int32_t) * N,
-                            hipMemcpyDeviceToHost, ctx->stream));
-    PLAN_HIP(hipStreamSynchronize(ctx->stream));
-    (void)hipFree(d_bad);
-    if (bad != INT32_MAX) {
-      otg_set_error(ctx, "net %d: pid must satisfy 1 <= pid[i] <= i - nodes_start[net]", bad);
-      return fail(OTG_ERR_BAD_TOPOLOGY);
-    }
-    if (hstart[0] != 0 || hstart[N] != topo->total_num_nodes) {
-      otg_set_error(ctx, "nodes_start must run from 0 to total_num_nodes");
-      return fail(OTG_ERR_INVALID_ARGUMENT);
-    }
-  }
-  // ---- binning --------------------------------------------------------------
-  // Nets of up to G nodes (G = 128 or 256) go into packed groups: consecutive
-  // nets, at most small_target positions and G/2 nets per group (a larger
-  // net alone).  Larger nets are streamed with the smallest ring (256, 512
-  // or 1024 slots, from min_ring up to W) that holds their parent offsets
-  // (maxpid <= ring - 64), and take the global-window path when none does.
-  // Streamed nets get 4 floats of scratch per node (their L), global-window
-  // nets 8 (16 compensated).  Streamed nets are queued largest first; packed
-  // groups are ordered by their number of depth steps, most first.
+  // ---- bins: per ring size a bin of ordinary nets and (compensated) one
+  // of chain nets, which take the kernel variant with the chain-window scans
   int nb = 0;
-  // per ring size a bin of ordinary nets and (compensated) one of chain
-  // nets, which take the kernel variant with the chain-window scans
   for (int k = 0; k < kRingSteps; ++k) {
     const int ring = kRingLadder[k];
     if (ring < p->tuning.min_ring || ring > W) continue;
@@ -2459,146 +2339,11 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
     }
   }
   p->num_bins = nb;
-  std::vector<RctItem> bin_list[otg_rct_plan::kMaxBins], groups, gitems;
-  std::vector<int64_t> net_scratch(N, -1);
-  std::vector<int32_t> net_group(N, -1);
-  int64_t scratch = 0, sched_words = 0;
-  const int64_t slot_floats = comp ? 16 : 8;
-  int64_t n_packed_nets = 0, n_stream = 0, n_global = 0, maxn = 0;
-  RctItem cur{};
-  bool open = false;
-  auto flush = [&]() {
-    if (open) {
-      cur.k = (int32_t)groups.size();
-      cur.scratch = sched_words;
-      sched_words += cur.pos_end - cur.pos_begin;
-      for (int k = cur.net_begin; k < cur.net_end; ++k) net_group[k] = cur.k;
-      groups.push_back(cur);
-    }
-    open = false;
-  };
-  for (int k = 0; k < N; ++k) {
-    const int32_t n = hstart[k + 1] - hstart[k];
-    maxn = std::max<int64_t>(maxn, n);
-    if (n <= 0) {
-      flush();
-      continue;
-    }
-    if (n <= G) {
-      ++n_packed_nets;
-      if (open && (cur.pos_end - cur.pos_begin) + n <= p->tuning.small_target &&
-          cur.net_end - cur.net_begin < G / 2) {
-        cur.pos_end += n;
-        cur.net_end = k + 1;
-      } else {
-        flush();
-        cur = RctItem{hstart[k], hstart[k + 1], k, k + 1, ITEM_PACKED, 0, 0};
-        open = true;
-      }
-      continue;
-    }
-    flush();
-    RctItem it{hstart[k], hstart[k + 1], k, k + 1, ITEM_STREAM, 1, scratch};
-    int bin = 0;
-    const int want_chain = comp && hchain[k];
-    while (bin < nb && (hmax[k] > p->bin_ring[bin] - kWave || p->bin_chain[bin] != want_chain))
-      ++bin;
-    if (bin < nb) {
-      scratch += 4LL * n;
-      ++n_stream;
-      bin_list[bin].push_back(it);
-    } else {
-      it.kind = ITEM_GLOBAL;
-      scratch += slot_floats * n;
-      ++n_global;
-      gitems.push_back(it);
-    }
-    net_scratch[k] = it.scratch;
+  // ---- work list, schedules and bins, built on the device ----
+  {
+    const otg_status st = otg_rct_plan_build_device(ctx, p, topo, ctx->stream);
+    if (st != OTG_OK) return fail(st);
   }
-  flush();
-  auto larger = [](const RctItem& a, const RctItem& b) {
-    return (a.pos_end - a.pos_begin) > (b.pos_end - b.pos_begin);
-  };
-  int64_t n_items = 0;
-  for (int b = 0; b < nb; ++b) {
-    std::stable_sort(bin_list[b].begin(), bin_list[b].end(), larger);
-    p->bin_items[b] = (int32_t)bin_list[b].size();
-    n_items += p->bin_items[b];
-  }
-  std::stable_sort(gitems.begin(), gitems.end(), larger);
-  p->num_gitems = (int32_t)gitems.size();
-  p->num_groups = (int32_t)groups.size();
-  p->stats[0] = p->num_groups;
-  p->stats[1] = n_packed_nets;
-  p->stats[2] = n_stream;
-  p->stats[3] = n_global;
-  p->stats[4] = maxn;
-  p->stats[5] = n_items + p->num_gitems + p->num_groups;
-  p->stats[6] = scratch;
-  p->stats[7] = W;
-  // ---- group schedules (host, from the parent offsets) ---------------------------
-  if (!groups.empty()) {
-    std::vector<int32_t> hpid(V);
-    PLAN_HIP(hipMemcpy(hpid.data(), topo->pid, sizeof(int32_t) * V, hipMemcpyDeviceToHost));
-    std::vector<uint32_t> sched(sched_words);
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const size_t ng = groups.size();
-    std::vector<int32_t> steps(ng);
-    std::vector<std::thread> pool;
-    for (unsigned t = 0; t < hw; ++t)
-      pool.emplace_back([&, t]() {
-        for (size_t g = t; g < ng; g += hw)
-          steps[g] = build_group_schedule(hpid.data(), hstart.data(), groups[g],
-                                          sched.data() + groups[g].scratch);
-      });
-    for (auto& th : pool) th.join();
-    // costliest groups first (the kernel deals them out in that order); a
-    // group's index k follows it
-    {
-      std::vector<int32_t> order(ng);
-      for (size_t g = 0; g < ng; ++g) order[g] = (int32_t)g;
-      std::stable_sort(order.begin(), order.end(),
-                       [&](int32_t a, int32_t b) { return steps[a] > steps[b]; });
-      std::vector<RctItem> sorted(ng);
-      for (size_t i = 0; i < ng; ++i) {
-        sorted[i] = groups[order[i]];
-        sorted[i].k = (int32_t)i;
-        for (int k = sorted[i].net_begin; k < sorted[i].net_end; ++k) net_group[k] = (int32_t)i;
-      }
-      groups.swap(sorted);
-    }
-    PLAN_HIP(hipMalloc(&p->d_sched, sizeof(uint32_t) * sched_words));
-    PLAN_HIP(hipMemcpy(p->d_sched, sched.data(), sizeof(uint32_t) * sched_words,
-                       hipMemcpyHostToDevice));
-    PLAN_HIP(hipMalloc(&p->d_groups, sizeof(RctItem) * ng));
-    PLAN_HIP(hipMemcpy(p->d_groups, groups.data(), sizeof(RctItem) * ng, hipMemcpyHostToDevice));
-    PLAN_HIP(hipMalloc(&p->d_sub_groups, sizeof(RctItem) * ng));
-    PLAN_HIP(hipMalloc(&p->d_group_flag, sizeof(int32_t) * ng));
-    PLAN_HIP(hipMemset(p->d_group_flag, 0, sizeof(int32_t) * ng));
-    PLAN_HIP(hipMalloc(&p->d_sub_count, sizeof(int32_t) * 2));
-    PLAN_HIP(hipMalloc(&p->d_net_group, sizeof(int32_t) * N));
-    PLAN_HIP(hipMemcpy(p->d_net_group, net_group.data(), sizeof(int32_t) * N,
-                       hipMemcpyHostToDevice));
-  }
-  for (int b = 0; b < nb; ++b) {
-    const size_t m = bin_list[b].size();
-    if (!m) continue;
-    PLAN_HIP(hipMalloc(&p->d_bin_items[b], sizeof(RctItem) * m));
-    PLAN_HIP(hipMemcpyAsync(p->d_bin_items[b], bin_list[b].data(), sizeof(RctItem) * m,
-                            hipMemcpyHostToDevice, ctx->stream));
-    PLAN_HIP(hipMalloc(&p->d_bin_sub[b], sizeof(RctItem) * m));
-  }
-  if (!gitems.empty()) {
-    PLAN_HIP(hipMalloc(&p->d_gitems, sizeof(RctItem) * gitems.size()));
-    PLAN_HIP(hipMemcpyAsync(p->d_gitems, gitems.data(), sizeof(RctItem) * gitems.size(),
-                            hipMemcpyHostToDevice, ctx->stream));
-  }
-  if (N > 0) {
-    PLAN_HIP(hipMemcpyAsync(p->d_net_scratch, net_scratch.data(), sizeof(int64_t) * N,
-                            hipMemcpyHostToDevice, ctx->stream));
-  }
-  p->scratch_floats = scratch;
-  if (scratch > 0) PLAN_HIP(hipMalloc(&p->d_scratch, sizeof(float) * scratch));
   PLAN_HIP(hipMalloc(&p->d_queue, sizeof(int32_t) * otg_rct_plan::kQueueInts));
   PLAN_HIP(hipMemsetAsync(p->d_queue, 0, sizeof(int32_t) * otg_rct_plan::kQueueInts, ctx->stream));
   PLAN_HIP(hipStreamSynchronize(ctx->stream));
