@@ -44,8 +44,13 @@ def parse():
     ap.add_argument("--no-fprop", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--tuning", default=os.path.join(ROOT, "opentimer_amd", "tuning.json"))
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="solve with the default tuning instead of autotuning each method first")
+    ap.add_argument("--tuning", default="",
+                    help="a tuning file (otg_rct_tuning_load) used for both methods instead of "
+                         "autotuning")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the host-pointer drop-in leg (otg_rct_bfs_and_compute at cfg1)")
     ap.add_argument("--only", default="",
                     help="rct | plain | levelize | fprop | copy (profiling runs) | precision (cfg3 report) "
                          "| incremental (cfg4 report) | spef (SPEF loader + GPU solve)")
@@ -61,6 +66,59 @@ def parse():
     return ap.parse_args()
 
 
+def kernel_sources_hash():
+    """sha256 over the HIP sources of the P1 solve (what a traffic
+    measurement depends on)."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("otg_rct.hip", "otg_plan.hip", "otg_internal.hpp"):
+        with open(os.path.join(ROOT, "opentimer_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def run_dropin(ctx, forest, method):
+    """The host-pointer drop-in (otg_rct_bfs_and_compute, what the bridge's
+    rct_bfs_and_compute_cuda calls) on the whole cfg1 forest: one call from
+    pageable host arrays, end to end, with its stage breakdown
+    (otg_rct_last_breakdown), after one warm-up call."""
+    import ctypes as C
+
+    from opentimer_amd import native as N
+    V, Nn = forest.num_nodes, forest.num_nets
+    out = {k: np.empty(4 * V, np.float32) for k in ("load", "delay", "ldelay", "impulse")}
+    n2b = np.empty(V, np.int32)
+    pid = np.empty(V, np.int32)
+    m = {"batched": N.BATCHED, "compensated": N.BATCHED_COMPENSATED}[method]
+    res = {}
+    for rep in range(2):
+        edges = forest.edges.copy()
+        a = N.RctEdgeArray(num_nets=Nn, total_num_nodes=V, total_num_edges=V - Nn,
+                           rct_nodes_start=N.addr(forest.nodes_start), rct_edges=N.addr(edges),
+                           rct_roots=N.addr(forest.roots), rct_edges_res=N.addr(forest.edge_res),
+                           rct_nodes_cap=N.addr(forest.node_cap), rct_node2bfs_order=N.addr(n2b),
+                           rct_pid=N.addr(pid), load=N.addr(out["load"]),
+                           delay=N.addr(out["delay"]), ldelay=N.addr(out["ldelay"]),
+                           impulse=N.addr(out["impulse"]))
+        t0 = time.perf_counter()
+        ctx.check(N.lib.otg_rct_bfs_and_compute(ctx.ptr, C.byref(a), m))
+        wall = (time.perf_counter() - t0) * 1e3
+        b = (C.c_double * 8)()
+        ctx.check(N.lib.otg_rct_last_breakdown(ctx.ptr, b))
+        names = ("alloc", "h2d", "bfs", "plan", "solve", "d2h", "free", "total")
+        res = {"ms": wall, "breakdown_ms": {k: float(b[i]) for i, k in enumerate(names)}}
+    h2d = 4 * (Nn + 1) + 8 * (V - Nn) + 4 * Nn + 4 * (V - Nn) + 16 * V
+    d2h = 8 * (V - Nn) + 8 * V + 4 * 16 * V
+    res.update({"method": method, "nets": Nn, "nodes": V, "nets_per_s": Nn / (res["ms"] / 1e3),
+                "h2d_bytes": h2d, "d2h_bytes": d2h,
+                "h2d_GBps": h2d / (res["breakdown_ms"]["h2d"] / 1e3) / 1e9,
+                "d2h_GBps": d2h / (res["breakdown_ms"]["d2h"] / 1e3) / 1e9,
+                "what": "otg_rct_bfs_and_compute from pageable host arrays: allocate, copy in, "
+                        "BFS-order, plan, solve, copy out, free (second of two calls)"})
+    log(f"drop-in: {res}")
+    return res
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -70,6 +128,16 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def host_threads():
+    """Threads for the CPU baselines: the host cores this process may run on,
+    capped by OMP_NUM_THREADS when it is set (the share of the host's cores
+    that goes with one GPU on a shared box)."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = min(aff, int(env)) if env.isdigit() and int(env) > 0 else aff
+    return n, aff
 
 
 def cpu_baseline(forest_host, topo, budget_s):
@@ -84,7 +152,7 @@ def cpu_baseline(forest_host, topo, budget_s):
 
     ns = topo["nodes_start"]
     pid, pres, cap, si = topo["pid"], topo["pres"], topo["cap"], topo["slew_in"]
-    cores = min(16, len(os.sched_getaffinity(0)))
+    cores, aff = host_threads()
     nnets = len(ns) - 1
 
     def run(a, b):
@@ -104,7 +172,7 @@ def cpu_baseline(forest_host, topo, budget_s):
                 break
     med = float(np.median(times))
     return {"value": nnets / med, "unit": "nets/s", "cores": cores, "kind": "port",
-            "cpu": cpu_model(),
+            "cpu": cpu_model(), "affinity_cores": aff,
             "sample": f"the whole workload ({nnets} nets, {int(ns[-1])} nodes), "
                       f"{len(times)} passes in {sum(times):.1f}s, median pass {med:.3f}s; "
                       f"oracle FP32 DFS + slew, {cores} threads"}
@@ -142,7 +210,7 @@ def run_levelize(ctx, stream, sh, reps=5):
     lev["levels_bitexact_vs_oracle"] = bool(np.array_equal(lv, olv) and np.array_equal(ends, oends))
     # CPU baselines: the sink-first Kahn serially (the run above) and over the
     # host cores (median of 3 runs, levels checked against the serial run)
-    cores = min(16, len(os.sched_getaffinity(0)))
+    cores, aff = host_threads()
     par = []
     for _ in range(3):
         t0 = time.perf_counter()
@@ -151,7 +219,7 @@ def run_levelize(ctx, stream, sh, reps=5):
         par.append((time.perf_counter() - t0) * 1e3)
     par_ms = float(np.median(par))
     lev["cpu_baseline"] = {"serial_ms": serial_ms, "parallel_ms": par_ms, "threads": cores,
-                           "cpu": cpu_model(),
+                           "affinity_cores": aff, "cpu": cpu_model(),
                            "parallel_levels_equal": bool(np.array_equal(plv, olv)
                                                          and np.array_equal(pends, oends)),
                            "gpu_speedup_vs_parallel": par_ms / lms}
@@ -618,9 +686,10 @@ def run_fprop(ctx, stream, sh, reps=10, cpu=True):
         gk = ("fanin_start", "fanin", "arc_kind", "arc_timing", "arc_rc_delay", "arc_rc_impulse",
               "pin_load", "pi_slew", "pi_at")
         lk = ("lut_hdr", "lut_val", "timing_sense", "timing_edge", "timing_lut", "timing_has")
+        cores, aff = host_threads()
         t0 = time.perf_counter()
         os_, oa, od = O.fprop({k: getattr(g, k) for k in gk}, {k: getattr(lib, k) for k in lk},
-                              fr, ends)
+                              fr, ends, nthreads=cores)
         cms = (time.perf_counter() - t0) * 1e3
 
         def same(a, b):
@@ -633,9 +702,10 @@ def run_fprop(ctx, stream, sh, reps=10, cpu=True):
         out["bitexact_vs_oracle"] = {"slew": same(slew, os_), "at": same(at, oa),
                                      "arc_delay": same(ad, od), "rat": same(rat, orat),
                                      "slack": same(slack, oslack)}
-        out["cpu_baseline"] = {"value": n / (cms / 1e3), "unit": "pins/s", "cores": 1,
-                               "kind": "port", "cpu": cpu_model(), "ms": cms,
-                               "sample": "the whole graph, one propagation"}
+        out["cpu_baseline"] = {"value": n / (cms / 1e3), "unit": "pins/s", "cores": cores,
+                               "affinity_cores": aff, "kind": "port", "cpu": cpu_model(),
+                               "ms": cms, "sample": "the whole graph, one propagation, the pins "
+                                                    "of each level split over the threads"}
     log(f"fprop: {out}")
     return out
 
@@ -658,8 +728,7 @@ def main():
     stream = torch.cuda.current_stream(dev)
     sh = stream.cuda_stream
     ctx = N.Context(0)
-    if os.path.exists(args.tuning) and not args.autotune:
-        import ctypes as C
+    if args.tuning:
         ctx.check(N.lib.otg_rct_tuning_load(ctx.ptr, args.tuning.encode(), None))
 
     if args.only == "levelize":
@@ -720,34 +789,55 @@ def main():
     del edges_d, roots_d, res_d, ncap_d
     log(f"bfs ordering: {bfs_ms:.2f} ms")
 
-    if args.autotune:
-        s.alloc_outputs()
-        best = N.RctTuning()
-        ctx.check(N.lib.otg_rct_autotune(ctx.ptr, C.byref(s.tree()), C.byref(s.result()),
-                                         N.BATCHED if args.method == "batched" else
-                                         N.BATCHED_COMPENSATED, args.tuning.encode(),
-                                         C.byref(best)))
-        log(f"autotuned: ring {best.ring_slots} min ring {best.min_ring} "
-            f"group target {best.small_target} grid/cu {best.grid_per_cu}")
-
+    import ctypes as C
+    # ---- tuning: each batched method autotuned on this forest (its own
+    # coordinate-descent search, otg_rct_autotune), unless a file is given
     other = "batched" if args.method == "compensated" else "compensated"
+    tunings = {}
+    tuning_info = {}
+    s.alloc_outputs()
+    tdir = os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) \
+        else "/tmp"
+    # (searched on a context of its own: the solver's context keeps the
+    # built-in defaults, and each plan gets its method's tuning explicitly)
+    tctx = N.Context(0)
+    for m in (args.method, other):
+        if args.tuning or args.no_autotune:
+            tunings[m] = None
+            tuning_info[m] = {"source": args.tuning or "defaults"}
+            continue
+        best = N.RctTuning()
+        path = os.path.join(tdir, f"tuning_{m}.json")
+        t0 = time.perf_counter()
+        tctx.check(N.lib.otg_rct_autotune(tctx.ptr, C.byref(s.tree()), C.byref(s.result()),
+                                         N.BATCHED if m == "batched" else N.BATCHED_COMPENSATED,
+                                         path.encode(), C.byref(best)))
+        tunings[m] = tuple(getattr(best, k) for k, _ in N.RctTuning._fields_)
+        rec = json.load(open(path))
+        tuning_info[m] = {"source": "autotuned on this forest", "ms_search": (time.perf_counter() - t0) * 1e3,
+                          "candidates": rec.get("candidates"),
+                          **{k: getattr(best, k) for k, _ in N.RctTuning._fields_}}
+        log(f"autotuned {m}: {tuning_info[m]}")
+    tctx.close()
+
     plans = {}
     for m in (args.method, other):
         t0 = time.perf_counter()
-        s.plan(m)
+        s.plan(m, tunings[m])
         torch.cuda.synchronize()
-        plans[m] = {"build_ms": (time.perf_counter() - t0) * 1e3, "stats": s.plan_stats(m)}
+        plans[m] = {"build_ms": (time.perf_counter() - t0) * 1e3, "stats": s.plan_stats(m, tunings[m])}
         log(f"plan {m}: {plans[m]['build_ms']:.1f} ms {plans[m]['stats']}")
     s.alloc_outputs()
 
     def timed(method, steps, warmup):
+        tun = tunings.get(method)
         for _ in range(warmup):
-            s.solve(method, stream=sh)
+            s.solve(method, stream=sh, tuning=tun)
         torch.cuda.synchronize()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
         for _ in range(steps):
-            s.solve(method, stream=sh)
+            s.solve(method, stream=sh, tuning=tun)
         b.record(stream)
         torch.cuda.synchronize()
         return a.elapsed_time(b) / max(steps, 1)
@@ -789,7 +879,7 @@ def main():
     idx_t = torch.from_numpy(idx).to(dev)
     parity = {"nets_checked": int(len(pick))}
     for m in ("batched", "compensated"):
-        s.solve(m, stream=sh)
+        s.solve(m, stream=sh, tuning=tunings.get(m))
         torch.cuda.synchronize()
         errs = {}
         for k in ("load", "delay", "ldelay", "beta", "impulse", "slew"):
@@ -815,13 +905,24 @@ def main():
         fprop = run_fprop(ctx, stream, sh, cpu=not args.no_cpu)
 
     achieved = alg_bytes / (ms / 1e3) / 1e9
-    traffic = None
+    # HBM traffic per step from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # summarised in profiles/, used only if it was measured on the kernel
+    # sources built here (their hash is stamped in the file)
+    traffic, traffic_note = None, "no traffic measurement in profiles/"
     tfile = os.path.join(ROOT, "profiles", f"rct_{args.method}_cfg1_traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+            rec = json.load(open(tfile))
+            if rec.get("sources_sha256") == kernel_sources_hash():
+                traffic = rec.get("hbm_bytes_per_launch")
+                traffic_note = f"rocprofv3 PMC, measured at commit {rec.get('commit')}"
+            else:
+                traffic_note = "the traffic in profiles/ was measured on other kernel sources"
         except Exception:
             traffic = None
+    dropin = None
+    if not args.no_dropin:
+        dropin = run_dropin(ctx, forest, args.method)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(forest, topo, args.cpu_seconds)
@@ -843,12 +944,14 @@ def main():
                    "nets": Nn, "nodes": V, "method": args.method,
                    "precision": ("fp32 with two-sum compensated subtree/path sums, fp64 final "
                                  "per-node combination") if args.method == "compensated" else "fp32",
-                   "plan": {k: int(v) for k, v in plans[args.method]["stats"].items()}},
+                   "plan": {k: int(v) for k, v in plans[args.method]["stats"].items()},
+                   "tuning": tuning_info[args.method]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "algorithmic_bytes": alg_bytes},
+                     "traffic_source": traffic_note, "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
         other: {"ms_per_step": ms_other, "nets_per_s": Nn / (ms_other / 1e3),
+                "tuning": tuning_info[other],
                 "achieved_GBps": alg_bytes / (ms_other / 1e3) / 1e9,
                 "frac": alg_bytes / (ms_other / 1e3) / 1e9 / PEAK_HBM_GBS},
         "plain": None if plain_ms is None else {"ms_per_step": plain_ms,
@@ -856,6 +959,7 @@ def main():
                                                 "speedup_of_" + args.method: plain_ms / ms,
                                                 "speedup_of_" + other: plain_ms / ms_other},
         "bfs_ordering_ms": bfs_ms,
+        "dropin": dropin,
         "plan_build_ms": {m: plans[m]["build_ms"] for m in plans},
         "levelize": lev,
         "fprop": fprop,

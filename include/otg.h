@@ -128,6 +128,18 @@ typedef struct otg_rct_tuning {
                           /* joined back, so that one launch's last, largest*/
                           /* nets overlap the others' work; 1 under graph   */
                           /* capture                                        */
+  int32_t small_max;      /* largest net solved in a packed group, 2..group */
+                          /* capacity (default: the capacity, 128 or 256    */
+                          /* from small_target); larger nets are streamed   */
+  int32_t ring_mask;      /* ring sizes that may hold nets: bit k allows the */
+                          /* k-th of 256, 320, 512, 1024, 2048 (0 = all)     */
+  int32_t first_on_caller;/* 1: the largest-ring launch goes to the caller's */
+                          /* stream, which is already running and starts    */
+                          /* first; 0 (default): the smallest-ring launch   */
+  int32_t small_grid_per_cu; /* workgroups per CU of the smallest-ring      */
+                          /* launch when several are spread over streams (0: */
+                          /* as many as fit), leaving room on every CU for  */
+                          /* the larger rings from the start                */
 } otg_rct_tuning;
 
 /* Build the work list of a topology (net sizes and parent offsets only):
@@ -268,6 +280,14 @@ typedef struct otg_rct_host {
 } otg_rct_host;
 
 otg_status otg_rct_compute(otg_context* ctx, const otg_rct_host* data, otg_rct_method method);
+
+/* Wall-clock milliseconds of the stages of the last otg_rct_bfs_and_compute
+ * or otg_rct_compute call on this context: [0] device allocation, [1] host
+ * to device copies, [2] BFS ordering (0 for otg_rct_compute), [3] plan
+ * build, [4] solve, [5] device to host copies, [6] release, [7] the whole
+ * call.  Each stage ends with a stream synchronisation, as the reference's
+ * stages do (ot/cuda/rct.cu:362-396, timed there by its _prof timers). */
+otg_status otg_rct_last_breakdown(const otg_context* ctx, double ms[8]);
 
 /* Autotuner: times candidate tunings of the batched launch on `tree`,
  * keeps the fastest in the context (used by later plans) and, when

@@ -279,6 +279,54 @@ struct DevBufs {
 };
 }  // namespace
 
+namespace {
+// Stage clock of the host-pointer entry points: each mark synchronises the
+// stream and records the milliseconds since the previous mark.
+struct StageClock {
+  otg_context* ctx;
+  hipStream_t s;
+  std::chrono::steady_clock::time_point t0, last;
+  explicit StageClock(otg_context* c, hipStream_t st) : ctx(c), s(st) {
+    for (double& v : ctx->last_breakdown) v = 0.0;
+    t0 = last = std::chrono::steady_clock::now();
+  }
+  hipError_t mark(int stage) {
+    const hipError_t e = hipStreamSynchronize(s);
+    const auto now = std::chrono::steady_clock::now();
+    ctx->last_breakdown[stage] += std::chrono::duration<double, std::milli>(now - last).count();
+    ctx->last_breakdown[7] = std::chrono::duration<double, std::milli>(now - t0).count();
+    last = now;
+    return e;
+  }
+};
+
+// Solve on device arrays with a plan built for this call (none for the
+// plain method), timing plan and solve separately.
+otg_status solve_timed(otg_context* ctx, const otg_rct_tree* tree, const otg_rct_result* out,
+                       otg_rct_method method, hipStream_t s, StageClock& clk) {
+  otg_rct_plan* plan = nullptr;
+  if (method != OTG_RCT_PLAIN) {
+    otg_status st = otg_rct_plan_create(ctx, tree, nullptr, method, &plan);
+    if (st != OTG_OK) return st;
+  }
+  OTG_HIP(ctx, clk.mark(3));
+  otg_status st = otg_rct_solve(ctx, plan, tree, out, method, s);
+  const hipError_t e = clk.mark(4);
+  otg_rct_plan_destroy(plan);
+  if (st == OTG_OK && e != hipSuccess) {
+    OTG_SET_ERR(ctx, "solve failed: %s", hipGetErrorString(e));
+    st = OTG_ERR_DEVICE;
+  }
+  return st;
+}
+}  // namespace
+
+extern "C" otg_status otg_rct_last_breakdown(const otg_context* ctx, double ms[8]) {
+  if (!ctx || !ms) return OTG_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 8; ++i) ms[i] = ctx->last_breakdown[i];
+  return OTG_OK;
+}
+
 extern "C" otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_array* d,
                                               otg_rct_method method) {
   if (!ctx || !d || d->num_nets < 0 || d->total_num_nodes < 0) return OTG_ERR_INVALID_ARGUMENT;
@@ -293,51 +341,59 @@ extern "C" otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_arr
     return OTG_ERR_INVALID_ARGUMENT;
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
+  StageClock clk(ctx, s);
   const int64_t N = d->num_nets, V = d->total_num_nodes, E = d->total_num_edges;
-  DevBufs B;
-  int32_t *ns, *ed, *rt, *n2b, *pid;
-  float *res, *ncap, *pres, *cap, *si = nullptr;
-  float *load, *delay, *ldelay, *beta, *imp, *slew = nullptr;
-  OTG_HIP(ctx, B.alloc(&ns, N + 1));
-  OTG_HIP(ctx, B.alloc(&ed, 2 * E));
-  OTG_HIP(ctx, B.alloc(&rt, N));
-  OTG_HIP(ctx, B.alloc(&res, E));
-  OTG_HIP(ctx, B.alloc(&ncap, 4 * V));
-  OTG_HIP(ctx, B.alloc(&n2b, V));
-  OTG_HIP(ctx, B.alloc(&pid, V));
-  OTG_HIP(ctx, B.alloc(&pres, V));
-  OTG_HIP(ctx, B.alloc(&cap, 4 * V));
-  OTG_HIP(ctx, B.alloc(&load, 4 * V));
-  OTG_HIP(ctx, B.alloc(&delay, 4 * V));
-  OTG_HIP(ctx, B.alloc(&ldelay, 4 * V));
-  OTG_HIP(ctx, B.alloc(&beta, 4 * V));
-  OTG_HIP(ctx, B.alloc(&imp, 4 * V));
-  if (d->slew) OTG_HIP(ctx, B.alloc(&slew, 4 * V));
-  if (d->slew_in) OTG_HIP(ctx, B.alloc(&si, 4 * N));
-  OTG_HIP(ctx, hipMemcpyAsync(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, s));
-  OTG_HIP(ctx, hipMemcpyAsync(ed, d->rct_edges, sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice, s));
-  OTG_HIP(ctx, hipMemcpyAsync(rt, d->rct_roots, sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
-  OTG_HIP(ctx, hipMemcpyAsync(res, d->rct_edges_res, sizeof(float) * E, hipMemcpyHostToDevice, s));
-  OTG_HIP(ctx, hipMemcpyAsync(ncap, d->rct_nodes_cap, sizeof(float) * 4 * V, hipMemcpyHostToDevice, s));
-  if (si) OTG_HIP(ctx, hipMemcpyAsync(si, d->slew_in, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s));
-  otg_rct_edges in{d->num_nets, d->total_num_nodes, ns, ed, rt, res, ncap};
-  otg_status st = otg_rct_bfs(ctx, &in, n2b, pid, ed, pres, cap, s);
-  if (st != OTG_OK) return st;
-  otg_rct_tree tree{d->num_nets, d->total_num_nodes, ns, pid, pres, cap, si};
-  otg_rct_result out{load, delay, ldelay, beta, imp, slew};
-  st = otg_rct_solve(ctx, nullptr, &tree, &out, method, s);
-  if (st != OTG_OK) return st;
-  OTG_HIP(ctx, hipMemcpyAsync(d->rct_edges, ed, sizeof(int32_t) * 2 * E, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(d->rct_node2bfs_order, n2b, sizeof(int32_t) * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(d->rct_pid, pid, sizeof(int32_t) * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(d->load, load, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(d->delay, delay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(d->ldelay, ldelay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(d->impulse, imp, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  if (d->beta) OTG_HIP(ctx, hipMemcpyAsync(d->beta, beta, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  if (d->slew) OTG_HIP(ctx, hipMemcpyAsync(d->slew, slew, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipStreamSynchronize(s));
-  return OTG_OK;
+  otg_status st = OTG_OK;
+  {
+    DevBufs B;
+    int32_t *ns, *ed, *rt, *n2b, *pid;
+    float *res, *ncap, *pres, *cap, *si = nullptr;
+    float *load, *delay, *ldelay, *beta, *imp, *slew = nullptr;
+    OTG_HIP(ctx, B.alloc(&ns, N + 1));
+    OTG_HIP(ctx, B.alloc(&ed, 2 * E));
+    OTG_HIP(ctx, B.alloc(&rt, N));
+    OTG_HIP(ctx, B.alloc(&res, E));
+    OTG_HIP(ctx, B.alloc(&ncap, 4 * V));
+    OTG_HIP(ctx, B.alloc(&n2b, V));
+    OTG_HIP(ctx, B.alloc(&pid, V));
+    OTG_HIP(ctx, B.alloc(&pres, V));
+    OTG_HIP(ctx, B.alloc(&cap, 4 * V));
+    OTG_HIP(ctx, B.alloc(&load, 4 * V));
+    OTG_HIP(ctx, B.alloc(&delay, 4 * V));
+    OTG_HIP(ctx, B.alloc(&ldelay, 4 * V));
+    OTG_HIP(ctx, B.alloc(&beta, 4 * V));
+    OTG_HIP(ctx, B.alloc(&imp, 4 * V));
+    if (d->slew) OTG_HIP(ctx, B.alloc(&slew, 4 * V));
+    if (d->slew_in) OTG_HIP(ctx, B.alloc(&si, 4 * N));
+    OTG_HIP(ctx, clk.mark(0));
+    OTG_HIP(ctx, hipMemcpyAsync(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, hipMemcpyAsync(ed, d->rct_edges, sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, hipMemcpyAsync(rt, d->rct_roots, sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, hipMemcpyAsync(res, d->rct_edges_res, sizeof(float) * E, hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, hipMemcpyAsync(ncap, d->rct_nodes_cap, sizeof(float) * 4 * V, hipMemcpyHostToDevice, s));
+    if (si) OTG_HIP(ctx, hipMemcpyAsync(si, d->slew_in, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, clk.mark(1));
+    otg_rct_edges in{d->num_nets, d->total_num_nodes, ns, ed, rt, res, ncap};
+    st = otg_rct_bfs(ctx, &in, n2b, pid, ed, pres, cap, s);
+    if (st != OTG_OK) return st;
+    OTG_HIP(ctx, clk.mark(2));
+    otg_rct_tree tree{d->num_nets, d->total_num_nodes, ns, pid, pres, cap, si};
+    otg_rct_result out{load, delay, ldelay, beta, imp, slew};
+    st = solve_timed(ctx, &tree, &out, method, s, clk);
+    if (st != OTG_OK) return st;
+    OTG_HIP(ctx, hipMemcpyAsync(d->rct_edges, ed, sizeof(int32_t) * 2 * E, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipMemcpyAsync(d->rct_node2bfs_order, n2b, sizeof(int32_t) * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipMemcpyAsync(d->rct_pid, pid, sizeof(int32_t) * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipMemcpyAsync(d->load, load, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipMemcpyAsync(d->delay, delay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipMemcpyAsync(d->ldelay, ldelay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipMemcpyAsync(d->impulse, imp, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    if (d->beta) OTG_HIP(ctx, hipMemcpyAsync(d->beta, beta, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    if (d->slew) OTG_HIP(ctx, hipMemcpyAsync(d->slew, slew, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, clk.mark(5));
+  }  // device buffers released here
+  OTG_HIP(ctx, clk.mark(6));
+  return st;
 }
 
 extern "C" otg_status otg_rct_compute(otg_context* ctx, const otg_rct_host* d,
@@ -349,39 +405,46 @@ extern "C" otg_status otg_rct_compute(otg_context* ctx, const otg_rct_host* d,
     return OTG_ERR_INVALID_ARGUMENT;
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
+  StageClock clk(ctx, s);
   const int64_t N = d->num_nets, V = d->total_num_nodes;
-  DevBufs B;
-  int32_t *ns, *pid;
-  float *pres, *cap, *si = nullptr;
-  float *load, *delay, *ldelay, *beta, *imp, *slew = nullptr;
-  OTG_HIP(ctx, B.alloc(&ns, N + 1));
-  OTG_HIP(ctx, B.alloc(&pid, V));
-  OTG_HIP(ctx, B.alloc(&pres, V));
-  OTG_HIP(ctx, B.alloc(&cap, 4 * V));
-  OTG_HIP(ctx, B.alloc(&load, 4 * V));
-  OTG_HIP(ctx, B.alloc(&delay, 4 * V));
-  OTG_HIP(ctx, B.alloc(&ldelay, 4 * V));
-  OTG_HIP(ctx, B.alloc(&beta, 4 * V));
-  OTG_HIP(ctx, B.alloc(&imp, 4 * V));
-  if (d->slew) OTG_HIP(ctx, B.alloc(&slew, 4 * V));
-  if (d->slew_in) OTG_HIP(ctx, B.alloc(&si, 4 * N));
-  OTG_HIP(ctx, hipMemcpyAsync(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, s));
-  OTG_HIP(ctx, hipMemcpyAsync(pid, d->rct_pid, sizeof(int32_t) * V, hipMemcpyHostToDevice, s));
-  OTG_HIP(ctx, hipMemcpyAsync(pres, d->pres, sizeof(float) * V, hipMemcpyHostToDevice, s));
-  OTG_HIP(ctx, hipMemcpyAsync(cap, d->cap, sizeof(float) * 4 * V, hipMemcpyHostToDevice, s));
-  if (si) OTG_HIP(ctx, hipMemcpyAsync(si, d->slew_in, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s));
-  otg_rct_tree tree{d->num_nets, d->total_num_nodes, ns, pid, pres, cap, si};
-  otg_rct_result out{load, delay, ldelay, beta, imp, slew};
-  otg_status st = otg_rct_solve(ctx, nullptr, &tree, &out, method, s);
-  if (st != OTG_OK) return st;
-  OTG_HIP(ctx, hipMemcpyAsync(d->load, load, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(d->delay, delay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(d->ldelay, ldelay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(d->impulse, imp, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  if (d->beta) OTG_HIP(ctx, hipMemcpyAsync(d->beta, beta, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  if (d->slew) OTG_HIP(ctx, hipMemcpyAsync(d->slew, slew, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipStreamSynchronize(s));
-  return OTG_OK;
+  otg_status st = OTG_OK;
+  {
+    DevBufs B;
+    int32_t *ns, *pid;
+    float *pres, *cap, *si = nullptr;
+    float *load, *delay, *ldelay, *beta, *imp, *slew = nullptr;
+    OTG_HIP(ctx, B.alloc(&ns, N + 1));
+    OTG_HIP(ctx, B.alloc(&pid, V));
+    OTG_HIP(ctx, B.alloc(&pres, V));
+    OTG_HIP(ctx, B.alloc(&cap, 4 * V));
+    OTG_HIP(ctx, B.alloc(&load, 4 * V));
+    OTG_HIP(ctx, B.alloc(&delay, 4 * V));
+    OTG_HIP(ctx, B.alloc(&ldelay, 4 * V));
+    OTG_HIP(ctx, B.alloc(&beta, 4 * V));
+    OTG_HIP(ctx, B.alloc(&imp, 4 * V));
+    if (d->slew) OTG_HIP(ctx, B.alloc(&slew, 4 * V));
+    if (d->slew_in) OTG_HIP(ctx, B.alloc(&si, 4 * N));
+    OTG_HIP(ctx, clk.mark(0));
+    OTG_HIP(ctx, hipMemcpyAsync(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, hipMemcpyAsync(pid, d->rct_pid, sizeof(int32_t) * V, hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, hipMemcpyAsync(pres, d->pres, sizeof(float) * V, hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, hipMemcpyAsync(cap, d->cap, sizeof(float) * 4 * V, hipMemcpyHostToDevice, s));
+    if (si) OTG_HIP(ctx, hipMemcpyAsync(si, d->slew_in, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s));
+    OTG_HIP(ctx, clk.mark(1));
+    otg_rct_tree tree{d->num_nets, d->total_num_nodes, ns, pid, pres, cap, si};
+    otg_rct_result out{load, delay, ldelay, beta, imp, slew};
+    st = solve_timed(ctx, &tree, &out, method, s, clk);
+    if (st != OTG_OK) return st;
+    OTG_HIP(ctx, hipMemcpyAsync(d->load, load, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipMemcpyAsync(d->delay, delay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipMemcpyAsync(d->ldelay, ldelay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipMemcpyAsync(d->impulse, imp, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    if (d->beta) OTG_HIP(ctx, hipMemcpyAsync(d->beta, beta, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    if (d->slew) OTG_HIP(ctx, hipMemcpyAsync(d->slew, slew, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, clk.mark(5));
+  }
+  OTG_HIP(ctx, clk.mark(6));
+  return st;
 }
 
 // ---------------------------------------------------------------------------
@@ -470,13 +533,17 @@ extern "C" otg_status otg_toposort_compute(otg_context* ctx, int32_t n, int32_t 
 // ---------------------------------------------------------------------------
 // Autotuner
 // ---------------------------------------------------------------------------
-static bool write_tuning(const char* path, const otg_rct_tuning& t, otg_rct_method m, double ms) {
+static bool write_tuning(const char* path, const otg_rct_tuning& t, otg_rct_method m, double ms,
+                         int evaluated) {
   FILE* f = fopen(path, "w");
   if (!f) return false;
   fprintf(f,
           "{\"ring_slots\": %d, \"min_ring\": %d, \"small_target\": %d, "
-          "\"grid_per_cu\": %d, \"streams\": %d, \"method\": %d, \"ms\": %.6f}\n",
-          t.ring_slots, t.min_ring, t.small_target, t.grid_per_cu, t.streams, (int)m, ms);
+          "\"grid_per_cu\": %d, \"streams\": %d, \"small_max\": %d, \"ring_mask\": %d, "
+          "\"first_on_caller\": %d, \"small_grid_per_cu\": %d, \"method\": %d, "
+          "\"ms\": %.6f, \"candidates\": %d}\n",
+          t.ring_slots, t.min_ring, t.small_target, t.grid_per_cu, t.streams, t.small_max,
+          t.ring_mask, t.first_on_caller, t.small_grid_per_cu, (int)m, ms, evaluated);
   fclose(f);
   return true;
 }
@@ -508,6 +575,10 @@ extern "C" otg_status otg_rct_tuning_load(otg_context* ctx, const char* path, ot
   r.small_target = json_int(s, "small_target", 0);
   r.grid_per_cu = json_int(s, "grid_per_cu", 0);
   r.streams = json_int(s, "streams", 0);
+  r.small_max = json_int(s, "small_max", 0);
+  r.ring_mask = json_int(s, "ring_mask", 0);
+  r.first_on_caller = json_int(s, "first_on_caller", 0);
+  r.small_grid_per_cu = json_int(s, "small_grid_per_cu", 0);
   ctx->tuning = r;
   if (t) *t = r;
   return OTG_OK;
@@ -523,57 +594,90 @@ extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tre
   if (st != OTG_OK) return st;
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  std::vector<otg_rct_tuning> cands;
-  // smallest ring size (LDS per wavefront, hence occupancy, against one
-  // more launch), packed-group size, and the number of persistent
-  // workgroups per CU (0: as many as fit); rings up to 2048 slots
-  for (int lo : {256, 512, 1024})
-    for (int target : {256, 128})
-      for (int g : {0, 4}) {
-        otg_rct_tuning t{};
-        t.ring_slots = 2048;
-        t.min_ring = lo;
-        t.small_target = target;
-        t.grid_per_cu = g;
-        cands.push_back(t);
-      }
+  // zero fields of a candidate mean the built-in defaults while searching
+  // (not a tuning loaded into the context earlier)
+  const otg_rct_tuning saved = ctx->tuning;
+  ctx->tuning = otg_rct_tuning{};
   hipEvent_t e0, e1;
-  OTG_HIP(ctx, hipEventCreate(&e0));
-  OTG_HIP(ctx, hipEventCreate(&e1));
-  double best_ms = 1e30;
-  otg_rct_tuning bt{};
-  for (const auto& t : cands) {
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    ctx->tuning = saved;
+    OTG_SET_ERR(ctx, "autotune: hipEventCreate failed");
+    return OTG_ERR_DEVICE;
+  }
+  int evaluated = 0;
+  // ms of one solve with tuning t (plan built and released here); < 0: the
+  // candidate did not run
+  auto measure = [&](const otg_rct_tuning& t, otg_rct_tuning* used) -> double {
     otg_rct_plan* plan = nullptr;
-    if (otg_rct_plan_create(ctx, tree, &t, method, &plan) != OTG_OK) continue;
-    st = otg_rct_launch_batched(ctx, plan, tree, scratch_out, nullptr, 0, s);  // warm-up
-    if (st != OTG_OK) {
-      otg_rct_plan_destroy(plan);
-      continue;
+    if (otg_rct_plan_create(ctx, tree, &t, method, &plan) != OTG_OK) return -1.0;
+    double ms = -1.0;
+    if (otg_rct_launch_batched(ctx, plan, tree, scratch_out, nullptr, 0, s) == OTG_OK) {  // warm-up
+      const int reps = 3;
+      bool ok = hipEventRecord(e0, s) == hipSuccess;
+      for (int r = 0; r < reps && ok; ++r)
+        ok = otg_rct_launch_batched(ctx, plan, tree, scratch_out, nullptr, 0, s) == OTG_OK;
+      ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+      float f = 0.f;
+      if (ok && hipEventElapsedTime(&f, e0, e1) == hipSuccess) ms = f / reps;
     }
-    const int reps = 3;
-    OTG_HIP(ctx, hipEventRecord(e0, s));
-    for (int r = 0; r < reps; ++r) otg_rct_launch_batched(ctx, plan, tree, scratch_out, nullptr, 0, s);
-    OTG_HIP(ctx, hipEventRecord(e1, s));
-    OTG_HIP(ctx, hipEventSynchronize(e1));
-    float ms = 0;
-    OTG_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-    ms /= reps;
-    otg_rct_tuning used = plan->tuning;
+    if (used) *used = plan->tuning;
     otg_rct_plan_destroy(plan);
-    if (ms < best_ms) {
-      best_ms = ms;
-      bt = used;
+    ++evaluated;
+    return ms;
+  };
+  // Coordinate descent from the defaults, two passes: every knob in turn
+  // takes each of its candidate values with the others at their best so far.
+  // The knobs: the smallest ring (LDS per wavefront, hence occupancy, against
+  // one more launch), the ring ladder (320 on or off), the largest net of a
+  // packed group (the group/stream threshold), the group size, which launch
+  // takes the caller's stream, the smallest-ring launch's workgroups per CU,
+  // and the workgroups per CU of every launch (0: as many as fit).
+  otg_rct_tuning cur{};
+  cur.ring_slots = 2048;
+  otg_rct_tuning resolved{};
+  double best_ms = measure(cur, &resolved);
+  if (best_ms < 0) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->tuning = saved;
+    OTG_SET_ERR(ctx, "autotune: the default tuning did not run");
+    return OTG_ERR_DEVICE;
+  }
+  cur = resolved;
+  const int all = (1 << kRingSteps) - 1;
+  struct Knob {
+    int32_t otg_rct_tuning::*field;
+    std::vector<int32_t> values;
+  };
+  const std::vector<Knob> knobs = {
+      {&otg_rct_tuning::min_ring, {256, 320, 512}},
+      {&otg_rct_tuning::ring_mask, {all, all & ~2}},
+      {&otg_rct_tuning::small_max, {64, 128, 256}},
+      {&otg_rct_tuning::small_target, {128, 256}},
+      {&otg_rct_tuning::first_on_caller, {0, 1}},
+      {&otg_rct_tuning::small_grid_per_cu, {0, 4, 6}},
+      {&otg_rct_tuning::grid_per_cu, {0, 4}},
+  };
+  for (int pass = 0; pass < 2; ++pass) {
+    for (const Knob& k : knobs) {
+      for (int32_t v : k.values) {
+        if (cur.*k.field == v) continue;
+        otg_rct_tuning t = cur;
+        t.*k.field = v;
+        otg_rct_tuning used{};
+        const double ms = measure(t, &used);
+        if (ms >= 0 && ms < best_ms) {
+          best_ms = ms;
+          cur = used;
+        }
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (best_ms >= 1e29) {
-    OTG_SET_ERR(ctx, "autotune: no candidate ran");
-    return OTG_ERR_DEVICE;
-  }
-  ctx->tuning = bt;
-  if (best) *best = bt;
-  if (json_path && !write_tuning(json_path, bt, method, best_ms)) {
+  ctx->tuning = cur;
+  if (best) *best = cur;
+  if (json_path && !write_tuning(json_path, cur, method, best_ms, evaluated)) {
     OTG_SET_ERR(ctx, "cannot write %s", json_path);
     return OTG_ERR_INVALID_ARGUMENT;
   }

@@ -436,6 +436,7 @@ __global__ __launch_bounds__(64) void rct_bfs_small_kernel(BfsParams P) {
 // ---------------------------------------------------------------------------
 constexpr int kMaxLds = 8192;
 constexpr uint32_t kUnseen = 0xffffu;
+constexpr int kBatchLd = 8;  // edge chunks whose loads are in flight together
 
 template <int CAP>
 struct BfsLds16 {
@@ -481,13 +482,25 @@ __device__ void bfs_net16(const BfsParams& P, int net, BfsLds16<CAP>& S, int lan
     S.q[k] = 0u;
   }
   lds_sync();
-  for (int k = lane; k < ne; k += kWave) {
-    const int s = e[2 * k], t = e[2 * k + 1];
-    if (s < 0 || s >= n || t < 0 || t >= n || s == t) {
-      bad = true;
-    } else {
-      add16(S.off, s + 1, 1);
-      add16(S.off, t + 1, 1);
+  // (edges are read kBatchLd chunks at a time: one memory latency per batch)
+  for (int k0 = 0; k0 < ne; k0 += kBatchLd * kWave) {
+    int es[kBatchLd], et[kBatchLd];
+#pragma unroll
+    for (int u = 0; u < kBatchLd; ++u) {
+      const int k = k0 + u * kWave + lane;
+      es[u] = k < ne ? e[2 * k] : 0;
+      et[u] = k < ne ? e[2 * k + 1] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kBatchLd; ++u) {
+      if (k0 + u * kWave + lane >= ne) continue;
+      const int s = es[u], t = et[u];
+      if (s < 0 || s >= n || t < 0 || t >= n || s == t) {
+        bad = true;
+      } else {
+        add16(S.off, s + 1, 1);
+        add16(S.off, t + 1, 1);
+      }
     }
   }
   lds_sync();
@@ -512,11 +525,22 @@ __device__ void bfs_net16(const BfsParams& P, int net, BfsLds16<CAP>& S, int lan
   }
   lds_sync();
   // ---- 3. adjacency ----
-  for (int k = lane; k < ne; k += kWave) {
-    const int s = e[2 * k], t = e[2 * k + 1];
-    const int cs = add16(S.q, s, 1), ct = add16(S.q, t, 1);
-    st16(S.adj, ld16(S.off, s) + cs, t);
-    st16(S.adj, ld16(S.off, t) + ct, s);
+  for (int k0 = 0; k0 < ne; k0 += kBatchLd * kWave) {
+    int es[kBatchLd], et[kBatchLd];
+#pragma unroll
+    for (int u = 0; u < kBatchLd; ++u) {
+      const int k = k0 + u * kWave + lane;
+      es[u] = k < ne ? e[2 * k] : 0;
+      et[u] = k < ne ? e[2 * k + 1] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kBatchLd; ++u) {
+      if (k0 + u * kWave + lane >= ne) continue;
+      const int s = es[u], t = et[u];
+      const int cs = add16(S.q, s, 1), ct = add16(S.q, t, 1);
+      st16(S.adj, ld16(S.off, s) + cs, t);
+      st16(S.adj, ld16(S.off, t) + ct, s);
+    }
   }
   lds_sync();
   // ---- 4. BFS ----
@@ -619,20 +643,45 @@ __device__ void bfs_net16(const BfsParams& P, int net, BfsLds16<CAP>& S, int lan
   }
   int32_t* eo = P.edges_out + 2LL * (off0 - net);
   const float* res = P.edge_res + (off0 - net);
-  for (int k = lane; k < ne; k += kWave) {
-    int s = e[2 * k], t = e[2 * k + 1];
-    int ps = ld16(S.dist, s), pt = ld16(S.dist, t);
-    if (ps > pt) {
-      const int x = s; s = t; t = x;
-      const int y = ps; ps = pt; pt = y;
+  for (int k0 = 0; k0 < ne; k0 += kBatchLd * kWave) {
+    int es[kBatchLd], et[kBatchLd];
+    float er[kBatchLd];
+#pragma unroll
+    for (int u = 0; u < kBatchLd; ++u) {
+      const int k = k0 + u * kWave + lane;
+      es[u] = k < ne ? e[2 * k] : 0;
+      et[u] = k < ne ? e[2 * k + 1] : 0;
+      er[u] = k < ne ? res[k] : 0.0f;
     }
-    eo[2 * k] = s;
-    eo[2 * k + 1] = t;
-    P.pid[off0 + pt] = pt - ps;
-    P.pres[off0 + pt] = res[k];
+#pragma unroll
+    for (int u = 0; u < kBatchLd; ++u) {
+      const int k = k0 + u * kWave + lane;
+      if (k >= ne) continue;
+      int s = es[u], t = et[u];
+      int ps = ld16(S.dist, s), pt = ld16(S.dist, t);
+      if (ps > pt) {
+        const int x = s; s = t; t = x;
+        const int y = ps; ps = pt; pt = y;
+      }
+      eo[2 * k] = s;
+      eo[2 * k + 1] = t;
+      P.pid[off0 + pt] = pt - ps;
+      P.pres[off0 + pt] = er[u];
+    }
   }
-  for (int k = lane; k < n; k += kWave)
-    P.cap_bfs[(int64_t)off0 + ld16(S.dist, k)] = P.node_cap[(int64_t)off0 + k];
+  for (int k0 = 0; k0 < n; k0 += kBatchLd * kWave) {
+    float4 c[kBatchLd];
+#pragma unroll
+    for (int u = 0; u < kBatchLd; ++u) {
+      const int k = k0 + u * kWave + lane;
+      c[u] = k < n ? P.node_cap[(int64_t)off0 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < kBatchLd; ++u) {
+      const int k = k0 + u * kWave + lane;
+      if (k < n) P.cap_bfs[(int64_t)off0 + ld16(S.dist, k)] = c[u];
+    }
+  }
   lds_sync();  // the next net reuses the LDS
 }
 

@@ -16,6 +16,7 @@ struct otg_context {
   int lds_per_cu = 0;
   std::string last_error;
   otg_rct_tuning tuning{};  // defaults for new plans (autotuner may update)
+  double last_breakdown[8] = {};  // stages of the last host-pointer P1 call (ms)
   // Auxiliary streams that the streaming launches of one batched solve are
   // spread over (forked from and joined back into the caller's stream),
   // created on first use.
@@ -68,7 +69,8 @@ struct otg_rct_plan {
   RctItem* d_bin_sub[kMaxBins] = {};           // subset solves: the bin's listed nets
   int bin_grid[kMaxBins] = {};                 // persistent workgroups
   int32_t num_groups = 0;
-  int32_t group_cap = 256;         // largest net of a packed group
+  int32_t group_cap = 256;         // positions of a packed group (kernel capacity)
+  int32_t small_max = 256;         // largest net solved in a packed group (<= group_cap)
   RctItem* d_groups = nullptr;     // packed groups (k: group index, scratch: schedule offset)
   uint32_t* d_sched = nullptr;     // schedule words of the packed groups
   int32_t* d_net_group = nullptr;  // [num_nets] group of a packed net, else -1

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(64) void plan_large_info_kernel(const int32_t* __re
 // the segment's offsets.
 template <bool WRITE>
 __global__ __launch_bounds__(256) void plan_pack_kernel(int32_t N, const int32_t* __restrict__ ns,
-                                                        int32_t G, int32_t target,
+                                                        int32_t G, int32_t cap, int32_t target,
                                                         int32_t* __restrict__ seg_groups,
                                                         int64_t* __restrict__ seg_words,
                                                         const int32_t* __restrict__ goff,
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void plan_pack_kernel(int32_t N, const int32_t
         flush();
         continue;
       }
-      if (open && (cur.pos_end - cur.pos_begin) + n <= target && cur.net_end - cur.net_begin < G / 2) {
+      if (open && (cur.pos_end - cur.pos_begin) + n <= target && cur.net_end - cur.net_begin < cap / 2) {
         cur.pos_end += n;
         cur.net_end = k + 1;
       } else {
@@ -374,7 +374,8 @@ otg_status otg_rct_plan_build_device(otg_context* ctx, otg_rct_plan* p, const ot
                                      hipStream_t s) {
   const int N = topo->num_nets;
   const int64_t V = topo->total_num_nodes;
-  const int G = p->group_cap;
+  const int G = p->small_max;     // nets up to G nodes are packed
+  const int cap = p->group_cap;   // a group holds at most cap positions and cap/2 nets
   const bool comp = p->method == OTG_RCT_BATCHED_COMPENSATED;
   const int W = p->tuning.ring_slots;
   const int32_t* ns = topo->nodes_start;
@@ -405,7 +406,7 @@ otg_status otg_rct_plan_build_device(otg_context* ctx, otg_rct_plan* p, const ot
   OTG_HIP(ctx, T.alloc(&seg_groups, nseg));
   OTG_HIP(ctx, T.alloc(&seg_words, nseg));
   hipLaunchKernelGGL(plan_pack_kernel<false>, dim3(grid_for(nseg, 256)), dim3(256), 0, s, N, ns, G,
-                     p->tuning.small_target, seg_groups, seg_words, nullptr, nullptr, nullptr,
+                     cap, p->tuning.small_target, seg_groups, seg_words, nullptr, nullptr, nullptr,
                      nullptr);
   OTG_HIP(ctx, hipGetLastError());
   PlanCtl hc{};
@@ -462,7 +463,7 @@ otg_status otg_rct_plan_build_device(otg_context* ctx, otg_rct_plan* p, const ot
     OTG_HIP(ctx, hipMalloc(&p->d_group_flag, sizeof(int32_t) * ng));
     OTG_HIP(ctx, hipMalloc(&p->d_sub_count, sizeof(int32_t) * 2));
     hipLaunchKernelGGL(plan_pack_kernel<true>, dim3(grid_for(nseg, 256)), dim3(256), 0, s, N, ns,
-                       G, p->tuning.small_target, nullptr, nullptr, dgoff, dwoff, unsorted,
+                       G, cap, p->tuning.small_target, nullptr, nullptr, dgoff, dwoff, unsorted,
                        tmp_group);
     hipLaunchKernelGGL(plan_sched_kernel, dim3((int)std::min<int64_t>(ng, (int64_t)ncu * 32)),
                        dim3(64), 0, s, (int32_t)ng, unsorted, ns, pid, p->d_sched, cost);

@@ -1578,10 +1578,13 @@ __global__ __launch_bounds__(64) void rct_stream_kernel(RctParams P) {
       pph = PH_TD;
       pc = c + 1;
       pv = true;
-    } else if ((ph == PH_TD || nc == 1) && pipe.has_next()) {
+    } else if (ph == PH_TD && pipe.has_next()) {
       // the current item's last chunk: the next item's first chunk
       // (top-down chunk 1 of a longer net is requested after the bottom-up
-      // chunk 0, whose wait completes the stores it reads)
+      // chunk 0, whose wait completes the stores it reads; the bottom-up
+      // chunk 0 requests nothing, since its registers become the top-down
+      // sweep's chunk 0 and a chunk taken over there would replace them --
+      // for a net of one chunk too)
       pit = pipe.n1;
       pv = stream_first_chunk(pit, pph, pc);
     }
@@ -2102,6 +2105,16 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
                                            : (m == OTG_RCT_BATCHED_COMPENSATED ? 256 : 512);
   if (!ring_ok(r.min_ring)) r.min_ring = 256;
   r.min_ring = std::min(r.min_ring, r.ring_slots);
+  const int cap = group_cap(r.small_target);
+  if (!r.small_max) r.small_max = d.small_max ? d.small_max : cap;
+  r.small_max = std::max(2, std::min(r.small_max, cap));
+  if (!r.ring_mask) r.ring_mask = d.ring_mask ? d.ring_mask : (1 << kRingSteps) - 1;
+  r.ring_mask &= (1 << kRingSteps) - 1;
+  if (!r.ring_mask) r.ring_mask = (1 << kRingSteps) - 1;
+  if (!r.first_on_caller) r.first_on_caller = d.first_on_caller;
+  r.first_on_caller = r.first_on_caller ? 1 : 0;
+  if (!r.small_grid_per_cu) r.small_grid_per_cu = d.small_grid_per_cu;
+  r.small_grid_per_cu = std::max(0, r.small_grid_per_cu);
   return r;
 }
 
@@ -2172,7 +2185,7 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   P.scratch = plan->d_scratch;
   P.sched = plan->d_sched;
   P.long_maxpid = plan->tuning.ring_slots - kWave;
-  P.group_max = plan->group_cap;
+  P.group_max = plan->small_max;
   const int G = plan->group_cap;
   // ---- packed groups ------------------------------------------------------------
   if (plan->num_groups > 0) {
@@ -2210,7 +2223,7 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
     hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, n_sub, plan->num_bins);
     hipLaunchKernelGGL(rct_subset_items_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
                        count, subset, plan->num_nets, tree->nodes_start, plan->d_net_maxpid,
-                       plan->d_net_scratch, plan->d_net_chain, G, sb, n_sub);
+                       plan->d_net_scratch, plan->d_net_chain, plan->small_max, sb, n_sub);
     OTG_HIP(ctx, hipGetLastError());
   }
   struct Launch {
@@ -2279,10 +2292,21 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
       st[ns++] = ctx->aux[k];
     }
   }
+  // (the smallest-ring launch, L[nl-1] unless global-window nets follow,
+  // may be held to fewer workgroups per CU so that the larger rings find
+  // room from the start)
+  if (ns > 1 && plan->tuning.small_grid_per_cu > 0 && plan->num_bins > 0) {
+    for (int i = nl - 1; i >= 0; --i) {
+      if (L[i].fn == pick_global(comp)) continue;
+      L[i].grid = std::min(L[i].grid, plan->tuning.small_grid_per_cu * std::max(1, ctx->num_cus));
+      break;
+    }
+  }
   hipError_t err = hipSuccess;
   for (int i = 0; i < nl && err == hipSuccess; ++i) {
     void* args[] = {&L[i].Q};
-    err = hipLaunchKernel(L[i].fn, dim3(L[i].grid), dim3(64), args, 0, st[ns - 1 - i % ns]);
+    const int si = plan->tuning.first_on_caller ? i % ns : ns - 1 - i % ns;
+    err = hipLaunchKernel(L[i].fn, dim3(L[i].grid), dim3(64), args, 0, st[si]);
   }
   // joined on every path, so that the caller's stream orders after all of it
   for (int k = 1; k < ns; ++k) {
@@ -2313,6 +2337,7 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   const int W = p->tuning.ring_slots;
   const int G = group_cap(p->tuning.small_target);
   p->group_cap = G;
+  p->small_max = std::min(p->tuning.small_max, G);
   const bool comp = method == OTG_RCT_BATCHED_COMPENSATED;
   auto fail = [&](otg_status st) {
     otg_rct_plan_destroy(p);
@@ -2331,7 +2356,7 @@ extern "C" otg_status otg_rct_plan_create(otg_context* ctx, const otg_rct_tree* 
   int nb = 0;
   for (int k = 0; k < kRingSteps; ++k) {
     const int ring = kRingLadder[k];
-    if (ring < p->tuning.min_ring || ring > W) continue;
+    if (ring < p->tuning.min_ring || ring > W || !((p->tuning.ring_mask >> k) & 1)) continue;
     for (int ch = 0; ch <= (comp ? 1 : 0); ++ch) {
       p->bin_ring[nb] = ring;
       p->bin_chain[nb] = ch;

@@ -65,7 +65,8 @@ class RctResult(C.Structure):
 class RctTuning(C.Structure):
     _fields_ = [("ring_slots", C.c_int32), ("min_ring", C.c_int32),
                 ("small_target", C.c_int32), ("grid_per_cu", C.c_int32),
-                ("streams", C.c_int32)]
+                ("streams", C.c_int32), ("small_max", C.c_int32), ("ring_mask", C.c_int32),
+                ("first_on_caller", C.c_int32), ("small_grid_per_cu", C.c_int32)]
 
 
 class RctEdges(C.Structure):
@@ -184,6 +185,7 @@ _sig("otg_rct_solve_subset", C.c_int, ctx_p, plan_p, C.POINTER(RctTree), C.POINT
 _sig("otg_rct_bfs", C.c_int, ctx_p, C.POINTER(RctEdges), vp, vp, vp, vp, vp, vp)
 _sig("otg_rct_bfs_and_compute", C.c_int, ctx_p, C.POINTER(RctEdgeArray), C.c_int)
 _sig("otg_rct_compute", C.c_int, ctx_p, C.POINTER(RctHost), C.c_int)
+_sig("otg_rct_last_breakdown", C.c_int, ctx_p, C.POINTER(C.c_double))
 _sig("otg_rct_query", C.c_int, ctx_p, C.c_int32, vp, vp, C.POINTER(RctResult), C.c_int32, vp, vp,
      vp, vp, vp, vp)
 _sig("otg_rct_autotune", C.c_int, ctx_p, C.POINTER(RctTree), C.POINTER(RctResult), C.c_int,
@@ -239,7 +241,7 @@ EXPORTED = [
     "otg_create", "otg_destroy", "otg_synchronize", "otg_status_string", "otg_last_error",
     "otg_version", "otg_rct_plan_create", "otg_rct_plan_destroy", "otg_rct_plan_stats",
     "otg_rct_solve", "otg_rct_reference_f64", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
-    "otg_rct_compute", "otg_rct_query",
+    "otg_rct_compute", "otg_rct_query", "otg_rct_last_breakdown",
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize", "otg_levelize_noalloc",
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",
     "otg_fprop", "otg_fprop_plan_create", "otg_fprop_plan_destroy", "otg_fprop_plan_sizes",

@@ -43,6 +43,7 @@ for name, args, res in [
     ("ora_counting_argsort", [C.c_int, vp, C.c_int, C.c_int, vp, vp, vp], None),
     ("ora_lut_eval", [vp, vp, C.c_int, C.c_float, C.c_float], C.c_float),
     ("ora_fprop", [C.c_int] + [vp] * 17 + [C.c_int, vp, vp, vp], None),
+    ("ora_fprop_par", [C.c_int] + [vp] * 17 + [C.c_int, vp, vp, vp, C.c_int], None),
     ("ora_bprop", [vp] * 8 + [C.c_int, vp, vp], None),
 ]:
     f = getattr(_lib, name)
@@ -189,8 +190,10 @@ def lut_eval(hdr, pool, l, val1, val2):
     return float(_lib.ora_lut_eval(_p(h), _p(p), int(l), float(val1), float(val2)))
 
 
-def fprop(graph, lib, frontiers, ends):
+def fprop(graph, lib, frontiers, ends, nthreads=1):
     """Level-wise forward propagation (timer.cpp:733-783, arc.cpp:82-147).
+    nthreads > 1 splits every level's pins over host threads (the same
+    results bit for bit: pins of a level write disjoint values).
 
     graph: dict with fanin_start, fanin, arc_kind, arc_timing, arc_rc_delay
     [m,4], arc_rc_impulse [m,4], pin_load [n,4], pi_slew [n,4], pi_at [n,4];
@@ -212,9 +215,13 @@ def fprop(graph, lib, frontiers, ends):
     slew = np.full(max(4 * n, 1), np.nan, np.float32)
     at = np.full(max(4 * n, 1), np.nan, np.float32)
     ad = np.full(max(8 * m, 1), np.nan, np.float32)
-    _lib.ora_fprop(n, _p(fs), _p(fi), _p(kind), _p(tim), _p(rcd), _p(rci), _p(load), _p(pis),
-                   _p(pia), _p(sense), _p(edge), _p(tlut), _p(thas), _p(hdr), _p(pool), _p(fr),
-                   _p(en), len(en) - 1, _p(slew), _p(at), _p(ad))
+    args = (n, _p(fs), _p(fi), _p(kind), _p(tim), _p(rcd), _p(rci), _p(load), _p(pis), _p(pia),
+            _p(sense), _p(edge), _p(tlut), _p(thas), _p(hdr), _p(pool), _p(fr), _p(en),
+            len(en) - 1, _p(slew), _p(at), _p(ad))
+    if nthreads > 1:
+        _lib.ora_fprop_par(*args, int(nthreads))
+    else:
+        _lib.ora_fprop(*args)
     return slew[: 4 * n].reshape(n, 4), at[: 4 * n].reshape(n, 4), ad[: 8 * m].reshape(m, 8)
 
 

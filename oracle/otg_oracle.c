@@ -570,6 +570,62 @@ static float relax(float cur, float v, int el) {
 /* Timer::_fprop_slew / _fprop_delay / _fprop_at (ot/timer/timer.cpp:      */
 /* 733-783) with Arc::_fprop_* (ot/timer/arc.cpp:82-147) for every pin of   */
 /* the levels, from the last level (sources) to level 0.                    */
+/* One pin's slew, arrival and fanin-arc delays (the body of the level loop
+ * of timer.cpp:1079-1086 over _fprop_slew/_delay/_at, timer.cpp:733-783). */
+typedef struct fprop_in {
+  const int32_t *fs, *fi, *kind, *timing;
+  const float *rc_delay, *rc_imp, *load, *pi_slew, *pi_at;
+  const int32_t *sense, *edge, *tlut, *thas, *hdr;
+  const float* pool;
+  float *slew, *at, *arc_delay;
+} fprop_in;
+
+static void fprop_pin(const fprop_in* P, int v) {
+  for (int el = 0; el < 2; ++el)
+    for (int trf = 0; trf < 2; ++trf) {
+      const int c = el * 2 + trf;
+      float s = P->pi_slew[4 * v + c], a = P->pi_at[4 * v + c];
+      for (int j = P->fs[v]; j < P->fs[v + 1]; ++j) {
+        const int u = P->fi[j], k = P->kind[j];
+        for (int frf = 0; frf < 2; ++frf) {
+          const float su = P->slew[4 * u + el * 2 + frf];
+          const float au = P->at[4 * u + el * 2 + frf];
+          /* every delay of the pin's fanin arcs is reset first (timer.cpp:757-759) */
+          float so = NAN, d = NAN;
+          if (k == 3) {
+            /* loop breaker: nothing propagates */
+          } else if (k == 0 || k == 1) {
+            /* net arcs carry rf -> rf only */
+            if (frf == trf) {
+              if (k == 0) {
+                const float imp = P->rc_imp[4 * j + c];
+                if (!isnan(su)) so = su < 0.0f ? -sqrtf(su * su + imp) : sqrtf(su * su + imp);
+                d = P->rc_delay[4 * j + c];
+              } else {
+                so = su;
+                d = 0.0f;
+              }
+            }
+          } else {
+            const int t = P->timing[j];
+            if (P->thas[2 * t + el] && !isnan(su)) {
+              const float lc = P->load[4 * v + c];
+              so = timing_value(P->sense, P->edge, P->tlut, P->hdr, P->pool, t, el, frf, trf, 2, su,
+                                lc);
+              d = timing_value(P->sense, P->edge, P->tlut, P->hdr, P->pool, t, el, frf, trf, 0, su,
+                               lc);
+            }
+          }
+          s = relax(s, so, el);
+          if (P->arc_delay) P->arc_delay[8 * j + 4 * el + 2 * frf + trf] = d;
+          if (!isnan(d) && !isnan(au)) a = relax(a, d + au, el);
+        }
+      }
+      P->slew[4 * v + c] = s;
+      P->at[4 * v + c] = a;
+    }
+}
+
 void ora_fprop(int n, const int32_t* fs, const int32_t* fi, const int32_t* kind,
                const int32_t* timing, const float* rc_delay, const float* rc_imp,
                const float* load, const float* pi_slew, const float* pi_at, const int32_t* sense,
@@ -577,52 +633,72 @@ void ora_fprop(int n, const int32_t* fs, const int32_t* fi, const int32_t* kind,
                const float* pool, const int32_t* frontiers, const int32_t* ends, int num_levels,
                float* slew, float* at, float* arc_delay) {
   (void)n;
-  for (int lv = num_levels - 1; lv >= 0; --lv) {
-    for (int q = ends[lv]; q < ends[lv + 1]; ++q) {
-      const int v = frontiers[q];
-      for (int el = 0; el < 2; ++el)
-        for (int trf = 0; trf < 2; ++trf) {
-          const int c = el * 2 + trf;
-          float s = pi_slew[4 * v + c], a = pi_at[4 * v + c];
-          for (int j = fs[v]; j < fs[v + 1]; ++j) {
-            const int u = fi[j], k = kind[j];
-            for (int frf = 0; frf < 2; ++frf) {
-              const float su = slew[4 * u + el * 2 + frf];
-              const float au = at[4 * u + el * 2 + frf];
-              /* every delay of the pin's fanin arcs is reset first (timer.cpp:757-759) */
-              float so = NAN, d = NAN;
-              if (k == 3) {
-                /* loop breaker: nothing propagates */
-              } else if (k == 0 || k == 1) {
-                /* net arcs carry rf -> rf only */
-                if (frf == trf) {
-                  if (k == 0) {
-                    const float imp = rc_imp[4 * j + c];
-                    if (!isnan(su)) so = su < 0.0f ? -sqrtf(su * su + imp) : sqrtf(su * su + imp);
-                    d = rc_delay[4 * j + c];
-                  } else {
-                    so = su;
-                    d = 0.0f;
-                  }
-                }
-              } else {
-                const int t = timing[j];
-                if (thas[2 * t + el] && !isnan(su)) {
-                  const float lc = load[4 * v + c];
-                  so = timing_value(sense, edge, tlut, hdr, pool, t, el, frf, trf, 2, su, lc);
-                  d = timing_value(sense, edge, tlut, hdr, pool, t, el, frf, trf, 0, su, lc);
-                }
-              }
-              s = relax(s, so, el);
-              if (arc_delay) arc_delay[8 * j + 4 * el + 2 * frf + trf] = d;
-              if (!isnan(d) && !isnan(au)) a = relax(a, d + au, el);
-            }
-          }
-          slew[4 * v + c] = s;
-          at[4 * v + c] = a;
-        }
-    }
+  const fprop_in P = {fs, fi, kind, timing, rc_delay, rc_imp, load, pi_slew, pi_at,
+                      sense, edge, tlut, thas, hdr, pool, slew, at, arc_delay};
+  for (int lv = num_levels - 1; lv >= 0; --lv)
+    for (int q = ends[lv]; q < ends[lv + 1]; ++q) fprop_pin(&P, frontiers[q]);
+}
+
+/* The same over host threads (the propagation's CPU baseline): the pins of a
+ * level are split into contiguous shares, one per thread, with a barrier
+ * between levels (the per-level parallel_for of timer.cpp:1079-1086).  Pins
+ * of one level write only their own values and their own fanin arcs, so the
+ * results equal the serial run's bit for bit. */
+typedef struct fprop_par {
+  const fprop_in* P;
+  const int32_t *frontiers, *ends;
+  int num_levels, nthreads;
+  pthread_barrier_t bar;
+} fprop_par;
+
+typedef struct fprop_arg {
+  fprop_par* s;
+  int t;
+} fprop_arg;
+
+static void* fprop_worker(void* p) {
+  fprop_arg* a = (fprop_arg*)p;
+  fprop_par* s = a->s;
+  for (int lv = s->num_levels - 1; lv >= 0; --lv) {
+    const int b = s->ends[lv], e = s->ends[lv + 1];
+    const int64_t len = e - b;
+    const int lo = b + (int)(len * a->t / s->nthreads), hi = b + (int)(len * (a->t + 1) / s->nthreads);
+    for (int q = lo; q < hi; ++q) fprop_pin(s->P, s->frontiers[q]);
+    pthread_barrier_wait(&s->bar);
   }
+  return NULL;
+}
+
+void ora_fprop_par(int n, const int32_t* fs, const int32_t* fi, const int32_t* kind,
+                   const int32_t* timing, const float* rc_delay, const float* rc_imp,
+                   const float* load, const float* pi_slew, const float* pi_at,
+                   const int32_t* sense, const int32_t* edge, const int32_t* tlut,
+                   const int32_t* thas, const int32_t* hdr, const float* pool,
+                   const int32_t* frontiers, const int32_t* ends, int num_levels, float* slew,
+                   float* at, float* arc_delay, int nthreads) {
+  (void)n;
+  if (nthreads < 1) nthreads = 1;
+  const fprop_in P = {fs, fi, kind, timing, rc_delay, rc_imp, load, pi_slew, pi_at,
+                      sense, edge, tlut, thas, hdr, pool, slew, at, arc_delay};
+  fprop_par s;
+  s.P = &P;
+  s.frontiers = frontiers;
+  s.ends = ends;
+  s.num_levels = num_levels;
+  s.nthreads = nthreads;
+  pthread_barrier_init(&s.bar, NULL, (unsigned)nthreads);
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+  fprop_arg* args = (fprop_arg*)malloc(sizeof(fprop_arg) * nthreads);
+  for (int t = 0; t < nthreads; ++t) {
+    args[t].s = &s;
+    args[t].t = t;
+    if (t) pthread_create(&th[t], NULL, fprop_worker, &args[t]);
+  }
+  fprop_worker(&args[0]);
+  for (int t = 1; t < nthreads; ++t) pthread_join(th[t], NULL);
+  pthread_barrier_destroy(&s.bar);
+  free(th);
+  free(args);
 }
 
 /* Timer::_bprop_rat (ot/timer/timer.cpp:812-841) with Arc::_bprop_rat    */

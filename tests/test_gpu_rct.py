@@ -462,3 +462,29 @@ def test_streams_do_not_change_results(ctx, method):
         out[streams] = {k: v.clone() for k, v in s.out.items()}
     for k in OUT:
         assert torch.equal(out[1][k], out[3][k]), k
+
+
+@pytest.mark.parametrize("tun", [
+    (0, 0, 0, 0, 3, 64),             # nets of 65..256 nodes streamed instead of packed
+    (0, 0, 0, 0, 3, 0, 0b11101),     # no 320-slot ring
+    (0, 0, 0, 0, 3, 0, 0, 1),        # largest ring on the caller's stream
+    (0, 0, 0, 0, 3, 0, 0, 0, 4),     # smallest-ring launch held to 4 workgroups per CU
+    (0, 320, 128, 0, 2, 100, 0b11110, 1, 3),
+])
+@pytest.mark.parametrize("method,tol64", [("batched", 1e-5), ("compensated", 1e-6)])
+def test_tuning_knobs_keep_results(ctx, tun, method, tol64):
+    cfg = W.RcConfig("knobs", 2500, [W.SizeClass(0.55, 2, 40), W.SizeClass(0.25, 41, 256),
+                                     W.SizeClass(0.15, 257, 3000, N.SHAPE_CHAIN, 0.1),
+                                     W.SizeClass(0.05, 600, 2500, N.SHAPE_BALANCED)], seed=47)
+    f, s = _forest(cfg, ctx)
+    h = s.host_topology()
+    s.solve(method, tuning=tun)
+    st = s.plan_stats(method, tun)
+    if len(tun) > 5 and tun[5]:
+        assert st["packed_nets"] == int(np.sum((np.diff(h["nodes_start"]) >= 1)
+                                               & (np.diff(h["nodes_start"]) <= tun[5])))
+    g = s.host()
+    r64 = O.rct_all(h["nodes_start"], h["pid"], h["pres"], h["cap"], h["slew_in"], "f64")
+    fl = floors(r64)
+    for k in OUT:
+        assert rel_err(g[k], r64[k], fl[k]) < tol64, (tun, k)

@@ -411,3 +411,24 @@ def test_bprop_equals_max_min_over_enumerated_paths(seed):
                 assert (np.isnan(s) and np.isnan(slack[u, el * 2 + rf])) or \
                     np.float32(s) == slack[u, el * 2 + rf]
     assert np.isfinite(rat).sum() > 20
+
+
+def test_threaded_propagation_matches_the_serial_oracle():
+    from opentimer_amd import workloads as W
+    # ora_fprop_par (the CPU baseline over host threads) writes the same bits
+    # as the serial restatement, NaN pattern included
+    csr = W.fanin_csr(W.generate_dag(W.DagConfig("par", 20_000, 50_000, 30, seed=17)))
+    lib = W.prop_library(num_timings=16, seed=5)
+    g = W.prop_graph(csr, lib, seed=6)
+    fr, ends, _ = O.levelize(csr.n, csr.start, csr.fanin, csr.out_degree, csr.frontier)
+    gk = ("fanin_start", "fanin", "arc_kind", "arc_timing", "arc_rc_delay", "arc_rc_impulse",
+          "pin_load", "pi_slew", "pi_at")
+    lk = ("lut_hdr", "lut_val", "timing_sense", "timing_edge", "timing_lut", "timing_has")
+    G = {k: getattr(g, k) for k in gk}
+    L = {k: getattr(lib, k) for k in lk}
+    one = O.fprop(G, L, fr, ends)
+    for t in (2, 5):
+        many = O.fprop(G, L, fr, ends, nthreads=t)
+        for a, b in zip(one, many):
+            assert np.array_equal(np.isnan(a), np.isnan(b))
+            assert np.array_equal(a[~np.isnan(a)].view(np.int32), b[~np.isnan(b)].view(np.int32))
