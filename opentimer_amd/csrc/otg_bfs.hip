@@ -20,6 +20,7 @@
 // atomic accesses (no stale L1 lines) and waits for its own outstanding
 // memory operations between phases.
 #include <hip/hip_runtime.h>
+#include <rocprim/block/block_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -685,6 +686,331 @@ __device__ void bfs_net16(const BfsParams& P, int net, BfsLds16<CAP>& S, int lan
   lds_sync();  // the next net reuses the LDS
 }
 
+// ---------------------------------------------------------------------------
+// Large nets, one workgroup per net: rooting by an Euler tour.
+//
+// The wavefront above needs one dependent step per BFS level, and the
+// chain-like nets of a few thousand nodes have thousands of levels.  This
+// path finds every node's distance from the root in O(log n) dependent
+// rounds instead: the Euler-tour technique of Tarjan and Vishkin (1985),
+// with list ranking by a sparse ruling set (Reid-Miller, 1994).  Phases:
+//   1. degrees (packed 16-bit LDS atomics) with the endpoint checks;
+//   2. CSR offsets by a block scan;
+//   3. adjacency slots and, for every slot (an arc u->v), its twin slot
+//      (the arc v->u);
+//   4. the tour successor of each arc u->v: the slot after its twin in v's
+//      list, cyclically; the arc whose successor would be the root's first
+//      slot ends the tour;
+//   5. list ranking: every S-th slot (S a power of two, at most T rulers)
+//      and the tour's first arc are rulers; one thread per ruler walks to the
+//      next ruler, the rulers' list is ranked by pointer jumping
+//      (bit_width(R) + 1 rounds), and a second walk writes every arc's tour
+//      position.  The successor map is a bijection on the arcs except at the
+//      tour's end, so a walk from a ruler ends at the end, at another ruler,
+//      or back at its start; every walk is also held to A = 2(n-1) steps;
+//   6. an arc is a descent (parent -> child) iff it comes before its twin in
+//      the tour; a block scan of +1 (descent) / -1 (ascent) over the tour
+//      gives each child its distance at its descent;
+//   7. BFS positions: the keys (distance, CAP-1-id) in a block radix sort,
+//      so that the position of u is #(smaller distance) + #(equal distance,
+//      larger id), the order of the reference's bucket-end counting sort;
+//   8. oriented edges, pid, R and C in BFS order, as the wavefront path.
+// A net that is not a spanning tree leaves arcs outside the tour from the
+// root (the rulers' list from the first arc covers fewer than A arcs), or a
+// node without a distance; either rejects it.  Every early exit is decided
+// from values that all threads read alike (the net's own header, or LDS
+// after a barrier, made scalar), so the workgroup leaves a net together.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kTourEnd = 0xffffu;
+
+template <int CAP, int T>
+struct TourLds {
+  uint32_t off[CAP / 2 + 4];  // 16-bit: degrees -> offsets [n+1]; distances, then positions [n]
+  uint32_t adj[CAP];          // 16-bit [A]: neighbour at each slot      } from phase 7:
+  uint32_t tw[CAP];           // 16-bit [A]: twin slot                   } the sort's
+  uint32_t nx[CAP];           // 16-bit [A]: tour successor; +-1 scan    } storage
+  uint32_t rk[CAP];           // 16-bit: fill cursors [n]; tour positions [A]
+  int32_t rnext[T + 2];       // ruler -> next ruler (-1: the end, -2: walk bound hit)
+  int32_t rsuf[T + 2];        // arcs from the ruler to the end
+  int32_t part[T / kWave];    // block-scan wave totals
+  int32_t ctl[4];             // [0] net index  [1] failure  [2] largest distance
+};
+
+template <int T>
+__device__ __forceinline__ int block_uniform(const int32_t* p) {
+  return __builtin_amdgcn_readfirstlane(*p);
+}
+
+// Inclusive scan of L 16-bit entries of a, in place (entries read as signed
+// if SIGNED).  Wave w scans its own whole-chunk segment, after a pass that
+// sums each segment.  Starts and ends with a barrier; returns the total.
+template <int T, bool SIGNED>
+__device__ int block_scan16(uint32_t* a, int L, int32_t* part, int tid) {
+  constexpr int NW = T / kWave;
+  const int lane = tid & (kWave - 1), w = tid / kWave;
+  const int seg = ((L + NW - 1) / NW + kWave - 1) & ~(kWave - 1);
+  const int b = w * seg, e = min(L, b + seg);
+  auto get = [&](int i) {
+    const int x = ld16(a, i);
+    return SIGNED ? (int)(int16_t)x : x;
+  };
+  __syncthreads();
+  int s = 0;
+  for (int i = b + lane; i < e; i += kWave) s += get(i);
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) s += __shfl_xor(s, d);
+  if (lane == 0) part[w] = s;
+  __syncthreads();
+  int carry = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    const int p = part[k];
+    carry += k < w ? p : 0;
+    total += p;
+  }
+  for (int c = b; c < e; c += kWave) {
+    const int i = c + lane;
+    int v = i < e ? get(i) : 0;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int t = __shfl_up(v, d);
+      if (lane >= d) v += t;
+    }
+    if (i < e) st16(a, i, v + carry);
+    carry += __shfl(v, kWave - 1);
+  }
+  __syncthreads();
+  return total;
+}
+
+template <int CAP, int T>
+__device__ void tour_net(const BfsParams& P, int net, TourLds<CAP, T>& S, int tid) {
+  constexpr int IPT = CAP / T;
+  constexpr int LOG = __builtin_ctz(CAP);
+  const int off0 = __builtin_amdgcn_readfirstlane(P.nodes_start[net]);
+  const int n = __builtin_amdgcn_readfirstlane(P.nodes_start[net + 1]) - off0;
+  if (n <= 1 || n > CAP) {  // not a net of this class (the lists never hold one)
+    if (tid == 0 && n > CAP) atomicMin(P.bad, net);
+    return;
+  }
+  const int ne = n - 1, A = 2 * ne;
+  const int32_t* e = P.edges_in + 2LL * (off0 - net);
+  const int root = __builtin_amdgcn_readfirstlane(P.roots[net]);
+  if (root < 0 || root >= n) {
+    if (tid == 0) atomicMin(P.bad, net);
+    return;
+  }
+  // ---- 1. degrees (entry v+1 counts v's edges) ----
+  for (int k = tid; k < n / 2 + 2; k += T) S.off[k] = 0u;
+  for (int k = tid; k < (n + 1) / 2; k += T) S.rk[k] = 0u;
+  if (tid == 0) {
+    S.ctl[1] = 0;
+    S.ctl[2] = 0;
+  }
+  __syncthreads();
+  bool bad = false;
+  for (int k = tid; k < ne; k += T) {
+    const int s = e[2 * k], t = e[2 * k + 1];
+    if (s < 0 || s >= n || t < 0 || t >= n || s == t) {
+      bad = true;
+    } else {
+      add16(S.off, s + 1, 1);
+      add16(S.off, t + 1, 1);
+    }
+  }
+  if (bad) atomicOr(&S.ctl[1], 1);
+  // ---- 2. offsets (the scan's barriers publish the failure flag too) ----
+  block_scan16<T, false>(S.off, n + 1, S.part, tid);
+  if (block_uniform<T>(&S.ctl[1])) {
+    if (tid == 0) atomicMin(P.bad, net);
+    return;
+  }
+  // ---- 3. adjacency slots and twins ----
+  for (int k = tid; k < ne; k += T) {
+    const int s = e[2 * k], t = e[2 * k + 1];
+    const int js = ld16(S.off, s) + add16(S.rk, s, 1);
+    const int jt = ld16(S.off, t) + add16(S.rk, t, 1);
+    st16(S.adj, js, t);
+    st16(S.adj, jt, s);
+    st16(S.tw, js, jt);
+    st16(S.tw, jt, js);
+  }
+  __syncthreads();
+  // ---- 4. tour successors (a root without edges is no tree's root) ----
+  const int h = __builtin_amdgcn_readfirstlane(ld16(S.off, root));
+  if (__builtin_amdgcn_readfirstlane(ld16(S.off, root + 1)) == h) {
+    if (tid == 0) atomicMin(P.bad, net);
+    return;
+  }
+  for (int j = tid; j < A; j += T) {
+    const int v = ld16(S.adj, j), t = ld16(S.tw, j);
+    const int nxt = t + 1 < ld16(S.off, v + 1) ? t + 1 : ld16(S.off, v);
+    st16(S.nx, j, nxt == h ? (int)kTourEnd : nxt);
+  }
+  __syncthreads();
+  // ---- 5. list ranking ----
+  int sh = 0;
+  while ((T << sh) < A) ++sh;
+  const int Sm = (1 << sh) - 1;
+  const int Rb = (A + Sm) >> sh;
+  const bool hr = (h & Sm) != 0;
+  const int R = Rb + (hr ? 1 : 0);  // at most T + 1
+  const int H = hr ? Rb : (h >> sh);
+  auto is_ruler = [&](int k) { return (k & Sm) == 0 || k == h; };
+  auto rid = [&](int k) { return (k & Sm) == 0 ? (k >> sh) : Rb; };
+  auto rstart = [&](int r) { return r < Rb ? (r << sh) : h; };
+  for (int r = tid; r < R; r += T) {
+    int k = ld16(S.nx, rstart(r)), cnt = 1;
+    for (int it = 0; it < A && k != (int)kTourEnd && !is_ruler(k); ++it) {
+      ++cnt;
+      k = ld16(S.nx, k);
+    }
+    S.rsuf[r] = cnt;
+    S.rnext[r] = k == (int)kTourEnd ? -1 : is_ruler(k) ? rid(k) : -2;
+  }
+  __syncthreads();
+  const int rounds = (32 - __clz(R)) + 1;
+  for (int rd = 0; rd < rounds; ++rd) {
+    int ns[2], nn[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int r = tid + u * T;
+      ns[u] = 0;
+      nn[u] = -1;
+      if (r < R) {
+        const int a = S.rnext[r];
+        ns[u] = S.rsuf[r];
+        nn[u] = a;
+        if (a >= 0) {
+          ns[u] += S.rsuf[a];
+          nn[u] = S.rnext[a];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int r = tid + u * T;
+      if (r < R) {
+        S.rsuf[r] = ns[u];
+        S.rnext[r] = nn[u];
+      }
+    }
+    __syncthreads();
+  }
+  // the tour from the root must hold every arc
+  if (block_uniform<T>(&S.rsuf[H]) != A || block_uniform<T>(&S.rnext[H]) != -1) {
+    if (tid == 0) atomicMin(P.bad, net);
+    __syncthreads();  // the next net's set-up writes what this phase read
+    return;
+  }
+  for (int r = tid; r < R; r += T) {
+    int j = rstart(r), p = A - S.rsuf[r];
+    st16(S.rk, j, p);
+    int k = ld16(S.nx, j);
+    for (int it = 0; it < A && k != (int)kTourEnd && !is_ruler(k); ++it) {
+      st16(S.rk, k, ++p);
+      k = ld16(S.nx, k);
+    }
+  }
+  __syncthreads();
+  // ---- 6. distances: +1 per descent, -1 per ascent, scanned over the tour ----
+  for (int j = tid; j < A; j += T) {
+    const int p = ld16(S.rk, j), q = ld16(S.rk, ld16(S.tw, j));
+    st16(S.nx, p, p < q ? 1 : 0xffff);
+  }
+  for (int v = tid; v < n; v += T) st16(S.off, v, (int)kUnseen);
+  block_scan16<T, true>(S.nx, A, S.part, tid);
+  for (int j = tid; j < A; j += T) {
+    const int p = ld16(S.rk, j), q = ld16(S.rk, ld16(S.tw, j));
+    if (p < q) st16(S.off, ld16(S.adj, j), ld16(S.nx, p));
+  }
+  if (tid == 0) st16(S.off, root, 0);
+  __syncthreads();
+  {
+    int mx = 0;
+    bool miss = false;
+    for (int v = tid; v < n; v += T) {
+      const int d = ld16(S.off, v);
+      if ((uint32_t)d == kUnseen) miss = true;
+      else mx = max(mx, d);
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) mx = max(mx, __shfl_xor(mx, d));
+    if ((tid & (kWave - 1)) == 0) atomicMax(&S.ctl[2], mx);
+    if (miss) atomicOr(&S.ctl[1], 1);
+  }
+  __syncthreads();
+  if (block_uniform<T>(&S.ctl[1])) {
+    if (tid == 0) atomicMin(P.bad, net);
+    __syncthreads();
+    return;
+  }
+  const int maxd = block_uniform<T>(&S.ctl[2]);
+  // ---- 7. positions: sort (distance, CAP-1-id) ----
+  {
+    using Sort = rocprim::block_radix_sort<uint32_t, T, IPT>;
+    static_assert(sizeof(typename Sort::storage_type) <= sizeof(uint32_t) * 4 * CAP,
+                  "the sort's storage must fit in the arc arrays");
+    auto& st = *reinterpret_cast<typename Sort::storage_type*>(S.adj);
+    uint32_t key[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      const int i = tid * IPT + k;
+      key[k] = i < n ? ((uint32_t)ld16(S.off, i) << LOG) | (uint32_t)(CAP - 1 - i) : 0xffffffffu;
+    }
+    const int end_bit = LOG + (32 - __clz(maxd)) + 1;
+    Sort().sort(key, st, 0, end_bit);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      const int pos = tid * IPT + k;
+      if (pos < n) {
+        const int id = CAP - 1 - (int)(key[k] & (uint32_t)(CAP - 1));
+        st16(S.off, id, pos);
+        P.node2bfs[off0 + id] = pos;
+      }
+    }
+  }
+  __syncthreads();
+  // ---- 8. oriented edges, pid, R and C in BFS order ----
+  if (tid == 0) {
+    P.pid[off0] = 0;
+    P.pres[off0] = 0.0f;
+  }
+  int32_t* eo = P.edges_out + 2LL * (off0 - net);
+  const float* res = P.edge_res + (off0 - net);
+  for (int k = tid; k < ne; k += T) {
+    int s = e[2 * k], t = e[2 * k + 1];
+    int ps = ld16(S.off, s), pt = ld16(S.off, t);
+    if (ps > pt) {
+      const int x = s; s = t; t = x;
+      const int y = ps; ps = pt; pt = y;
+    }
+    eo[2 * k] = s;
+    eo[2 * k + 1] = t;
+    P.pid[off0 + pt] = pt - ps;
+    P.pres[off0 + pt] = res[k];
+  }
+  for (int k = tid; k < n; k += T) P.cap_bfs[(int64_t)off0 + ld16(S.off, k)] = P.node_cap[(int64_t)off0 + k];
+  __syncthreads();  // the next net reuses the LDS
+}
+
+template <int CAP, int T>
+__global__ __launch_bounds__(T) void rct_bfs_tour_kernel(BfsParams P, const int32_t* list,
+                                                         int32_t count, int32_t* head) {
+  __shared__ TourLds<CAP, T> S;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) S.ctl[0] = atomicAdd(head, 1);
+    __syncthreads();
+    const int idx = block_uniform<T>(&S.ctl[0]);
+    __syncthreads();  // read before the next net's claim overwrites it
+    if (idx >= count) break;
+    tour_net<CAP, T>(P, list[idx], S, tid);
+  }
+}
+
 // Size classes of the on-chip kernel, and the class of the nets larger than
 // kMaxLds (global scratch).
 constexpr int kClasses = 5;
@@ -764,6 +1090,11 @@ void* bfs_lds_ptr() {
   return reinterpret_cast<void*>(&rct_bfs_lds_kernel<CAP>);
 }
 
+template <int CAP, int T>
+void* bfs_tour_ptr() {
+  return reinterpret_cast<void*>(&rct_bfs_tour_kernel<CAP, T>);
+}
+
 }  // namespace
 
 otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t* node2bfs,
@@ -821,22 +1152,26 @@ otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t
     // largest class first
     for (int c = kClasses - 2; c >= 0 && le == hipSuccess; --c) {
       if (h[c] <= 0) continue;
+      // nets of up to 1024 nodes: one wavefront each; larger: the Euler
+      // tour on a workgroup of CAP/8 threads
       void* fn = nullptr;
+      int threads = kWave;
       switch (kClassCap[c]) {
         case 1024: fn = bfs_lds_ptr<1024>(); break;
-        case 2048: fn = bfs_lds_ptr<2048>(); break;
-        case 4096: fn = bfs_lds_ptr<4096>(); break;
-        default: fn = bfs_lds_ptr<kMaxLds>(); break;
+        case 2048: fn = bfs_tour_ptr<2048, 256>(); threads = 256; break;
+        case 4096: fn = bfs_tour_ptr<4096, 512>(); threads = 512; break;
+        default: fn = bfs_tour_ptr<kMaxLds, 1024>(); threads = 1024; break;
       }
       int per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, 0) != hipSuccess || per_cu <= 0)
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess ||
+          per_cu <= 0)
         per_cu = 1;
       const int grid = std::max(1, std::min(h[c], per_cu * ncu));
       const int32_t* lc = list + L.first[c];
       int32_t cnt = h[c];
       int32_t* head = ctl + 10 + c;
       void* args[] = {&P, &lc, &cnt, &head};
-      le = hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, s);
+      le = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s);
     }
     const int c4 = kClasses - 1;
     if (h[c4] > 0 && le == hipSuccess) {
