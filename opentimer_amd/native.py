@@ -121,6 +121,24 @@ class BpropGraph(C.Structure):
                 ("at", vp)]
 
 
+class TimerDesign(C.Structure):
+    _fields_ = [("num_nets", C.c_int32), ("total_num_nodes", C.c_int32),
+                ("nodes_start", vp), ("edges", vp), ("roots", vp), ("edge_res", vp),
+                ("node_cap", vp), ("num_pins", C.c_int32), ("num_arcs", C.c_int32),
+                ("fanin_start", vp), ("fanin", vp), ("arc_kind", vp), ("arc_timing", vp),
+                ("arc_net", vp), ("arc_node", vp), ("pin_net", vp), ("pin_load", vp),
+                ("pi_slew", vp), ("pi_at", vp), ("po_rat", vp)]
+
+
+class TimerResults(C.Structure):
+    _fields_ = [("slew", vp), ("at", vp), ("rat", vp), ("slack", vp), ("arc_delay", vp),
+                ("arc_rc_delay", vp), ("arc_rc_impulse", vp), ("pin_load", vp),
+                ("frontiers", vp), ("frontier_ends", vp), ("num_levels", C.c_int32)]
+
+
+REPORT_SLEW, REPORT_AT, REPORT_RAT, REPORT_SLACK = 0, 1, 2, 3
+
+
 class GenSizeClass(C.Structure):
     _fields_ = [("weight", C.c_double), ("min_nodes", C.c_int32), ("max_nodes", C.c_int32),
                 ("shape", C.c_int32), ("pad", C.c_int32), ("branch_prob", C.c_double)]
@@ -213,6 +231,13 @@ _sig("otg_fprop_run", C.c_int, ctx_p, C.c_void_p, C.POINTER(PropGraph), C.POINTE
      C.POINTER(PropResult), vp)
 _sig("otg_bprop", C.c_int, ctx_p, C.POINTER(BpropGraph), vp, vp, C.c_int32, vp, vp, vp)
 _sig("otg_build_fanout_csr", C.c_int, C.c_int32, vp, vp, vp, vp, vp)
+timer_p = C.c_void_p
+_sig("otg_timer_create", C.c_int, ctx_p, C.POINTER(TimerDesign), C.POINTER(PropLib), C.c_int,
+     C.POINTER(timer_p))
+_sig("otg_timer_destroy", C.c_int, timer_p)
+_sig("otg_timer_update", C.c_int, timer_p, vp)
+_sig("otg_timer_results_get", C.c_int, timer_p, C.POINTER(TimerResults))
+_sig("otg_timer_report", C.c_int, timer_p, C.c_int, C.c_int32, vp, vp, vp)
 _sig("otg_graph_begin", C.c_int, ctx_p, vp)
 _sig("otg_graph_end", C.c_int, ctx_p, vp, C.POINTER(C.c_void_p))
 _sig("otg_graph_launch", C.c_int, ctx_p, C.c_void_p, vp)
@@ -245,7 +270,8 @@ EXPORTED = [
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize", "otg_levelize_noalloc",
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",
     "otg_fprop", "otg_fprop_plan_create", "otg_fprop_plan_destroy", "otg_fprop_plan_sizes",
-    "otg_fprop_run", "otg_bprop", "otg_build_fanout_csr", "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
+    "otg_fprop_run", "otg_bprop", "otg_build_fanout_csr", "otg_timer_create", "otg_timer_destroy",
+    "otg_timer_update", "otg_timer_results_get", "otg_timer_report", "otg_graph_begin", "otg_graph_end", "otg_graph_launch", "otg_graph_destroy",
     "otg_gen_rc_sizes", "otg_gen_rc_fill", "otg_gen_dag", "otg_build_fanin_csr",
     "otg_spef_read", "otg_spef_read_buffer", "otg_spef_last_error", "otg_spef_forest_get",
     "otg_spef_net_name", "otg_spef_node_name", "otg_spef_find_net", "otg_spef_find_node",

@@ -41,10 +41,13 @@ def spef_forest(design="c17"):
     return SpefForest.read(os.path.join(GOLD, "spef", f"{design}.spef"))
 
 
-def build(rc_solve, design="c17"):
-    """rc_solve(forest) -> (node2bfs, load[4V], delay[4V], impulse[4V]) in the
-    BFS order of every net.  Returns (graph dict, lib dict, pin names,
-    golden arrival [n,4] with NaN for pins the reference did not report)."""
+def assemble(design="c17"):
+    """The design as the timer sees it before any RC solve: the forest (with
+    the sink pins' capacitances added; the caller closes it), the library,
+    and the arcs sorted by sink, each (from, to, kind, timing, RC net or -1,
+    sink node or -1); per pin the RC net it drives (-1: none) and the load
+    of a driver of a net without parasitics (the sinks' capacitances);
+    primary-input slew and arrival; the reference's arrival times."""
     J = load(design)
     cells_e, cells_l = J["cells"]["early"], J["cells"]["late"]
     pins = list(J["inputs"]) + list(J["outputs"])
@@ -92,12 +95,6 @@ def build(rc_solve, design="c17"):
             if p != drv:
                 forest.add_pin_cap(k, loc, np.asarray(pin_cap(p), np.float32))
         rc_nets[net] = k
-    n2b, ld, dl, im = rc_solve(forest)
-    ns = forest.nodes_start
-
-    def at_node(k, loc, arr):
-        i = 4 * (ns[k] + n2b[ns[k] + loc])
-        return arr[i: i + 4]
 
     # library: one timing per (cell, propagating timing), both splits
     hdr, pool, sense, edge, tluts = [], [], [], [], []
@@ -126,31 +123,27 @@ def build(rc_solve, design="c17"):
            "timing_sense": np.asarray(sense, np.int32), "timing_edge": np.asarray(edge, np.int32),
            "timing_lut": np.asarray(tluts, np.int32), "timing_has": np.ones(2 * T, np.int32)}
 
-    arcs = []  # (from, to, kind, timing, rc delay[4], rc impulse[4])
-    nan4 = np.full(4, np.nan, np.float32)
-    zero4 = np.zeros(4, np.float32)
-    load4 = np.zeros((n, 4), np.float32)
+    arcs = []
+    pin_net = np.full(n, -1, np.int32)
+    load_norc = np.zeros((n, 4), np.float32)
     for net, members in net_pins.items():
         drv = driver(net)
         if net in rc_nets:
             k = rc_nets[net]
             for p in members:
                 if p != drv:
-                    loc = forest.find_node(k, p)
-                    arcs.append((pid[drv], pid[p], 0, 0, at_node(k, loc, dl), at_node(k, loc, im)))
-            load4[pid[drv]] = at_node(k, int(forest.roots[k]), ld)
+                    arcs.append((pid[drv], pid[p], 0, 0, k, forest.find_node(k, p)))
+            pin_net[pid[drv]] = k
         else:
             for p in members:
                 if p != drv:
-                    arcs.append((pid[drv], pid[p], 1, 0, zero4, zero4))
-            load4[pid[drv]] = np.sum([pin_cap(p) for p in members if p != drv], axis=0)
+                    arcs.append((pid[drv], pid[p], 1, 0, -1, -1))
+            load_norc[pid[drv]] = np.sum([pin_cap(p) for p in members if p != drv], axis=0)
     for inst in J["instances"]:
         c = inst["cell"]
         for t, a in enumerate(cells_e[c]["arcs"]):
             arcs.append((pid[f"{inst['name']}:{a['from']}"], pid[f"{inst['name']}:{a['to']}"], 2,
-                         toff[c] + t, nan4, nan4))
-    for p in J["inputs"]:
-        load4[pid[p]] = 0.0  # a primary input drives through no cell arc
+                         toff[c] + t, -1, -1))
     arcs.sort(key=lambda x: x[1])
     fs = np.zeros(n + 1, np.int32)
     for a in arcs:
@@ -163,20 +156,52 @@ def build(rc_solve, design="c17"):
             pis[pid[p]] = J["slew"][p]
         if p in J["at"]:
             pia[pid[p]] = J["at"][p]
-    graph = {
-        "fanin_start": fs,
-        "fanin": np.array([a[0] for a in arcs], np.int32),
-        "arc_kind": np.array([a[2] for a in arcs], np.int32),
-        "arc_timing": np.array([a[3] for a in arcs], np.int32),
-        "arc_rc_delay": np.array([a[4] for a in arcs], np.float32).reshape(-1, 4),
-        "arc_rc_impulse": np.array([a[5] for a in arcs], np.float32).reshape(-1, 4),
-        "pin_load": load4, "pi_slew": pis, "pi_at": pia,
-    }
     gold = np.full((n, 4), np.nan)
     for p, v in J["golden_at"].items():
         gold[pid[p]] = v
+    return {"pins": pins, "forest": forest, "lib": lib, "arcs": arcs, "fanin_start": fs,
+            "pin_net": pin_net, "load_norc": load_norc, "pi_slew": pis, "pi_at": pia,
+            "gold": gold, "inputs": [pid[p] for p in J["inputs"]]}
+
+
+def build(rc_solve, design="c17"):
+    """rc_solve(forest) -> (node2bfs, load[4V], delay[4V], impulse[4V]) in the
+    BFS order of every net.  Returns (graph dict, lib dict, pin names,
+    golden arrival [n,4] with NaN for pins the reference did not report)."""
+    D = assemble(design)
+    forest, arcs, pins = D["forest"], D["arcs"], D["pins"]
+    n = len(pins)
+    n2b, ld, dl, im = rc_solve(forest)
+    ns = forest.nodes_start
+
+    def at_node(k, loc, arr):
+        i = 4 * (ns[k] + n2b[ns[k] + loc])
+        return arr[i: i + 4]
+
+    nan4 = np.full(4, np.nan, np.float32)
+    zero4 = np.zeros(4, np.float32)
+    rc_d, rc_i = [], []
+    for a in arcs:
+        kind, k, loc = a[2], a[4], a[5]
+        rc_d.append(at_node(k, loc, dl) if kind == 0 else zero4 if kind == 1 else nan4)
+        rc_i.append(at_node(k, loc, im) if kind == 0 else zero4 if kind == 1 else nan4)
+    load4 = D["load_norc"].copy()
+    for p in range(n):
+        if D["pin_net"][p] >= 0:
+            load4[p] = at_node(int(D["pin_net"][p]), int(forest.roots[D["pin_net"][p]]), ld)
+    for p in D["inputs"]:
+        load4[p] = 0.0  # a primary input drives through no cell arc
+    graph = {
+        "fanin_start": D["fanin_start"],
+        "fanin": np.array([a[0] for a in arcs], np.int32),
+        "arc_kind": np.array([a[2] for a in arcs], np.int32),
+        "arc_timing": np.array([a[3] for a in arcs], np.int32),
+        "arc_rc_delay": np.array(rc_d, np.float32).reshape(-1, 4),
+        "arc_rc_impulse": np.array(rc_i, np.float32).reshape(-1, 4),
+        "pin_load": load4, "pi_slew": D["pi_slew"], "pi_at": D["pi_at"],
+    }
     forest.close()
-    return graph, lib, pins, gold
+    return graph, D["lib"], pins, D["gold"]
 
 
 def oracle_rc(forest):
