@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "otg_internal.hpp"
@@ -125,54 +126,26 @@ __global__ __launch_bounds__(256) void rct_plain_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Double-float helpers (compensated mode).  The compensated sweeps keep
-// every sum (subtree sums of load and L, path sums of delay and impulse) as
-// an unevaluated hi + lo pair built with two-sum, and form each product in
-// plain FP32: all terms are non-negative, so a product's rounding stays a
-// relative error of a few units in the last place of the sums it enters,
-// while an uncompensated sum's error would grow with the number of terms
-// (chains of 10^4 nodes).
+// Compensated mode: every sum (subtree sums of load and L, path sums of delay
+// and impulse) is kept in FP64; the inputs and the spilled per-node load and
+// L stay FP32.  gfx950 executes v_add_f64 / v_fma_f64 at the rate of
+// unpacked FP32 operations, so a compensated addition is one instruction
+// (a double-float two-sum took six to ten).  All terms are non-negative, so
+// the sums carry no cancellation, and FP64 keeps their relative error near
+// 1e-16 whatever their length or order.
+//
+// A double travels through the rings and the (hi, lo) float pairs of the
+// window code as its two 32-bit halves (hi: the upper word, lo: the lower),
+// so the FP32 method's one float per value and the compensated method's
+// pair share the code paths; converting between the halves and the double
+// is a register reinterpretation.
 // ---------------------------------------------------------------------------
-struct ff {
-  float hi, lo;
-};
-__device__ __forceinline__ ff two_sum(float a, float b) {
-  const float s = a + b;
-  const float bb = s - a;
-  return {s, (a - (s - bb)) + (b - bb)};
+__device__ __forceinline__ double as_f64(float hi, float lo) {
+  return __hiloint2double(__float_as_int(hi), __float_as_int(lo));
 }
-__device__ __forceinline__ float two_sum_err(float a, float b, float s) {
-  const float bb = s - a;
-  return (a - (s - bb)) + (b - bb);
-}
-__device__ __forceinline__ ff quick_two_sum(float a, float b) {
-  const float s = a + b;
-  return {s, b - (s - a)};
-}
-__device__ __forceinline__ ff ff_add(ff a, ff b) {
-  ff s = two_sum(a.hi, b.hi);
-  return quick_two_sum(s.hi, s.lo + (a.lo + b.lo));
-}
-__device__ __forceinline__ ff ff_add_f(ff a, float b) {
-  ff s = two_sum(a.hi, b);
-  return quick_two_sum(s.hi, s.lo + a.lo);
-}
-// a + b with the rounding error of the high parts kept in the low part and
-// no renormalisation: for the k children added into one node's accumulator
-// (the low part stays within k units in the last place of the high part, so
-// its own rounding adds a relative k^2 u^2), renormalised by the next
-// ff_add_f / quick_two_sum.
-__device__ __forceinline__ ff ff_acc(ff a, ff b) {
-  const ff s = two_sum(a.hi, b.hi);
-  return {s.hi, s.lo + (a.lo + b.lo)};
-}
-// a + b likewise unnormalised: after k such additions along a path the low
-// part is within k units in the last place of the high part and its own
-// rounding adds a relative k^2 u^2 (u = 2^-24), so a path renormalised every
-// few dozen steps stays exact to about 1e-12.
-__device__ __forceinline__ ff ff_acc_f(ff a, float b) {
-  const ff s = two_sum(a.hi, b);
-  return {s.hi, s.lo + a.lo};
+__device__ __forceinline__ void split_f64(double v, float& hi, float& lo) {
+  hi = __int_as_float(__double2hiint(v));
+  lo = __int_as_float(__double2loint(v));
 }
 
 __device__ __forceinline__ void unpack(float4 v, float (&a)[4]) {
@@ -251,101 +224,99 @@ struct RctParams {
 template <int W, bool COMP>
 struct RctLds {
   static constexpr int KP = (W < 256 ? W : 256) / 64;  // positions per lane, packed items
-  float4 ra[W];                // bottom-up: load acc (hi) | top-down: delay (hi)
-  float4 rb[W];                // bottom-up: L acc (hi)    | top-down: impulse (hi)
-  float4 ral[COMP ? W : 1];    // low parts (compensated mode)
-  float4 rbl[COMP ? W : 1];
   int32_t spid[W];             // parent offsets of a packed item, kept for the top-down sweep
   float sres[W];               // resistances of a packed item
   uint32_t rootbits[W / 32];   // driver positions of a packed group
   int32_t cnt[kWave * KP];     // per-position child counters / done flags of a window
 };
 
-// Window storage: either the LDS ring (index mod W) or a per-node region in
-// global memory (no wrap).  lo arrays are used only in compensated mode.
+// Window storage of the global-window path: a per-node region in global
+// memory (no wrap).  FP32 method: a and b hold one float4 per node.
+// Compensated method: a and b hold four doubles per node (each twice the
+// size), handed to the window code as (hi, lo) halves; children add into
+// their parent with FP64 atomics.
 template <bool GLOBAL, bool COMP>
 struct Ring {
   float4* a;
   float4* b;
-  float4* al;
-  float4* bl;
   int32_t base;
-  int32_t mask;  // W-1 for the LDS ring, -1 (no wrap) for global
+  int32_t mask;  // -1 (no wrap) for global memory
   __device__ __forceinline__ int slot(int i) const { return (i - base) & mask; }
 
-  __device__ __forceinline__ float4 lda(const float4* p, int k) const {
+  __device__ __forceinline__ static float ld32(const float* q) {
+    if constexpr (GLOBAL) return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else return *q;
+  }
+  __device__ __forceinline__ static double ld64(const double* q) {
     if constexpr (GLOBAL) {
-      const float* q = reinterpret_cast<const float*>(p + k);
-      return make_float4(__hip_atomic_load(q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                         __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                         __hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                         __hip_atomic_load(q + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      const uint64_t v = __hip_atomic_load(reinterpret_cast<const uint64_t*>(q), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      return __builtin_bit_cast(double, v);
     } else {
-      return p[k];
+      return *q;
+    }
+  }
+  __device__ __forceinline__ static void get1(const float4* p, int k, float (&x)[4], float (&xl)[4]) {
+    if constexpr (COMP) {
+      const double* q = reinterpret_cast<const double*>(p) + 4LL * k;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) split_f64(ld64(q + c), x[c], xl[c]);
+    } else {
+      const float* q = reinterpret_cast<const float*>(p + k);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        x[c] = ld32(q + c);
+        xl[c] = 0.f;
+      }
+    }
+  }
+  __device__ __forceinline__ static void put1(float4* p, int k, const float (&x)[4],
+                                              const float (&xl)[4]) {
+    if constexpr (COMP) {
+      double* q = reinterpret_cast<double*>(p) + 4LL * k;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) q[c] = as_f64(x[c], xl[c]);
+    } else {
+      p[k] = pack(x);
+    }
+  }
+  __device__ __forceinline__ static void add1(float4* p, int k, const float (&x)[4],
+                                              const float (&xl)[4]) {
+    if constexpr (COMP) {
+      double* q = reinterpret_cast<double*>(p) + 4LL * k;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if constexpr (GLOBAL) unsafeAtomicAdd(q + c, as_f64(x[c], xl[c]));
+        else atomicAdd(q + c, as_f64(x[c], xl[c]));
+      }
+    } else {
+      float* q = reinterpret_cast<float*>(p + k);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if constexpr (GLOBAL) unsafeAtomicAdd(q + c, x[c]);
+        else atomicAdd(q + c, x[c]);
+      }
     }
   }
   __device__ __forceinline__ void get(int k, float (&x)[4], float (&y)[4], float (&xl)[4],
                                       float (&yl)[4]) const {
-    unpack(lda(a, k), x);
-    unpack(lda(b, k), y);
-    if constexpr (COMP) {
-      unpack(lda(al, k), xl);
-      unpack(lda(bl, k), yl);
-    } else {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) xl[c] = yl[c] = 0.f;
-    }
+    get1(a, k, x, xl);
+    get1(b, k, y, yl);
   }
   __device__ __forceinline__ void put(int k, const float (&x)[4], const float (&y)[4],
                                       const float (&xl)[4], const float (&yl)[4]) const {
-    a[k] = pack(x);
-    b[k] = pack(y);
-    if constexpr (COMP) {
-      al[k] = pack(xl);
-      bl[k] = pack(yl);
-    }
+    put1(a, k, x, xl);
+    put1(b, k, y, yl);
   }
   __device__ __forceinline__ void zero(int k) const {
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    a[k] = z;
-    b[k] = z;
-    if constexpr (COMP) {
-      al[k] = z;
-      bl[k] = z;
-    }
+    const float z[4] = {0.f, 0.f, 0.f, 0.f};
+    put(k, z, z, z, z);
   }
-  __device__ __forceinline__ static float atomic_add_rtn(float* p, float v) {
-    if constexpr (GLOBAL) return unsafeAtomicAdd(p, v);
-    else return atomicAdd(p, v);
-  }
-  __device__ __forceinline__ static void atomic_add(float* p, float v) {
-    if constexpr (GLOBAL) unsafeAtomicAdd(p, v);
-    else atomicAdd(p, v);
-  }
-  // acc[k] += (x, xl), acc'[k] += (y, yl); compensated: the rounding error of
-  // the high-part add is recovered from the returned old value (two-sum) and
-  // added to the low part.
+  // acc[k] += (x, xl), acc'[k] += (y, yl)
   __device__ __forceinline__ void add(int k, const float (&x)[4], const float (&y)[4],
                                       const float (&xl)[4], const float (&yl)[4]) const {
-    float* pa = reinterpret_cast<float*>(a + k);
-    float* pb = reinterpret_cast<float*>(b + k);
-    if constexpr (COMP) {
-      float* pal = reinterpret_cast<float*>(al + k);
-      float* pbl = reinterpret_cast<float*>(bl + k);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const float oa = atomic_add_rtn(pa + c, x[c]);
-        atomic_add(pal + c, two_sum_err(oa, x[c], oa + x[c]) + xl[c]);
-        const float ob = atomic_add_rtn(pb + c, y[c]);
-        atomic_add(pbl + c, two_sum_err(ob, y[c], ob + y[c]) + yl[c]);
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        atomic_add(pa + c, x[c]);
-        atomic_add(pb + c, y[c]);
-      }
-    }
+    add1(a, k, x, xl);
+    add1(b, k, y, yl);
   }
   __device__ __forceinline__ void fence() const {
     if constexpr (GLOBAL) __threadfence();
@@ -487,15 +458,12 @@ __device__ __forceinline__ void bu_window(const ItemCtx<W, COMP>& X, const Ring<
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if constexpr (COMP) {
-          // compensated sums, FP32 products
-          const ff t = ff_add_f(ff{xa[k][c], xal[k][c]}, cap4[c]);
-          ld[c] = t.hi;
-          ldl[c] = t.lo;
+          // FP64 sums and products
+          const double t = as_f64(xa[k][c], xal[k][c]) + (double)cap4[c];
+          split_f64(t, ld[c], ldl[c]);
           L[c] = xb[k][c];
           Ll[c] = xbl[k][c];
-          const ff q = ff_add_f(quick_two_sum(L[c], Ll[c]), in.r[k] * t.hi * t.hi);
-          w[c] = q.hi;
-          wl[c] = q.lo;
+          split_f64(as_f64(L[c], Ll[c]) + (double)in.r[k] * t * t, w[c], wl[c]);
         } else {
           ld[c] = cap4[c] + xa[k][c];
           ldl[c] = 0.f;
@@ -530,8 +498,8 @@ __device__ __forceinline__ void bu_window(const ItemCtx<W, COMP>& X, const Ring<
       float o1[4], o2[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        o1[c] = COMP ? ld[c] + ldl[c] : ld[c];
-        o2[c] = COMP ? L[c] + Ll[c] : L[c];
+        o1[c] = COMP ? (float)as_f64(ld[c], ldl[c]) : ld[c];
+        o2[c] = COMP ? (float)as_f64(L[c], Ll[c]) : L[c];
       }
       bstore4(X.B.load, (i - X.pb) * 16, o1);
       bstore4(X.B.gl, (i - X.pb) * 16, o2);
@@ -577,7 +545,13 @@ __device__ __forceinline__ void td_window(const ItemCtx<W, COMP>& X, const Ring<
       unpack(in.ld[KEEP ? 0 : k], ld[k]);
       unpack(in.L[KEEP ? 0 : k], L[k]);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) ldl[k][c] = Ll[k][c] = 0.f;
+      for (int c = 0; c < 4; ++c) {
+        ldl[k][c] = Ll[k][c] = 0.f;
+        if constexpr (COMP) {  // FP32 values read back, as (hi, lo) halves of doubles
+          split_f64((double)ld[k][c], ld[k][c], ldl[k][c]);
+          split_f64((double)L[k][c], L[k][c], Ll[k][c]);
+        }
+      }
     }
   }
   wave_sync();
@@ -604,15 +578,11 @@ __device__ __forceinline__ void td_window(const ItemCtx<W, COMP>& X, const Ring<
         const float qd = has_par ? pd[k][c] : 0.f, qdl = has_par ? pdl[k][c] : 0.f;
         const float qm = has_par ? pm[k][c] : 0.f, qml = has_par ? pml[k][c] : 0.f;
         if constexpr (COMP) {
-          // FP32 products of the rounded load and L, compensated path sums
-          const float u = r * (ld[k][c] + ldl[k][c]);
-          const float z = u * u + 2.0f * r * (L[k][c] + Ll[k][c]);
-          const ff dd = ff_add_f(ff{qd, qdl}, u);
-          const ff mm = ff_add_f(ff{qm, qml}, z);
-          d[c] = dd.hi;
-          dl[c] = dd.lo;
-          m[c] = mm.hi;
-          ml[c] = mm.lo;
+          // FP64 products and path sums
+          const double u = (double)r * as_f64(ld[k][c], ldl[k][c]);
+          const double z = u * u + 2.0 * (double)r * as_f64(L[k][c], Ll[k][c]);
+          split_f64(as_f64(qd, qdl) + u, d[c], dl[c]);
+          split_f64(as_f64(qm, qml) + z, m[c], ml[c]);
         } else {
           const float u = r * ld[k][c];
           d[c] = qd + u;
@@ -640,18 +610,19 @@ __device__ __forceinline__ void td_window(const ItemCtx<W, COMP>& X, const Ring<
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       if constexpr (COMP) {
-        // final per-node combination in FP64
-        const double dd = (double)d[c] + (double)dl[c];
-        const double md = (double)m[c] + (double)ml[c];
-        const double ldd = (double)ld[k][c] + (double)ldl[k][c];
-        const double Ld = (double)L[k][c] + (double)Ll[k][c];
+        // final per-node combination in FP64; the slew's radicand is formed
+        // in FP64 and its square root taken in FP32, as in node_outputs
+        const double dd = as_f64(d[c], dl[c]);
+        const double md = as_f64(m[c], ml[c]);
+        const double ldd = as_f64(ld[k][c], ldl[k][c]);
+        const double Ld = as_f64(L[k][c], Ll[k][c]);
         o_ld[c] = (float)ldd;
         o_d[c] = (float)dd;
         o_m[c] = (float)md;
         o_ldel[c] = (float)(dd * ldd + Ld);
         o_b[c] = (float)(0.5 * (dd * dd + md));
-        const double q = sqrt((double)sv[c] * (double)sv[c] + md);
-        o_s[c] = (float)(sv[c] < 0.f ? -q : q);
+        const float q = sqrtf((float)((double)sv[c] * (double)sv[c] + md));
+        o_s[c] = sv[c] < 0.f ? -q : q;
       } else {
         o_ld[c] = ld[k][c];
         o_d[c] = d[c];
@@ -822,8 +793,7 @@ __global__ __launch_bounds__(64) void rct_global_kernel(RctParams P) {
     if (it.pos_end <= it.pos_begin || it.kind != ITEM_GLOBAL) continue;
     const int64_t nn = it.pos_end - it.pos_begin;
     float4* base = reinterpret_cast<float4*>(P.scratch + it.scratch);
-    Ring<true, COMP> R{base, base + nn, COMP ? base + 2 * nn : base, COMP ? base + 3 * nn : base,
-                       it.pos_begin, -1};
+    Ring<true, COMP> R{base, base + (COMP ? 2 : 1) * nn, it.pos_begin, -1};
     process_item<1, true, W, COMP, true>(P, S, R, it, lane);
   }
   queue_release(P.queue, lane);
@@ -997,7 +967,7 @@ struct SRing {
 // Chain windows (compensated mode).  When every position of a window but
 // the first has the previous position as its parent, the window is a path:
 // its in-window subtree sums are suffix sums over the lanes and its path
-// sums are prefix sums.  Both are taken as wave-wide scans of double-float
+// sums are prefix sums.  Both are taken as wave-wide scans of FP64
 // values (6 shuffle stages) instead of one depth step per position (64 for
 // a chain).  The compensated sums keep their accuracy in any summation
 // order; the FP32 method keeps the sequential order of the CPU timer.
@@ -1007,33 +977,27 @@ __device__ __forceinline__ bool chain_window(bool valid, int par, int i, int lan
   return vm != 0 && cm == vm && (vm & 1ull) && __popcll(vm) > 8;
 }
 
-__device__ __forceinline__ ff shfl_ff_down(ff v, int off) {
-  return ff{__shfl_down(v.hi, off), __shfl_down(v.lo, off)};
-}
-__device__ __forceinline__ ff shfl_ff_up(ff v, int off) {
-  return ff{__shfl_up(v.hi, off), __shfl_up(v.lo, off)};
-}
 // inclusive suffix sums over lanes (lane l: sum of lanes l..63)
-__device__ __forceinline__ void suffix_scan(ff (&v)[4], int lane) {
+__device__ __forceinline__ void suffix_scan(double (&v)[4], int lane) {
   // (stages not unrolled: the scan is inlined into every window of a chunk,
   // and unrolled stages would triple the kernel's code)
 #pragma unroll 1
   for (int off = 1; off < kWave; off <<= 1) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const ff o = shfl_ff_down(v[c], off);
-      if (lane + off < kWave) v[c] = ff_add(v[c], o);
+      const double o = __shfl_down(v[c], off);
+      if (lane + off < kWave) v[c] += o;
     }
   }
 }
 // inclusive prefix sums over lanes (lane l: sum of lanes 0..l)
-__device__ __forceinline__ void prefix_scan(ff (&v)[4], int lane) {
+__device__ __forceinline__ void prefix_scan(double (&v)[4], int lane) {
 #pragma unroll 1
   for (int off = 1; off < kWave; off <<= 1) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const ff o = shfl_ff_up(v[c], off);
-      if (lane >= off) v[c] = ff_add(v[c], o);
+      const double o = __shfl_up(v[c], off);
+      if (lane >= off) v[c] += o;
     }
   }
 }
@@ -1067,34 +1031,27 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
     // pushes (load, w) to its parent in an earlier window
     float xa[4], xb[4], xal[4], xbl[4];
     R.get(sl, xa, xb, xal, xbl);
-    ff S[4], V[4];
+    double S[4], V[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      S[c] = valid ? ff_add_f(ff{xa[c], xal[c]}, cap4[c]) : ff{0.f, 0.f};
+    for (int c = 0; c < 4; ++c) S[c] = valid ? as_f64(xa[c], xal[c]) + (double)cap4[c] : 0.0;
     suffix_scan(S, lane);
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      V[c] = valid ? ff_add_f(ff{xb[c], xbl[c]}, r * S[c].hi * S[c].hi) : ff{0.f, 0.f};
+    for (int c = 0; c < 4; ++c) V[c] = valid ? as_f64(xb[c], xbl[c]) + (double)r * S[c] * S[c] : 0.0;
     suffix_scan(V, lane);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      ff nx = shfl_ff_down(V[c], 1);  // w of the in-window child
-      if (lane == kWave - 1) nx = ff{0.f, 0.f};
-      const ff L = ff_add(nx, ff{xb[c], xbl[c]});
-      ldo[c] = S[c].hi;
-      Lo[c] = L.hi + L.lo;
+      double nx = __shfl_down(V[c], 1);  // w of the in-window child
+      if (lane == kWave - 1) nx = 0.0;
+      ldo[c] = (float)S[c];
+      Lo[c] = (float)(nx + as_f64(xb[c], xbl[c]));
     }
     if (lane == 0 && par >= 0) {
       float pa[4], pb[4], pal[4], pbl[4];
       R.get(ps, pa, pb, pal, pbl);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const ff sa = ff_acc(ff{pa[c], pal[c]}, S[c]);
-        const ff sb = ff_acc(ff{pb[c], pbl[c]}, V[c]);
-        pa[c] = sa.hi;
-        pal[c] = sa.lo;
-        pb[c] = sb.hi;
-        pbl[c] = sb.lo;
+        split_f64(as_f64(pa[c], pal[c]) + S[c], pa[c], pal[c]);
+        split_f64(as_f64(pb[c], pbl[c]) + V[c], pb[c], pbl[c]);
       }
       R.put(ps, pa, pb, pal, pbl);
     }
@@ -1117,18 +1074,13 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if constexpr (COMP) {
-          // sums compensated, products in FP32 (every term is positive, so
-          // a product's rounding stays a relative error of a few units in
-          // the last place of the sum; an uncompensated sum's error would
-          // grow with the number of terms)
-          const ff tt = ff_add_f(ff{xa[c], xal[c]}, cap4[c]);
-          ld[c] = tt.hi;
-          ldl[c] = tt.lo;
-          const ff q = ff_add_f(ff{xb[c], xbl[c]}, r * tt.hi * tt.hi);
-          w[c] = q.hi;
-          wl[c] = q.lo;
-          ldo[c] = tt.hi;
-          Lo[c] = xb[c] + xbl[c];
+          // sums and products in FP64
+          const double tt = as_f64(xa[c], xal[c]) + (double)cap4[c];
+          const double Ld = as_f64(xb[c], xbl[c]);
+          split_f64(tt, ld[c], ldl[c]);
+          split_f64(Ld + (double)r * tt * tt, w[c], wl[c]);
+          ldo[c] = (float)tt;
+          Lo[c] = (float)Ld;
         } else {
           ld[c] = cap4[c] + xa[c];
           w[c] = xb[c] + r * ld[c] * ld[c];
@@ -1150,12 +1102,8 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           if constexpr (COMP) {
-            const ff sa = ff_acc(ff{pa[c], pal[c]}, ff{ld[c], ldl[c]});
-            const ff sb = ff_acc(ff{pb[c], pbl[c]}, ff{w[c], wl[c]});
-            pa[c] = sa.hi;
-            pal[c] = sa.lo;
-            pb[c] = sb.hi;
-            pbl[c] = sb.lo;
+            split_f64(as_f64(pa[c], pal[c]) + as_f64(ld[c], ldl[c]), pa[c], pal[c]);
+            split_f64(as_f64(pb[c], pbl[c]) + as_f64(w[c], wl[c]), pb[c], pbl[c]);
           } else {
             pa[c] += ld[c];
             pb[c] += w[c];
@@ -1171,8 +1119,9 @@ __device__ __forceinline__ void bu_window(const SRing<W, COMP>& R, int ws, int p
   wave_sync();
 }
 
-// Final per-node outputs from delay/impulse (d, m, with low parts in the
-// compensated mode), the rounded load and L, and the driver slew.
+// Final per-node outputs from delay/impulse (d, m; in the compensated mode
+// with dl, ml the (hi, lo) halves of FP64 values), the rounded load and L,
+// and the driver slew.
 template <bool COMP>
 __device__ __forceinline__ void node_outputs(const float (&d)[4], const float (&dl)[4],
                                              const float (&m)[4], const float (&ml)[4],
@@ -1186,8 +1135,8 @@ __device__ __forceinline__ void node_outputs(const float (&d)[4], const float (&
   for (int c = 0; c < 4; ++c) {
     if constexpr (COMP) {
       // final per-node combination in FP64
-      const double dd = (double)d[c] + (double)dl[c];
-      const double md = (double)m[c] + (double)ml[c];
+      const double dd = as_f64(d[c], dl[c]);
+      const double md = as_f64(m[c], ml[c]);
       const double ldd = (double)ld[c];
       o_d[c] = (float)dd;
       o_m[c] = (float)md;
@@ -1210,43 +1159,43 @@ __device__ __forceinline__ void node_outputs(const float (&d)[4], const float (&
 }
 
 // Top-down increments of one node: u = R load (added to the delay) and
-// z = (R load)^2 + 2 R L (added to the impulse); double-float in the
-// compensated mode.  They depend on the node alone, so a window computes
-// them once for all its lanes before its depth steps.
+// z = (R load)^2 + 2 R L (added to the impulse); FP64 in the compensated
+// mode (the product of two floats is exact in FP64).  They depend on the
+// node alone, so a window computes them once for all its lanes before its
+// depth steps.
+template <bool COMP>
 struct TdIncr {
-  float u[4], z[4];
+  using T = typename std::conditional<COMP, double, float>::type;
+  T u[4], z[4];
 };
 
-// FP32 products in both modes (positive terms: a few units in the last
-// place); the compensated mode's compensation is in the path sums of
-// td_accum.
 template <bool COMP>
-__device__ __forceinline__ TdIncr td_incr(float rr, const float (&ld)[4], const float (&L)[4]) {
-  TdIncr t;
+__device__ __forceinline__ TdIncr<COMP> td_incr(float rr, const float (&ld)[4], const float (&L)[4]) {
+  TdIncr<COMP> t;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    t.u[c] = rr * ld[c];
-    t.z[c] = t.u[c] * t.u[c] + 2.0f * rr * L[c];
+    if constexpr (COMP) {
+      t.u[c] = (double)rr * (double)ld[c];
+      t.z[c] = t.u[c] * t.u[c] + 2.0 * (double)rr * (double)L[c];
+    } else {
+      t.u[c] = rr * ld[c];
+      t.z[c] = t.u[c] * t.u[c] + 2.0f * rr * L[c];
+    }
   }
   return t;
 }
 
-// delay = delay_parent + u, impulse = impulse_parent + z; NORM renormalises
-// the compensated sums (needed once every few dozen steps along a path).
-template <bool COMP, bool NORM>
-__device__ __forceinline__ void td_accum(const TdIncr& t, const float (&pd)[4],
+// delay = delay_parent + u, impulse = impulse_parent + z
+template <bool COMP>
+__device__ __forceinline__ void td_accum(const TdIncr<COMP>& t, const float (&pd)[4],
                                          const float (&pdl)[4], const float (&pm)[4],
                                          const float (&pml)[4], float (&d)[4], float (&dl)[4],
                                          float (&m)[4], float (&ml)[4]) {
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     if constexpr (COMP) {
-      const ff dd = NORM ? ff_add_f(ff{pd[c], pdl[c]}, t.u[c]) : ff_acc_f(ff{pd[c], pdl[c]}, t.u[c]);
-      const ff mm = NORM ? ff_add_f(ff{pm[c], pml[c]}, t.z[c]) : ff_acc_f(ff{pm[c], pml[c]}, t.z[c]);
-      d[c] = dd.hi;
-      dl[c] = dd.lo;
-      m[c] = mm.hi;
-      ml[c] = mm.lo;
+      split_f64(as_f64(pd[c], pdl[c]) + t.u[c], d[c], dl[c]);
+      split_f64(as_f64(pm[c], pml[c]) + t.z[c], m[c], ml[c]);
     } else {
       d[c] = pd[c] + t.u[c];
       m[c] = pm[c] + t.z[c];
@@ -1270,23 +1219,23 @@ __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int p
   const float rr = root ? 0.f : r;
   const int sl = R.slot(i);
   const int ps = R.slot(par);
-  const TdIncr inc = td_incr<COMP>(rr, ld, L);
+  const TdIncr<COMP> inc = td_incr<COMP>(rr, ld, L);
   if (CHAIN && COMP && chain_window(valid, par, i, lane)) {
     // delay_l = delay of lane 0's parent + sum over lanes j <= l of u_j
     // (impulse likewise with z)
-    ff D[4], M[4];
+    double D[4], M[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      D[c] = valid ? ff{inc.u[c], 0.f} : ff{0.f, 0.f};
-      M[c] = valid ? ff{inc.z[c], 0.f} : ff{0.f, 0.f};
+      D[c] = valid ? (double)inc.u[c] : 0.0;
+      M[c] = valid ? (double)inc.z[c] : 0.0;
     }
     if (lane == 0 && par >= 0) {
       float pd[4], pm[4], pdl[4], pml[4];
       R.get(ps, pd, pm, pdl, pml);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        D[c] = ff_add_f(ff{pd[c], pdl[c]}, inc.u[c]);
-        M[c] = ff_add_f(ff{pm[c], pml[c]}, inc.z[c]);
+        D[c] += as_f64(pd[c], pdl[c]);
+        M[c] += as_f64(pm[c], pml[c]);
       }
     }
     prefix_scan(D, lane);
@@ -1294,10 +1243,8 @@ __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int p
     float d[4], dl[4], m[4], ml[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      d[c] = D[c].hi;
-      dl[c] = D[c].lo;
-      m[c] = M[c].hi;
-      ml[c] = M[c].lo;
+      split_f64(D[c], d[c], dl[c]);
+      split_f64(M[c], m[c], ml[c]);
     }
     if (valid) R.put(sl, d, m, dl, ml);
     wave_sync();
@@ -1309,13 +1256,11 @@ __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int p
   float d[4], dl[4], m[4], ml[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) d[c] = dl[c] = m[c] = ml[c] = 0.f;
-  // depth 0 (parents in earlier windows; the driver reads the ring's zero
-  // slot): the path sums are renormalised here, once per window, and carried
-  // unnormalised through the window's deeper steps
+  // depth 0: parents in earlier windows (the driver reads the ring's zero slot)
   if (dw == 0) {
     float pd[4], pm[4], pdl[4], pml[4];
     R.get(par >= 0 ? ps : W, pd, pm, pdl, pml);
-    td_accum<COMP, true>(inc, pd, pdl, pm, pml, d, dl, m, ml);
+    td_accum<COMP>(inc, pd, pdl, pm, pml, d, dl, m, ml);
     R.put(sl, d, m, dl, ml);
   }
   wave_sync();
@@ -1323,7 +1268,7 @@ __device__ __forceinline__ void td_window(const SRing<W, COMP>& R, int ws, int p
     if (dw == t) {
       float pd[4], pm[4], pdl[4], pml[4];
       R.get(ps, pd, pm, pdl, pml);
-      td_accum<COMP, false>(inc, pd, pdl, pm, pml, d, dl, m, ml);
+      td_accum<COMP>(inc, pd, pdl, pm, pml, d, dl, m, ml);
       R.put(sl, d, m, dl, ml);
     }
     wave_sync();
@@ -1672,14 +1617,13 @@ template <int G, bool COMP>
 struct GroupLds {
   static constexpr int K = G / kWave;        // positions per lane
   static constexpr int NS = G / (2 * kWave);  // driver slews per lane (<= G/2 nets)
+  // (hi) / (lo): the halves of an FP64 value in the compensated mode
   float4 cap[G];   // bottom-up: cap by position   | top-down: delay (hi) by position
-  float4 w[G];     // bottom-up: w by schedule     | top-down: impulse (hi) by position
-  float4 ld[G];    // load (hi) by schedule index
+  float4 w[G];     // bottom-up: w (hi) by schedule | top-down: impulse (hi) by position
+  float4 ld[G];    // load (hi) by schedule index, rounded to FP32 after the sweep
   float4 L[G];     // L rounded to FP32 by schedule index
   float4 ldl[COMP ? G : 1];  // bottom-up: load (lo) by schedule index |
-                             // top-down: impulse (lo) by position (the sweep
-                             // and the outputs take load's normalised high
-                             // part, which is its rounding to FP32)
+                             // top-down: impulse (lo) by position
   float4 wl[COMP ? G : 1];   // bottom-up: w (lo) | top-down: delay (lo) by position
   float r[G];
   int32_t pid[G];
@@ -1765,11 +1709,12 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
       float cap4[4];
       unpack(S.cap[q], cap4);
       const float r = S.r[q];
-      ff ld[4], sw[4];
+      using T = typename std::conditional<COMP, double, float>::type;
+      T ld[4], sw[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        ld[c] = ff{cap4[c], 0.f};
-        sw[c] = ff{0.f, 0.f};
+        ld[c] = (T)cap4[c];
+        sw[c] = (T)0;
       }
       for (int k0 = 0; k0 < cn; k0 += kPull) {
         float4 cl[kPull], cw[kPull], cll[kPull], cwl[kPull];
@@ -1796,11 +1741,11 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             if constexpr (COMP) {
-              ld[c] = ff_acc(ld[c], ff{a1[c], a3[c]});
-              sw[c] = ff_acc(sw[c], ff{a2[c], a4[c]});
+              ld[c] += as_f64(a1[c], a3[c]);
+              sw[c] += as_f64(a2[c], a4[c]);
             } else {
-              ld[c].hi += a1[c];
-              sw[c].hi += a2[c];
+              ld[c] += a1[c];
+              sw[c] += a2[c];
             }
           }
         }
@@ -1809,20 +1754,13 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if constexpr (COMP) {
-          // children accumulated unnormalised: the load sum is
-          // renormalised (its high part enters the products); the L sum is
-          // renormalised by the compensated addition of the node's own term
-          const ff l2 = quick_two_sum(ld[c].hi, ld[c].lo);
-          const ff q2 = ff_add_f(sw[c], r * l2.hi * l2.hi);
-          o_ld[c] = l2.hi;
-          o_ldl[c] = l2.lo;
-          o_w[c] = q2.hi;
-          o_wl[c] = q2.lo;
-          o_L[c] = sw[c].hi + sw[c].lo;
+          split_f64(ld[c], o_ld[c], o_ldl[c]);
+          split_f64(sw[c] + (double)r * ld[c] * ld[c], o_w[c], o_wl[c]);
+          o_L[c] = (float)sw[c];
         } else {
-          o_ld[c] = ld[c].hi;
-          o_w[c] = sw[c].hi + r * ld[c].hi * ld[c].hi;
-          o_L[c] = sw[c].hi;
+          o_ld[c] = ld[c];
+          o_w[c] = sw[c] + r * ld[c] * ld[c];
+          o_L[c] = sw[c];
         }
       }
       S.ld[j] = pack(o_ld);
@@ -1836,6 +1774,19 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
     wave_sync();
     e = b;
     pwj = pwn;
+  }
+  // compensated mode: each node's load rounded to FP32 for the top-down sweep
+  // and the outputs (which reuse ldl by position)
+  if constexpr (COMP) {
+    for (int j = lane; j < n; j += kWave) {
+      float hi[4], lo[4], o[4];
+      unpack(S.ld[j], hi);
+      unpack(S.ldl[j], lo);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) o[c] = (float)as_f64(hi[c], lo[c]);
+      S.ld[j] = pack(o);
+    }
+    wave_sync();
   }
   // ---- top-down, depth 0 first: delay, impulse and net by position -----------
   pwj = lane < n ? S.pw[lane] : 0xffffffffu;
@@ -1872,8 +1823,7 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
         for (int c = 0; c < 4; ++c) pd[c] = pm[c] = pdl[c] = pml[c] = 0.f;
       }
       float d[4], dl[4], mm[4], ml[4];
-      // paths inside a group are at most 255 edges long: no renormalisation
-      td_accum<COMP, false>(td_incr<COMP>(rr, ldv, Lv), pd, pdl, pm, pml, d, dl, mm, ml);
+      td_accum<COMP>(td_incr<COMP>(rr, ldv, Lv), pd, pdl, pm, pml, d, dl, mm, ml);
       S.cap[q] = pack(d);
       S.w[q] = pack(mm);
       if constexpr (COMP) {
