@@ -172,7 +172,10 @@ otg_status otg_rct_reference_f64(otg_context* ctx, const otg_rct_tree* tree, dou
  * rct_bfs_and_compute_cuda (reference ot/cuda/rct.cu:385-389, kernel
  * compute_net_timing rct.cu:11-59) on device-resident data; adds beta and
  * slew.  plan may be NULL (then one is built and released inside the call,
- * which allocates).  Asynchronous on `stream`. */
+ * which allocates).  The batched methods move each node's four corners as
+ * one 16-byte vector: cap, slew_in and every result array must be 16-byte
+ * aligned (OTG_ERR_INVALID_ARGUMENT otherwise; hipMalloc'd arrays are).
+ * Asynchronous on `stream`. */
 otg_status otg_rct_solve(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tree* tree,
                          const otg_rct_result* out, otg_rct_method method,
                          otg_stream stream);
@@ -182,7 +185,8 @@ otg_status otg_rct_solve(otg_context* ctx, otg_rct_plan* plan, const otg_rct_tre
  * solved in the packed groups of the plan, so the other nets of a listed
  * net's group are re-solved too, from their current inputs (unchanged
  * inputs give bit-identical outputs); all other nets' outputs are
- * untouched.  Used by incremental re-timing.  Capturable in a hipGraph (no
+ * untouched.  Same 16-byte alignment rule as otg_rct_solve.  Used by
+ * incremental re-timing.  Capturable in a hipGraph (no
  * allocation, no memset, no host synchronisation); calls on one plan must
  * not overlap.  No reference counterpart (the reference CUDA path
  * re-flattens all nets, ot/timer/timer.cpp:931). */

@@ -81,9 +81,15 @@ typedef struct otg_prop_lib {
   const int32_t* timing_has;   /* [2*num_timings]                   */
 } otg_prop_lib;
 
-/* The graph: the fanin CSR of the candidate pins (the arrays otg_levelize
- * takes: fanin_start[n+1], fanin[m], arc j runs fanin[j] -> its pin), the
- * arcs' kinds and data, and per-pin inputs, all device arrays. */
+/* The graph: the fanin CSR of every pin (fanin_start[n+1], fanin[m]; arc j
+ * runs fanin[j] -> its pin, a pin's arcs in Pin::_fanin order, loop
+ * breakers kept and marked OTG_ARC_SKIP), the arcs' kinds and data, and
+ * per-pin inputs, all device arrays.  This is not the candidate CSR that
+ * otg_levelize takes for an incremental update (that one leaves out arcs
+ * from pins outside the candidates): a pin of the updated cone relaxes over
+ * all its fanin arcs (timer.cpp:733-783), reading the slew and arrival of a
+ * source outside the frontiers as the caller's arrays hold them (the
+ * previous results).  The frontiers come from the candidate levelization. */
 typedef struct otg_prop_graph {
   int32_t num_pins;
   int32_t num_arcs;
@@ -157,7 +163,12 @@ otg_status otg_fprop_run(otg_context* ctx, otg_fprop_plan* plan, const otg_prop_
  * (setup / hold tests of sequential cells, timer.cpp:823-835) are not
  * computed: po_rat is the only source.  slack (optional, needs at):
  * MIN at - rat, MAX rat - at (Pin::slack, pin.cpp:251-256).  arc_delay and
- * rat must be 8-byte aligned.  Asynchronous on `stream`. */
+ * rat must be 8-byte aligned.  Asynchronous on `stream`.
+ * Inputs read outside the levels: the rat of a fanout sink that lies
+ * outside the frontiers, and the arc_delay of an arc that otg_fprop did not
+ * write (an arc into a pin outside its frontiers), are read as the caller
+ * left them -- earlier results of an incremental update, or NaN (unset) for
+ * a first update; otg_bprop writes only the pins of the frontiers. */
 typedef struct otg_bprop_graph {
   int32_t num_pins;
   int32_t num_arcs;

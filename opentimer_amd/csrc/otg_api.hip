@@ -118,6 +118,24 @@ static otg_status check_tree(otg_context* ctx, const otg_rct_tree* t) {
   return OTG_OK;
 }
 
+// The batched kernels move the per-corner arrays as float4 (16-byte vector
+// loads and stores).
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+static otg_status check_vec_alignment(otg_context* ctx, const otg_rct_tree* t,
+                                      const otg_rct_result* o) {
+  const void* arrays[] = {t->cap, t->slew_in, o->load, o->delay, o->ldelay, o->beta,
+                          o->impulse, o->slew};
+  for (const void* p : arrays) {
+    if (p && !aligned16(p)) {
+      OTG_SET_ERR(ctx, "cap, slew_in and the result arrays must be 16-byte aligned "
+                       "(per-node corner quadruples are moved as float4)");
+      return OTG_ERR_INVALID_ARGUMENT;
+    }
+  }
+  return OTG_OK;
+}
+
 extern "C" otg_status otg_rct_reference_f64(otg_context* ctx, const otg_rct_tree* tree,
                                             double* load, double* delay, double* ldelay,
                                             double* beta, double* impulse, double* slew,
@@ -141,6 +159,8 @@ extern "C" otg_status otg_rct_solve(otg_context* ctx, otg_rct_plan* plan, const 
   if (method == OTG_RCT_PLAIN) return otg_rct_launch_plain(ctx, tree, out, s);
   if (method != OTG_RCT_BATCHED && method != OTG_RCT_BATCHED_COMPENSATED)
     return OTG_ERR_INVALID_ARGUMENT;
+  st = check_vec_alignment(ctx, tree, out);
+  if (st != OTG_OK) return st;
   if (plan) {
     if (plan->num_nets != tree->num_nets || plan->total_nodes != tree->total_num_nodes) {
       OTG_SET_ERR(ctx, "plan was built for a different topology");
@@ -178,6 +198,8 @@ extern "C" otg_status otg_rct_solve_subset(otg_context* ctx, otg_rct_plan* plan,
     return OTG_ERR_INVALID_ARGUMENT;
   }
   if (plan->num_nets != tree->num_nets) return OTG_ERR_INVALID_ARGUMENT;
+  st = check_vec_alignment(ctx, tree, out);
+  if (st != OTG_OK) return st;
   return otg_rct_launch_batched(ctx, plan, tree, out, net_ids, count, otg_pick_stream(ctx, stream));
 }
 

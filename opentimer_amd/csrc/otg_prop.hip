@@ -459,26 +459,47 @@ __global__ void plan_fill_kernel(int32_t* a, int64_t n, int32_t v) {
     a[i] = v;
 }
 
-__global__ void plan_pos_kernel(const int32_t* frontiers, int32_t n_lv, const int32_t* fanin_start,
-                                int32_t* pin_pos, int32_t* deg) {
+// A frontier pin out of range, or a fanin range outside the arcs, reports the
+// position (bad[0], smallest); such a position gets no arcs.
+__global__ void plan_pos_kernel(const int32_t* frontiers, int32_t n_lv, int32_t num_pins,
+                                int32_t num_arcs, const int32_t* fanin_start, int32_t* pin_pos,
+                                int32_t* deg, int32_t* bad) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
        q += (int64_t)gridDim.x * blockDim.x) {
     const int v = frontiers[q];
-    pin_pos[v] = (int)q;
-    deg[q] = fanin_start[v + 1] - fanin_start[v];
+    int d = 0;
+    if (v < 0 || v >= num_pins) {
+      atomicMin(bad, (int)q);
+    } else {
+      const int b = fanin_start[v], e = fanin_start[v + 1];
+      if (b < 0 || e < b || e > num_arcs) atomicMin(bad, (int)q);
+      else d = e - b;
+      pin_pos[v] = (int)q;
+    }
+    deg[q] = d;
   }
 }
 
-__global__ void plan_slot_kernel(const int32_t* frontiers, int32_t n_lv, const int32_t* fanin_start,
-                                 const int32_t* fanin, const int32_t* pin_pos,
-                                 const int32_t* aofs, int32_t* slot_src, int32_t* slot_arc,
-                                 int32_t* slot_pos) {
+// A fanin source out of range reports its arc (bad[1], smallest) and is read
+// as pin 0 (the plan is then discarded).
+__global__ void plan_slot_kernel(const int32_t* frontiers, int32_t n_lv, int32_t num_pins,
+                                 const int32_t* fanin_start, const int32_t* fanin,
+                                 const int32_t* pin_pos, const int32_t* aofs, int32_t* slot_src,
+                                 int32_t* slot_arc, int32_t* slot_pos, int32_t* bad) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
        q += (int64_t)gridDim.x * blockDim.x) {
+    const int k0 = aofs[q], dn = aofs[q + 1] - k0;
+    if (dn == 0) continue;  // (also every position the previous kernel rejected)
     const int v = frontiers[q];
-    const int j0 = fanin_start[v], k0 = aofs[q], dn = aofs[q + 1] - k0;
+    const int j0 = fanin_start[v];
     for (int r = 0; r < dn; ++r) {
-      const int j = j0 + r, u = fanin[j], pu = pin_pos[u];
+      const int j = j0 + r;
+      int u = fanin[j];
+      if (u < 0 || u >= num_pins) {
+        atomicMin(bad + 1, j);
+        u = 0;
+      }
+      const int pu = pin_pos[u];
       slot_src[k0 + r] = pu >= 0 ? pu : -u - 1;
       slot_arc[k0 + r] = j;
       slot_pos[k0 + r] = (int)q;
@@ -753,30 +774,37 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
   p->num_levels = num_levels;
   int32_t* pin_pos = nullptr;
   int32_t* deg = nullptr;
+  int32_t* bad = nullptr;
   void* tmp = nullptr;
   auto fail = [&](hipError_t e) {
     (void)hipStreamSynchronize(s);
     if (pin_pos) (void)hipFree(pin_pos);
     if (deg) (void)hipFree(deg);
+    if (bad) (void)hipFree(bad);
     if (tmp) (void)hipFree(tmp);
     plan_free(p);
     otg_set_error(ctx, "otg_fprop_plan_create: %s", hipGetErrorString(e));
     return e == hipErrorOutOfMemory ? OTG_ERR_OUT_OF_MEMORY : OTG_ERR_DEVICE;
   };
   hipError_t e;
-  int32_t e0 = 0, n_lv = 0;
+  int32_t n_lv = 0;
+  // the level ends, on the host: from 0, never decreasing, at most num_pins
+  std::vector<int32_t> ends(num_levels + 1, 0);
   if (num_levels > 0) {
-    if ((e = hipMemcpyAsync(&e0, frontier_ends, sizeof(int32_t), hipMemcpyDeviceToHost, s)) ||
-        (e = hipMemcpyAsync(&n_lv, frontier_ends + num_levels, sizeof(int32_t),
+    if ((e = hipMemcpyAsync(ends.data(), frontier_ends, sizeof(int32_t) * (num_levels + 1),
                             hipMemcpyDeviceToHost, s)) ||
         (e = hipStreamSynchronize(s)))
       return fail(e);
-    if (e0 != 0 || n_lv < 0 || n_lv > num_pins) {
+    int32_t bad_lv = ends[0] != 0 ? 0 : -1;
+    for (int32_t lv = 0; lv < num_levels && bad_lv < 0; ++lv)
+      if (ends[lv + 1] < ends[lv] || ends[lv + 1] > num_pins) bad_lv = lv + 1;
+    if (bad_lv >= 0) {
       plan_free(p);
-      otg_set_error(ctx, "otg_fprop_plan_create: level ends run from %d to %d over %d pins", e0,
-                    n_lv, num_pins);
+      otg_set_error(ctx, "otg_fprop_plan_create: level end %d is %d (ends must run from 0 "
+                         "without decreasing to at most %d pins)", bad_lv, ends[bad_lv], num_pins);
       return OTG_ERR_INVALID_ARGUMENT;
     }
+    n_lv = ends[num_levels];
   }
   p->n_lv = n_lv;
   const size_t I = sizeof(int32_t);
@@ -784,8 +812,9 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
       (e = hipMalloc(&p->ends, I * (num_levels + 1))) ||
       (e = hipMalloc(&p->aofs, I * (n_lv + 1))) ||
       (e = hipMalloc(&pin_pos, I * std::max(num_pins, 1))) ||
-      (e = hipMalloc(&deg, I * std::max(n_lv, 1))))
+      (e = hipMalloc(&deg, I * std::max(n_lv, 1))) || (e = hipMalloc(&bad, I * 2)))
     return fail(e);
+  hipLaunchKernelGGL(plan_fill_kernel, dim3(1), dim3(64), 0, s, bad, (int64_t)2, INT32_MAX);
   if (n_lv > 0 &&
       (e = hipMemcpyAsync(p->frontiers, frontiers, I * n_lv, hipMemcpyDeviceToDevice, s)))
     return fail(e);
@@ -797,7 +826,7 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
   hipLaunchKernelGGL(plan_fill_kernel, dim3(1), dim3(64), 0, s, p->aofs, (int64_t)1, 0);
   if (n_lv > 0) {
     hipLaunchKernelGGL(plan_pos_kernel, dim3(grid_for(ctx, n_lv, 256)), dim3(256), 0, s,
-                       p->frontiers, n_lv, fanin_start, pin_pos, deg);
+                       p->frontiers, n_lv, num_pins, num_arcs, fanin_start, pin_pos, deg, bad);
     size_t tb = 0;
     if ((e = hipcub::DeviceScan::InclusiveSum(nullptr, tb, deg, p->aofs + 1, n_lv, s)) ||
         (e = hipMalloc(&tmp, std::max<size_t>(tb, 1))) ||
@@ -819,17 +848,30 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
     return fail(e);
   if (n_lv > 0)
     hipLaunchKernelGGL(plan_slot_kernel, dim3(grid_for(ctx, n_lv, 256)), dim3(256), 0, s,
-                       p->frontiers, n_lv, fanin_start, fanin, pin_pos, p->aofs, p->slot_src,
-                       p->slot_arc, p->slot_pos);
+                       p->frontiers, n_lv, num_pins, fanin_start, fanin, pin_pos, p->aofs,
+                       p->slot_src, p->slot_arc, p->slot_pos, bad);
   // wavefront chunks, level by level: consecutive positions, at most 16 of
   // them and at most 16 arc slots (a position with more slots alone)
   {
-    std::vector<int32_t> aofs(n_lv + 1), ends(num_levels + 1, 0);
+    std::vector<int32_t> aofs(n_lv + 1);
+    int32_t hbad[2] = {INT32_MAX, INT32_MAX};
     if ((e = hipMemcpyAsync(aofs.data(), p->aofs, I * (n_lv + 1), hipMemcpyDeviceToHost, s)) ||
-        (num_levels > 0 && (e = hipMemcpyAsync(ends.data(), p->ends, I * (num_levels + 1),
-                                               hipMemcpyDeviceToHost, s))) ||
+        (e = hipMemcpyAsync(hbad, bad, I * 2, hipMemcpyDeviceToHost, s)) ||
         (e = hipStreamSynchronize(s)))
       return fail(e);
+    if (hbad[0] != INT32_MAX || hbad[1] != INT32_MAX) {
+      (void)hipFree(pin_pos);
+      (void)hipFree(deg);
+      (void)hipFree(bad);
+      if (tmp) (void)hipFree(tmp);
+      plan_free(p);
+      if (hbad[0] != INT32_MAX)
+        otg_set_error(ctx, "otg_fprop_plan_create: frontier position %d names a pin out of range "
+                           "or a fanin range outside the arcs", hbad[0]);
+      else
+        otg_set_error(ctx, "otg_fprop_plan_create: arc %d comes from a pin out of range", hbad[1]);
+      return OTG_ERR_INVALID_ARGUMENT;
+    }
     std::vector<int2> chunks;
     std::vector<int32_t> wstart(num_levels + 1, 0);
     chunks.reserve(n_lv / 4 + num_levels + 1);
@@ -857,6 +899,7 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
   if ((e = hipStreamSynchronize(s)) || (e = hipGetLastError())) return fail(e);
   (void)hipFree(pin_pos);
   (void)hipFree(deg);
+  (void)hipFree(bad);
   if (tmp) (void)hipFree(tmp);
   *plan = p;
   return OTG_OK;

@@ -375,3 +375,125 @@ def test_fprop_empty_and_invalid(ctx):
     st = N.lib.otg_fprop(ctx.ptr, None, C.byref(P.lib), z.data_ptr(), z.data_ptr(), 1,
                          C.byref(out), None)
     assert st == 1
+
+
+def test_fprop_plan_rejects_malformed_levels(ctx):
+    # level ends that do not start at 0, decrease or pass the pin count, a
+    # frontier pin out of range and a fanin source out of range are refused
+    # with OTG_ERR_INVALID_ARGUMENT; the well-formed levels build a plan
+    dag = W.generate_dag(W.DagConfig("dag-small", 2000, 5000, 12, seed=31), nthreads=4)
+    csr = W.fanin_csr(dag)
+    L = Levelizer(csr, ctx)
+    L.run()
+    fr, ends, _ = L.host()
+    nl = len(ends) - 1
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(dev)  # noqa: E731
+    fs, fi = t(csr.start), t(csr.fanin)
+
+    def create(frontiers, level_ends, fanin=fi):
+        plan = C.c_void_p()
+        dfr, dends = t(frontiers), t(level_ends)  # held until the call returns
+        st = N.lib.otg_fprop_plan_create(ctx.ptr, csr.n, len(csr.fanin), fs.data_ptr(),
+                                         fanin.data_ptr(), dfr.data_ptr(), dends.data_ptr(), nl,
+                                         None, C.byref(plan))
+        if st == 0:
+            N.lib.otg_fprop_plan_destroy(plan)
+        return st
+
+    assert create(fr, ends) == 0
+    e = ends.copy()
+    e[0] = 1
+    assert create(fr, e) == 1
+    e = ends.copy()
+    e[nl // 2] = e[nl // 2 + 1] + 1  # a level end above the next one
+    assert create(fr, e) == 1
+    e = ends.copy()
+    e[-1] = csr.n + 1
+    assert create(fr, e) == 1
+    f2 = fr.copy()
+    f2[7] = csr.n + 5
+    assert create(f2, ends) == 1
+    f2[7] = -2
+    assert create(f2, ends) == 1
+    bad_fanin = csr.fanin.copy()
+    bad_fanin[len(bad_fanin) // 3] = csr.n
+    dbad = t(bad_fanin)
+    assert create(fr, ends, dbad) == 1
+    assert create(fr, ends) == 0  # the context stays usable
+
+
+@pytest.mark.parametrize("use_plan", [False, True])
+def test_fprop_partial_cone_over_the_full_fanin_csr(ctx, use_plan):
+    # Incremental update: one pin's load changes; its fanout cone is
+    # levelized from the candidate CSR (arcs between candidates only), and
+    # propagated over the FULL fanin CSR, starting from the previous results
+    # (a cone pin still relaxes over its fanin arcs from outside the cone,
+    # timer.cpp:733-783).  Equal, bit for bit, to a full propagation of the
+    # changed design.
+    import dataclasses
+    dag = W.generate_dag(W.DagConfig("dag-cone", 20_000, 50_000, 30, seed=41), nthreads=4)
+    csr = W.fanin_csr(dag)
+    lib = W.prop_library(num_timings=64, seed=11)
+    g = W.prop_graph(csr, lib, seed=12)
+    L = Levelizer(csr, ctx)
+    L.run()
+    P1 = Propagator(g, lib, ctx)
+    P1.run(L.frontiers, L.ends, L.num_ends - 1)
+    before = P1.host()
+    # the changed input: the load of one pin (the cell arcs driving it see
+    # it), whose fanout cone holds a few hundred pins (bounded searches)
+    n = csr.n
+    fo = [[] for _ in range(n)]
+    for a, b in zip(dag.arc_from, dag.arc_to):
+        fo[a].append(b)
+    rng = np.random.default_rng(5)
+    cell_driven = np.zeros(n, bool)
+    cell_driven[np.repeat(np.arange(n), np.diff(csr.start))[g.arc_kind == 2]] = True
+    best = None
+    for p0 in rng.permutation(np.flatnonzero(cell_driven))[:400]:
+        seen = {int(p0)}
+        stack = [int(p0)]
+        while stack and len(seen) <= 5000:
+            for v in fo[stack.pop()]:
+                if v not in seen:
+                    seen.add(v)
+                    stack.append(v)
+        if 100 <= len(seen) <= 5000 and not stack:
+            best = (int(p0), seen)
+            break
+    assert best is not None
+    p0, cone = best
+    load = g.pin_load.copy()
+    load[p0] *= np.float32(1.5)
+    g2 = dataclasses.replace(g, pin_load=load)
+    # expected: a full propagation of the changed design
+    P2 = Propagator(g2, lib, ctx)
+    P2.run(L.frontiers, L.ends, L.num_ends - 1)
+    want = P2.host()
+    # incremental: candidate levelization, full CSR, previous results
+    mask = np.zeros(n, np.uint8)
+    mask[list(cone)] = 1
+    cc = W.fanin_csr(dag, cand=mask)
+    Lc = Levelizer(cc, ctx)
+    Lc.run()
+    fr_c, ends_c, _ = Lc.host()
+    assert set(fr_c[: ends_c[-1]].tolist()) == cone
+    # some cone pin has a fanin arc from outside the cone (the case the
+    # candidate CSR would lose)
+    fs = csr.start
+    assert any(int(u) not in cone for v in cone for u in csr.fanin[fs[v]:fs[v + 1]])
+    P3 = Propagator(g2, lib, ctx)
+    P3.slew.copy_(P1.slew)
+    P3.at.copy_(P1.at)
+    P3.arc_delay.copy_(P1.arc_delay)
+    if use_plan:
+        P3.build_plan(Lc.frontiers, Lc.ends, Lc.num_ends - 1)
+        P3.run_plan()
+        P3.close_plan()
+    else:
+        P3.run(Lc.frontiers, Lc.ends, Lc.num_ends - 1)
+    got = P3.host()
+    for a, b in zip(got, want):
+        _same(a, b)
+    assert not np.array_equal(np.nan_to_num(want[1]), np.nan_to_num(before[1]))

@@ -488,3 +488,40 @@ def test_tuning_knobs_keep_results(ctx, tun, method, tol64):
     fl = floors(r64)
     for k in OUT:
         assert rel_err(g[k], r64[k], fl[k]) < tol64, (tun, k)
+
+
+def test_batched_solves_reject_unaligned_corner_arrays(ctx):
+    # the batched kernels move a node's four corners as one 16-byte vector:
+    # a result array (or cap) 4 bytes off a 16-byte boundary is refused
+    # before any launch, by the full and the subset solve; the plain method
+    # takes it
+    import ctypes as C
+    cfg = W.RcConfig("align", 300, [W.SizeClass(0.7, 2, 40), W.SizeClass(0.3, 300, 700)], seed=5)
+    f, s = _forest(cfg, ctx)
+    good = s.result()
+    V = s.num_nodes
+    spare = torch.empty(4 * V + 4, dtype=torch.float32, device=s.device)
+    ids = torch.arange(s.num_nets, dtype=torch.int32, device=s.device)
+    for method in ("batched", "compensated"):
+        m = {"batched": N.BATCHED, "compensated": N.BATCHED_COMPENSATED}[method]
+        plan = s.plan(method)
+        for field in ("load", "delay", "slew"):
+            bad = N.RctResult(*[getattr(good, k) for k, _ in N.RctResult._fields_])
+            setattr(bad, field, spare.data_ptr() + 4)
+            st = N.lib.otg_rct_solve(ctx.ptr, plan, C.byref(s.tree()), C.byref(bad), m, None)
+            assert st == 1, (method, field, st)  # OTG_ERR_INVALID_ARGUMENT
+            st = N.lib.otg_rct_solve_subset(ctx.ptr, plan, C.byref(s.tree()), C.byref(bad), m,
+                                            ids.data_ptr(), s.num_nets, None)
+            assert st == 1, (method, field, st)
+        tree = s.tree()
+        cap2 = torch.empty(4 * V + 4, dtype=torch.float32, device=s.device)
+        cap2[1:4 * V + 1].copy_(s.cap)
+        tree.cap = cap2.data_ptr() + 4
+        assert N.lib.otg_rct_solve(ctx.ptr, plan, C.byref(tree), C.byref(good), m, None) == 1
+    # the plain kernel reads scalars: an offset result array is fine
+    bad = N.RctResult(*[getattr(good, k) for k, _ in N.RctResult._fields_])
+    bad.delay = spare.data_ptr() + 4
+    ctx.check(N.lib.otg_rct_solve(ctx.ptr, None, C.byref(s.tree()), C.byref(bad), N.PLAIN, None))
+    torch.cuda.synchronize()
+    s.solve("plain")
+    assert torch.equal(spare[1:4 * V + 1], s.out["delay"][:4 * V])
