@@ -45,10 +45,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-autotune", action="store_true",
-                    help="solve with the default tuning instead of autotuning each method first")
+                    help="solve with the built-in default tuning")
+    ap.add_argument("--autotune", action="store_true",
+                    help="search each method's tuning on this forest (otg_rct_autotune) instead "
+                         "of loading the shipped MI355X tuning for the workload")
     ap.add_argument("--tuning", default="",
-                    help="a tuning file (otg_rct_tuning_load) used for both methods instead of "
-                         "autotuning")
+                    help="a tuning file (otg_rct_tuning_load) used for both methods")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the host-pointer drop-in leg (otg_rct_bfs_and_compute at cfg1)")
     ap.add_argument("--only", default="",
@@ -790,21 +792,31 @@ def main():
     log(f"bfs ordering: {bfs_ms:.2f} ms")
 
     import ctypes as C
-    # ---- tuning: each batched method autotuned on this forest (its own
-    # coordinate-descent search, otg_rct_autotune), unless a file is given
+    # ---- tuning: by default each batched method takes the tuning shipped
+    # for this workload on MI355X (opentimer_amd/tunings/, written by
+    # otg_rct_autotune on this forest); --autotune searches it anew
     other = "batched" if args.method == "compensated" else "compensated"
     tunings = {}
     tuning_info = {}
     s.alloc_outputs()
     tdir = os.path.join(ROOT, "gpurun_out") if os.path.isdir(os.path.join(ROOT, "gpurun_out")) \
         else "/tmp"
-    # (searched on a context of its own: the solver's context keeps the
-    # built-in defaults, and each plan gets its method's tuning explicitly)
+    # (loaded or searched on a context of its own: the solver's context keeps
+    # the built-in defaults, and each plan gets its method's tuning explicitly)
     tctx = N.Context(0)
     for m in (args.method, other):
         if args.tuning or args.no_autotune:
             tunings[m] = None
             tuning_info[m] = {"source": args.tuning or "defaults"}
+            continue
+        shipped = os.path.join(ROOT, "opentimer_amd", "tunings", f"{cfg.name}_{m}.json")
+        if not args.autotune and args.nets == 4_000_000 and os.path.exists(shipped):
+            t = N.RctTuning()
+            tctx.check(N.lib.otg_rct_tuning_load(tctx.ptr, shipped.encode(), C.byref(t)))
+            tunings[m] = tuple(getattr(t, k) for k, _ in N.RctTuning._fields_)
+            tuning_info[m] = {"source": os.path.relpath(shipped, ROOT),
+                              **{k: getattr(t, k) for k, _ in N.RctTuning._fields_}}
+            log(f"tuning {m}: {tuning_info[m]}")
             continue
         best = N.RctTuning()
         path = os.path.join(tdir, f"tuning_{m}.json")
