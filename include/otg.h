@@ -140,6 +140,12 @@ typedef struct otg_rct_tuning {
                           /* launch when several are spread over streams (0: */
                           /* as many as fit), leaving room on every CU for  */
                           /* the larger rings from the start                */
+  int32_t group_alongside;/* 0 (default): the packed groups run first, alone,*/
+                          /* dealt out to a resident grid; k > 0: they are   */
+                          /* taken from a queue by k workgroups per CU on a  */
+                          /* stream of their own, launched beside the       */
+                          /* streaming launches (1..16; alone under graph   */
+                          /* capture, where the launches share one stream)  */
 } otg_rct_tuning;
 
 /* Build the work list of a topology (net sizes and parent offsets only):

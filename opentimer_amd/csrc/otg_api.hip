@@ -562,10 +562,11 @@ static bool write_tuning(const char* path, const otg_rct_tuning& t, otg_rct_meth
   fprintf(f,
           "{\"ring_slots\": %d, \"min_ring\": %d, \"small_target\": %d, "
           "\"grid_per_cu\": %d, \"streams\": %d, \"small_max\": %d, \"ring_mask\": %d, "
-          "\"first_on_caller\": %d, \"small_grid_per_cu\": %d, \"method\": %d, "
-          "\"ms\": %.6f, \"candidates\": %d}\n",
+          "\"first_on_caller\": %d, \"small_grid_per_cu\": %d, \"group_alongside\": %d, "
+          "\"method\": %d, \"ms\": %.6f, \"candidates\": %d}\n",
           t.ring_slots, t.min_ring, t.small_target, t.grid_per_cu, t.streams, t.small_max,
-          t.ring_mask, t.first_on_caller, t.small_grid_per_cu, (int)m, ms, evaluated);
+          t.ring_mask, t.first_on_caller, t.small_grid_per_cu, t.group_alongside, (int)m, ms,
+          evaluated);
   fclose(f);
   return true;
 }
@@ -601,6 +602,7 @@ extern "C" otg_status otg_rct_tuning_load(otg_context* ctx, const char* path, ot
   r.ring_mask = json_int(s, "ring_mask", 0);
   r.first_on_caller = json_int(s, "first_on_caller", 0);
   r.small_grid_per_cu = json_int(s, "small_grid_per_cu", 0);
+  r.group_alongside = json_int(s, "group_alongside", 0);
   ctx->tuning = r;
   if (t) *t = r;
   return OTG_OK;
@@ -653,7 +655,9 @@ extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tre
   // one more launch), the ring ladder (320 on or off), the largest net of a
   // packed group (the group/stream threshold), the group size, which launch
   // takes the caller's stream, the smallest-ring launch's workgroups per CU,
-  // and the workgroups per CU of every launch (0: as many as fit).
+  // the workgroups per CU of every launch (0: as many as fit), and whether
+  // the packed groups run first or beside the streaming launches (with how
+  // many workgroups per CU).
   otg_rct_tuning cur{};
   cur.ring_slots = 2048;
   otg_rct_tuning resolved{};
@@ -679,6 +683,7 @@ extern "C" otg_status otg_rct_autotune(otg_context* ctx, const otg_rct_tree* tre
       {&otg_rct_tuning::first_on_caller, {0, 1}},
       {&otg_rct_tuning::small_grid_per_cu, {0, 4, 6}},
       {&otg_rct_tuning::grid_per_cu, {0, 4}},
+      {&otg_rct_tuning::group_alongside, {0, 1, 2, 4}},
   };
   for (int pass = 0; pass < 2; ++pass) {
     for (const Knob& k : knobs) {

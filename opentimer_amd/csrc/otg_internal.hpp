@@ -17,10 +17,11 @@ struct otg_context {
   std::string last_error;
   otg_rct_tuning tuning{};  // defaults for new plans (autotuner may update)
   double last_breakdown[8] = {};  // stages of the last host-pointer P1 call (ms)
-  // Auxiliary streams that the streaming launches of one batched solve are
-  // spread over (forked from and joined back into the caller's stream),
-  // created on first use.
-  static constexpr int kAux = 2;
+  // Auxiliary streams that the launches of one batched solve are spread over
+  // (forked from and joined back into the caller's stream), created on first
+  // use: up to two for streaming launches, one for packed groups that run
+  // beside them.
+  static constexpr int kAux = 3;
   hipStream_t aux[kAux] = {};
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[kAux] = {};
@@ -82,7 +83,8 @@ struct otg_rct_plan {
   int32_t* d_queue = nullptr;      // [kQueueInts]: next item and finished workgroups of
                                    // streaming bin b [2b, 2b+1] and of the global-window
                                    // kernel [2B, 2B+1] (B = kMaxBins); [2B + 2 + b]: bin
-                                   // b's nets of a subset solve
+                                   // b's nets of a subset solve; [3B+2, 3B+3]: packed
+                                   // groups that run beside the streaming launches
   float* d_scratch = nullptr;      // window storage of stream/global items
   int64_t scratch_floats = 0;
   int32_t* d_net_maxpid = nullptr;  // per net, for subset solves

@@ -96,8 +96,19 @@ __device__ __forceinline__ int owner_of(int t, int off_mine, int lane_count) {
   return lo;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void levelize_level_kernel(LevelizeParams P, int lv) {
+// Appends to the next frontier are gathered per workgroup: a wavefront takes
+// slots in an LDS staging buffer with an LDS atomic, and after the level's
+// work the workgroup reserves its whole run of the frontier with one global
+// atomic and copies the buffer out.  (All wavefronts of the grid append to
+// one counter; with one global atomic per wavefront and fanin step, those
+// same-address atomics serialise and took about half of a level's time.)
+// A wavefront whose appends do not fit the buffer appends the rest directly.
+constexpr int kStage = 2048;
+
+template <int MODE, int BS>
+__global__ __launch_bounds__(BS) void levelize_level_kernel(LevelizeParams P, int lv) {
+  __shared__ int32_t s_buf[kStage];
+  __shared__ int s_cnt, s_base;
   if (lv > P.n) return;
   const int b = P.ends[lv];
   const int e = lv == 0 ? (P.first_dev ? *P.first_dev : P.first_size) : b + P.cnt[lv - 1];
@@ -107,6 +118,10 @@ __global__ __launch_bounds__(kBlock) void levelize_level_kernel(LevelizeParams P
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) P.ends[lv + 1] = e;
   const int last = e - b;
+  // (the return above is uniform over the workgroup: b and e are)
+  if ((int64_t)blockIdx.x * BS >= last) return;  // no chunk for this workgroup
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
   int32_t* next = P.frontiers + e;
   int32_t* ctr = P.cnt + lv;
   const int lane = threadIdx.x & 63;
@@ -158,19 +173,42 @@ __global__ __launch_bounds__(kBlock) void levelize_level_kernel(LevelizeParams P
         tot += __popcll(m[k]);
       }
       if (tot) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(ctr, tot);
-        base = __shfl(base, 0);
+        // slots [off, off + tot) of the staging buffer; those past its end
+        // go straight to the frontier
+        int so = 0;
+        if (lane == 0) so = atomicAdd(&s_cnt, tot);
+        so = __shfl(so, 0);
+        const int spill = so + tot - max(so, kStage);  // appends past the buffer
+        int gb = 0;
+        if (spill > 0 && lane == 0) gb = atomicAdd(ctr, spill);
+        gb = __shfl(gb, 0) - max(so, kStage);
+        int r = so;
 #pragma unroll
         for (int k = 0; k < kArcs; ++k) {
           if (old[k] == 1) {
-            next[base + __popcll(m[k] & ((1ull << lane) - 1ull))] = v[k];
-            if (P.level) P.level[v[k]] = lv + 1;
+            const int slot = r + __popcll(m[k] & ((1ull << lane) - 1ull));
+            if (slot < kStage) {
+              s_buf[slot] = v[k];
+            } else {
+              next[gb + slot] = v[k];
+              if (P.level) P.level[v[k]] = lv + 1;
+            }
           }
-          base += __popcll(m[k]);
+          r += __popcll(m[k]);
         }
       }
     }
+  }
+  __syncthreads();
+  const int ns = min(s_cnt, kStage);
+  if (ns == 0) return;  // uniform: every thread reads the same count
+  if (threadIdx.x == 0) s_base = atomicAdd(ctr, ns);
+  __syncthreads();
+  const int base = s_base;
+  for (int i = threadIdx.x; i < ns; i += BS) {
+    const int v = s_buf[i];
+    next[base + i] = v;
+    if (P.level) P.level[v] = lv + 1;
   }
 }
 
@@ -285,7 +323,8 @@ otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_si
   int32_t ne = INT32_MAX;
   for (int lv0 = 0; lv0 <= n; lv0 += kBatch) {
     for (int lv = lv0; lv < lv0 + kBatch && lv <= n; ++lv)
-      hipLaunchKernelGGL(levelize_level_kernel<LV_KAHN>, dim3(grid), dim3(kBlock), 0, s, P, lv);
+      hipLaunchKernelGGL((levelize_level_kernel<LV_KAHN, kBlock>), dim3(grid), dim3(kBlock), 0, s,
+                         P, lv);
     OTG_HIP(ctx, hipGetLastError());
     OTG_HIP(ctx, hipMemcpyAsync(&ne, P.num_ends, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     OTG_HIP(ctx, hipStreamSynchronize(s));
@@ -468,14 +507,16 @@ extern "C" otg_status otg_incr_relevelize(otg_incr* I, const int32_t* net_ids, i
   LevelizeParams R{S.n, 0, I->fo_start, I->fo, S.flag, S.cone, S.cone_ends, nullptr,
                    S.cone_cnt, S.counters + 2, S.counters + 0, nullptr};
   for (int lv = 0; lv <= L; ++lv)
-    hipLaunchKernelGGL(levelize_level_kernel<LV_REACH>, dim3(I->grid), dim3(kBlock), 0, s, R, lv);
+    hipLaunchKernelGGL((levelize_level_kernel<LV_REACH, kBlock>), dim3(I->grid), dim3(kBlock), 0, s,
+                       R, lv);
   // sink-first Kahn sweep over the cone
   hipLaunchKernelGGL(incr_kahn_init_kernel, dim3(I->grid), dim3(kBlock), 0, s, S, I->out_degree,
                      I->level);
   LevelizeParams K{S.n, 0, I->fi_start, I->fi, S.kout, S.kfront, S.kends, I->level,
                    S.kcnt, S.counters + 3, S.counters + 1, S.flag};
   for (int lv = 0; lv <= L; ++lv)
-    hipLaunchKernelGGL(levelize_level_kernel<LV_KAHN>, dim3(I->grid), dim3(kBlock), 0, s, K, lv);
+    hipLaunchKernelGGL((levelize_level_kernel<LV_KAHN, kBlock>), dim3(I->grid), dim3(kBlock), 0, s,
+                       K, lv);
   hipLaunchKernelGGL(incr_cleanup_kernel, dim3(I->grid), dim3(kBlock), 0, s, S);
   OTG_HIP(ctx, hipGetLastError());
   return OTG_OK;

@@ -1393,9 +1393,6 @@ __device__ __forceinline__ void run_td_chunk(StreamLds<W, COMP>& S, const RctPar
 // its first loads are requested.  Scalar loads count against lgkmcnt, which
 // every LDS wait drains in passing; a vector load of the descriptor would be
 // waited for with vmcnt, i.e. together with every store still in flight.
-#ifndef GROUP_QUEUE
-#define GROUP_QUEUE false
-#endif
 typedef const __attribute__((address_space(4))) RctItem ConstItem;
 
 template <bool QUEUE>
@@ -1880,12 +1877,15 @@ __device__ __forceinline__ void group_outputs(GroupLds<G, COMP>& S, const RctPar
   wave_sync();
 }
 
-template <int G, bool COMP>
+// QUEUE: groups are taken from a queue (the kernel runs beside streaming
+// launches, so its workgroups start at different times); otherwise they are
+// dealt out to a grid that is resident from the start.
+template <int G, bool COMP, bool QUEUE>
 __global__ __launch_bounds__(64) void rct_group_kernel(RctParams P) {
   __shared__ GroupLds<G, COMP> S;
   const int lane = threadIdx.x;
   const int total = P.num_items_dev ? gld(P.num_items_dev) : P.num_items;
-  ItemPipe<GROUP_QUEUE> pipe{P.items, total, P.queue, lane};
+  ItemPipe<QUEUE> pipe{P.items, total, P.queue, lane};
   bool have = false;
   Item it = pipe.start(&have);
   // One register buffer: a group's inputs are staged into LDS first, after
@@ -1914,7 +1914,7 @@ __global__ __launch_bounds__(64) void rct_group_kernel(RctParams P) {
     it = nit;
     have = nhave;
   }
-  if constexpr (GROUP_QUEUE) queue_release(P.queue, lane);
+  if constexpr (QUEUE) queue_release(P.queue, lane);
 }
 
 // Group capacity (positions) of a plan: 128 or 256, from small_target.
@@ -2004,12 +2004,17 @@ void* pick_stream(int W, bool comp, bool chain = false) {
   return chain ? pick_stream_t<true, true>(W) : pick_stream_t<true, false>(W);
 }
 
-void* pick_group(int G, bool comp) {
+template <bool QUEUE>
+void* pick_group_t(int G, bool comp) {
   if (comp)
-    return G <= 128 ? reinterpret_cast<void*>(&rct_group_kernel<128, true>)
-                    : reinterpret_cast<void*>(&rct_group_kernel<256, true>);
-  return G <= 128 ? reinterpret_cast<void*>(&rct_group_kernel<128, false>)
-                  : reinterpret_cast<void*>(&rct_group_kernel<256, false>);
+    return G <= 128 ? reinterpret_cast<void*>(&rct_group_kernel<128, true, QUEUE>)
+                    : reinterpret_cast<void*>(&rct_group_kernel<256, true, QUEUE>);
+  return G <= 128 ? reinterpret_cast<void*>(&rct_group_kernel<128, false, QUEUE>)
+                  : reinterpret_cast<void*>(&rct_group_kernel<256, false, QUEUE>);
+}
+
+void* pick_group(int G, bool comp, bool queue = false) {
+  return queue ? pick_group_t<true>(G, comp) : pick_group_t<false>(G, comp);
 }
 
 void* pick_global(bool comp) {
@@ -2047,7 +2052,7 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   r.small_target = std::max(64, std::min(r.small_target, 256));
   if (!r.grid_per_cu) r.grid_per_cu = d.grid_per_cu;
   if (!r.streams) r.streams = d.streams ? d.streams : 3;
-  r.streams = std::max(1, std::min(r.streams, 1 + otg_context::kAux));
+  r.streams = std::max(1, std::min(r.streams, otg_context::kAux));
   // the compensated kernel's ring entries are twice as large, and its
   // occupancy with the 512-slot ring (4 wavefronts per CU) leaves LDS
   // latency exposed: its default splits the streamed nets by ring size
@@ -2065,6 +2070,8 @@ otg_rct_tuning otg_rct_resolve_tuning(const otg_context* ctx, const otg_rct_tuni
   r.first_on_caller = r.first_on_caller ? 1 : 0;
   if (!r.small_grid_per_cu) r.small_grid_per_cu = d.small_grid_per_cu;
   r.small_grid_per_cu = std::max(0, r.small_grid_per_cu);
+  if (!r.group_alongside) r.group_alongside = d.group_alongside;
+  r.group_alongside = std::max(0, std::min(r.group_alongside, 16));
   return r;
 }
 
@@ -2138,26 +2145,41 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   P.group_max = plan->small_max;
   const int G = plan->group_cap;
   // ---- packed groups ------------------------------------------------------------
+  // Either first and alone on the caller's stream, dealt out to a grid that
+  // is resident from the start, or (group_alongside) from a queue on a stream
+  // of their own beside the streaming launches, so that the small nets fill
+  // the CUs that the largest nets' launches leave idle.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusActive;
+  const bool can_fork =
+      hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
+  const bool along = plan->num_groups > 0 && plan->tuning.group_alongside > 0 && can_fork;
+  RctParams QG = P;
+  int grid_g = 0;
   if (plan->num_groups > 0) {
-    RctParams Q = P;
     if (subset) {
       hipLaunchKernelGGL(rct_zero_kernel, dim3(1), dim3(64), 0, s, plan->d_sub_count, 1);
       hipLaunchKernelGGL(rct_subset_groups_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
                          count, subset, plan->num_nets, plan->d_net_group, plan->d_group_flag,
                          plan->d_groups, plan->d_sub_groups, plan->d_sub_count);
       OTG_HIP(ctx, hipGetLastError());
-      Q.items = plan->d_sub_groups;
-      Q.num_items = plan->num_groups;  // bound; the count is read on the device
-      Q.num_items_dev = plan->d_sub_count;
-      Q.group_flag = plan->d_group_flag;
+      QG.items = plan->d_sub_groups;
+      QG.num_items = plan->num_groups;  // bound; the count is read on the device
+      QG.num_items_dev = plan->d_sub_count;
+      QG.group_flag = plan->d_group_flag;
     } else {
-      Q.items = plan->d_groups;
-      Q.num_items = plan->num_groups;
+      QG.items = plan->d_groups;
+      QG.num_items = plan->num_groups;
     }
-    const int grid = std::min(plan->grid_group, subset ? std::min(count, plan->num_groups)
-                                                       : plan->num_groups);
-    void* args[] = {&Q};
-    OTG_HIP(ctx, hipLaunchKernel(pick_group(G, comp), dim3(std::max(1, grid)), dim3(64), args, 0, s));
+    const int most = subset ? std::min(count, plan->num_groups) : plan->num_groups;
+    if (along) {
+      QG.queue = plan->d_queue + 3 * otg_rct_plan::kMaxBins + 2;
+      grid_g = std::min(plan->tuning.group_alongside * std::max(1, ctx->num_cus), most);
+    } else {
+      grid_g = std::min(plan->grid_group, most);
+      void* args[] = {&QG};
+      OTG_HIP(ctx, hipLaunchKernel(pick_group(G, comp), dim3(std::max(1, grid_g)), dim3(64),
+                                   args, 0, s));
+    }
   }
   // ---- streamed nets, one launch per ring size (largest ring first), and the
   // nets whose parent offsets exceed every ring --------------------------------
@@ -2213,25 +2235,25 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
     const int gt = subset ? count : plan->num_gitems;
     L[nl++] = Launch{pick_global(comp), std::min(plan->grid, std::max(1, gt)), Q};
   }
-  // These launches take their nets from queues, so they may share the GPU
-  // (the packed-group kernel deals its groups out to a grid that it assumes
-  // resident, so it runs alone, before them).  Launch i goes to stream
-  // n-1-(i mod n) of n: the caller's stream and up to two auxiliary streams
-  // forked from it after the packed groups (one stream under graph
+  // These launches take their nets from queues, so they may share the GPU.
+  // Launch i goes to stream n-1-(i mod n) of n: the caller's stream and up
+  // to two auxiliary streams forked from it (one stream under graph
   // capture).  The launches then start together: one holds the CUs first,
   // and as its workgroups run out of nets the others take their place, so
   // the last, largest nets of one launch overlap the work of the others
-  // instead of leaving CUs idle.
-  hipStream_t st[1 + otg_context::kAux];
+  // instead of leaving CUs idle.  Packed groups that run beside them take
+  // the third auxiliary stream.
+  hipStream_t st[otg_context::kAux];
   int ns = 1;
   st[0] = s;
-  const int want = std::min(std::min(plan->tuning.streams, nl), 1 + otg_context::kAux);
-  hipStreamCaptureStatus cap = hipStreamCaptureStatusActive;
-  if (want > 1 && hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone) {
+  const int want = can_fork ? std::min(std::min(plan->tuning.streams, nl), otg_context::kAux) : 1;
+  const int naux = (want - 1) + (along ? 1 : 0);
+  hipStream_t gs = s;  // stream of packed groups that run beside the streaming launches
+  if (naux > 0) {
     if (!ctx->ev_fork)
       OTG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
-    for (int k = 0; k + 1 < want; ++k) {
-      if (!ctx->aux[k]) {
+    for (int k = 0; k < otg_context::kAux; ++k) {
+      if ((k < want - 1 || (along && k == otg_context::kAux - 1)) && !ctx->aux[k]) {
         OTG_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
         OTG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join[k], hipEventDisableTiming));
       }
@@ -2240,6 +2262,10 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
     for (int k = 0; k + 1 < want; ++k) {
       OTG_HIP(ctx, hipStreamWaitEvent(ctx->aux[k], ctx->ev_fork, 0));
       st[ns++] = ctx->aux[k];
+    }
+    if (along) {
+      gs = ctx->aux[otg_context::kAux - 1];
+      OTG_HIP(ctx, hipStreamWaitEvent(gs, ctx->ev_fork, 0));
     }
   }
   // (the smallest-ring launch, L[nl-1] unless global-window nets follow,
@@ -2253,15 +2279,22 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
     }
   }
   hipError_t err = hipSuccess;
+  if (along) {
+    void* args[] = {&QG};
+    err = hipLaunchKernel(pick_group(G, comp, true), dim3(std::max(1, grid_g)), dim3(64), args,
+                          0, gs);
+  }
   for (int i = 0; i < nl && err == hipSuccess; ++i) {
     void* args[] = {&L[i].Q};
     const int si = plan->tuning.first_on_caller ? i % ns : ns - 1 - i % ns;
     err = hipLaunchKernel(L[i].fn, dim3(L[i].grid), dim3(64), args, 0, st[si]);
   }
   // joined on every path, so that the caller's stream orders after all of it
-  for (int k = 1; k < ns; ++k) {
-    const hipError_t e1 = hipEventRecord(ctx->ev_join[k - 1], st[k]);
-    const hipError_t e2 = hipStreamWaitEvent(s, ctx->ev_join[k - 1], 0);
+  for (int k = 0; k < otg_context::kAux; ++k) {
+    const bool used = (k < ns - 1) || (along && k == otg_context::kAux - 1);
+    if (!used) continue;
+    const hipError_t e1 = hipEventRecord(ctx->ev_join[k], ctx->aux[k]);
+    const hipError_t e2 = hipStreamWaitEvent(s, ctx->ev_join[k], 0);
     if (err == hipSuccess) err = e1 != hipSuccess ? e1 : e2;
   }
   OTG_HIP(ctx, err);

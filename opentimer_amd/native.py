@@ -66,7 +66,8 @@ class RctTuning(C.Structure):
     _fields_ = [("ring_slots", C.c_int32), ("min_ring", C.c_int32),
                 ("small_target", C.c_int32), ("grid_per_cu", C.c_int32),
                 ("streams", C.c_int32), ("small_max", C.c_int32), ("ring_mask", C.c_int32),
-                ("first_on_caller", C.c_int32), ("small_grid_per_cu", C.c_int32)]
+                ("first_on_caller", C.c_int32), ("small_grid_per_cu", C.c_int32),
+                ("group_alongside", C.c_int32)]
 
 
 class RctEdges(C.Structure):
