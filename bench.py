@@ -862,6 +862,21 @@ def main():
         return
     ms_other = timed(other, max(3, args.steps // 2), 1)
     log(f"{other}: {ms_other:.3f} ms/step")
+    # ---- the streaming ceiling of a solve's traffic on this device: the same
+    # algorithmic bytes over the same arrays with no tree work
+    # (otg_rct_stream_ceiling; it overwrites the outputs, which the parity
+    # leg below solves again)
+    for _ in range(2):
+        s.stream_ceiling(stream=sh)
+    torch.cuda.synchronize()
+    ca, cb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ca.record(stream)
+    for _ in range(max(5, args.steps)):
+        s.stream_ceiling(stream=sh)
+    cb.record(stream)
+    torch.cuda.synchronize()
+    ceil_ms = ca.elapsed_time(cb) / max(5, args.steps)
+    log(f"streaming ceiling: {ceil_ms:.3f} ms ({alg_bytes / ceil_ms / 1e6:.0f} GB/s)")
     plain_ms = None
     if not args.no_plain and args.only in ("", "plain"):
         plain_ms = timed("plain", args.plain_steps, 1)
@@ -954,12 +969,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{cfg.name} (4M heavy-tailed RC trees, BFS order, 4 corners)",
                    "nets": Nn, "nodes": V, "method": args.method,
-                   "precision": ("fp32 with two-sum compensated subtree/path sums, fp64 final "
-                                 "per-node combination") if args.method == "compensated" else "fp32",
+                   "precision": ("fp32 inputs and outputs; subtree and path sums and the final "
+                                 "per-node combination in fp64") if args.method == "compensated" else "fp32",
                    "plan": {k: int(v) for k, v in plans[args.method]["stats"].items()},
                    "tuning": tuning_info[args.method]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "ceiling": {"ms": ceil_ms, "GBps": alg_bytes / ceil_ms / 1e6,
+                                 "frac_of_peak": alg_bytes / ceil_ms / 1e6 / PEAK_HBM_GBS,
+                                 "step_frac_of_ceiling": ceil_ms / ms,
+                                 "what": "otg_rct_stream_ceiling: the step's algorithmic bytes "
+                                         "(24 B in, 96 B out per node; 20 B per net) moved over "
+                                         "the same arrays, coalesced, with no tree work"},
                      "traffic_source": traffic_note, "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
         other: {"ms_per_step": ms_other, "nets_per_s": Nn / (ms_other / 1e3),

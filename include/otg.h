@@ -72,9 +72,10 @@ typedef enum otg_rct_method {
   OTG_RCT_PLAIN = 0,
   /* Size-binned persistent launch, FP32 arithmetic. */
   OTG_RCT_BATCHED = 1,
-  /* Size-binned persistent launch, FP32 with compensated (two-sum)
-   * accumulation of every subtree and path sum; FP64 only in the final
-   * per-node combination that forms beta, impulse and slew. */
+  /* Size-binned persistent launch, FP32 inputs and outputs with every
+   * subtree and path sum (load, L, delay, impulse) accumulated in FP64, and
+   * the final per-node combination that forms ldelay, beta and the slew
+   * radicand in FP64 (the slew's square root in FP32). */
   OTG_RCT_BATCHED_COMPENSATED = 2
 } otg_rct_method;
 
@@ -123,7 +124,7 @@ typedef struct otg_rct_tuning {
   int32_t grid_per_cu;    /* persistent workgroups per CU (0 = occupancy)   */
   int32_t streams;        /* streams the streaming launches of a solve (one */
                           /* per ring size, global-window nets) are spread  */
-                          /* over, 1..3 (default 3): forked from the        */
+                          /* over, 1..4 (default 3): forked from the        */
                           /* caller's stream after the packed groups and    */
                           /* joined back, so that one launch's last, largest*/
                           /* nets overlap the others' work; 1 under graph   */
@@ -173,6 +174,18 @@ otg_status otg_rct_plan_stats(const otg_rct_plan* plan, int64_t stats[8]);
 otg_status otg_rct_reference_f64(otg_context* ctx, const otg_rct_tree* tree, double* load,
                                  double* delay, double* ldelay, double* beta, double* impulse,
                                  double* slew, otg_stream stream);
+
+/* Measuring instrument, no reference counterpart: moves exactly the
+ * algorithmic bytes of one batched solve of `tree` (per node: pid, pres and
+ * the four capacitances read, the six outputs written; per net: the node
+ * offset and the four driver slews read) with coalesced accesses and no tree
+ * work, so that its time is the shortest a solve over the same arrays could
+ * take if memory traffic alone bounded it.  Overwrites every output array
+ * with a fixed function of the node inputs (load = cap, delay = cap * (pres
+ * + pid), ...) that has no timing meaning.  Same 16-byte alignment rule as
+ * otg_rct_solve.  Asynchronous on `stream`. */
+otg_status otg_rct_stream_ceiling(otg_context* ctx, const otg_rct_tree* tree,
+                                  const otg_rct_result* out, otg_stream stream);
 
 /* Solve P1 for every net.  Replaces the compute stage of
  * rct_bfs_and_compute_cuda (reference ot/cuda/rct.cu:385-389, kernel

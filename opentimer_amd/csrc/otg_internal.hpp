@@ -19,9 +19,9 @@ struct otg_context {
   double last_breakdown[8] = {};  // stages of the last host-pointer P1 call (ms)
   // Auxiliary streams that the launches of one batched solve are spread over
   // (forked from and joined back into the caller's stream), created on first
-  // use: up to two for streaming launches, one for packed groups that run
+  // use: up to three for streaming launches, one for packed groups that run
   // beside them.
-  static constexpr int kAux = 3;
+  static constexpr int kAux = 4;
   hipStream_t aux[kAux] = {};
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[kAux] = {};

@@ -2237,7 +2237,7 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   }
   // These launches take their nets from queues, so they may share the GPU.
   // Launch i goes to stream n-1-(i mod n) of n: the caller's stream and up
-  // to two auxiliary streams forked from it (one stream under graph
+  // to three auxiliary streams forked from it (one stream under graph
   // capture).  The launches then start together: one holds the CUs first,
   // and as its workgroups run out of nets the others take their place, so
   // the last, largest nets of one launch overlap the work of the others

@@ -199,6 +199,7 @@ _sig("otg_rct_plan_destroy", C.c_int, plan_p)
 _sig("otg_rct_plan_stats", C.c_int, plan_p, i64p)
 _sig("otg_rct_solve", C.c_int, ctx_p, plan_p, C.POINTER(RctTree), C.POINTER(RctResult), C.c_int, vp)
 _sig("otg_rct_reference_f64", C.c_int, ctx_p, C.POINTER(RctTree), vp, vp, vp, vp, vp, vp, vp)
+_sig("otg_rct_stream_ceiling", C.c_int, ctx_p, C.POINTER(RctTree), C.POINTER(RctResult), vp)
 _sig("otg_rct_solve_subset", C.c_int, ctx_p, plan_p, C.POINTER(RctTree), C.POINTER(RctResult),
      C.c_int, vp, C.c_int32, vp)
 _sig("otg_rct_bfs", C.c_int, ctx_p, C.POINTER(RctEdges), vp, vp, vp, vp, vp, vp)
@@ -266,7 +267,7 @@ _sig("otg_spef_destroy", C.c_int, spef_p)
 EXPORTED = [
     "otg_create", "otg_destroy", "otg_synchronize", "otg_status_string", "otg_last_error",
     "otg_version", "otg_rct_plan_create", "otg_rct_plan_destroy", "otg_rct_plan_stats",
-    "otg_rct_solve", "otg_rct_reference_f64", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
+    "otg_rct_solve", "otg_rct_reference_f64", "otg_rct_stream_ceiling", "otg_rct_solve_subset", "otg_rct_bfs", "otg_rct_bfs_and_compute",
     "otg_rct_compute", "otg_rct_query", "otg_rct_last_breakdown",
     "otg_rct_autotune", "otg_rct_tuning_load", "otg_toposort_compute", "otg_levelize", "otg_levelize_noalloc",
     "otg_incr_create", "otg_incr_destroy", "otg_incr_relevelize", "otg_incr_query",

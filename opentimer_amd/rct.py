@@ -127,6 +127,15 @@ class RcSolver:
                                            m, stream))
         return self.out
 
+    def stream_ceiling(self, stream=None):
+        """otg_rct_stream_ceiling over this forest and its output arrays: the
+        algorithmic bytes of one solve moved with no tree work (overwrites the
+        outputs)."""
+        stream = torch_stream(self.device) if stream is None else stream
+        self.ctx.check(N.lib.otg_rct_stream_ceiling(self.ctx.ptr, C.byref(self.tree()),
+                                                    C.byref(self.result()), stream))
+        return self.out
+
     def reference_f64(self, stream=None) -> dict:
         """FP64 reference values of every output (otg_rct_reference_f64): the
         reference's sweeps (ot/timer/net.cpp:101-193) in double on the device,

@@ -525,3 +525,31 @@ def test_batched_solves_reject_unaligned_corner_arrays(ctx):
     torch.cuda.synchronize()
     s.solve("plain")
     assert torch.equal(spare[1:4 * V + 1], s.out["delay"][:4 * V])
+
+
+def test_stream_ceiling_instrument(ctx):
+    """otg_rct_stream_ceiling writes its documented function of the node
+    inputs into every output (load = cap, delay = cap * (pres + pid), ...),
+    so every load it is timed on is real; nets' data are read, not written."""
+    f = W.generate_rc(W.config0(num_nets=700, seed=9))
+    s = RcSolver.from_edges(f, ctx)
+    s.alloc_outputs()
+    for k in OUT:
+        s.out[k].fill_(float("nan"))
+    s.stream_ceiling()
+    torch.cuda.synchronize()
+    t = s.host_topology()
+    g = s.host()
+    fv = (t["pres"] + t["pid"].astype(np.float32)).astype(np.float32)
+    c = t["cap"].reshape(-1, 4)
+    fc = fv[:, None]
+    want = {
+        "load": c,
+        "delay": c * fc,
+        "ldelay": c + fc,
+        "beta": c - fc,
+        "impulse": np.stack([fv, c[:, 0], c[:, 1], c[:, 2]], 1),
+        "slew": np.stack([c[:, 3], fv, c[:, 0], c[:, 1]], 1),
+    }
+    for k in OUT:
+        assert np.array_equal(g[k].reshape(-1, 4), want[k].astype(np.float32)), k
