@@ -16,7 +16,7 @@ otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_si
                                    const int32_t* fanin_start, const int32_t* fanin,
                                    int32_t* out_copy, int32_t* frontiers, int32_t* ends,
                                    int32_t* level, int32_t* scratch, int32_t* num_ends_host,
-                                   hipStream_t s);
+                                   hipStream_t s, bool pipelined);
 
 namespace {
 thread_local std::string g_last_error;  // errors raised before a context exists
@@ -96,6 +96,9 @@ extern "C" otg_status otg_destroy(otg_context* ctx) {
     if (ctx->ev_join[k]) (void)hipEventDestroy(ctx->ev_join[k]);
   }
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  for (hipEvent_t e : ctx->lv_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->lv_flag) (void)hipHostFree(ctx->lv_flag);
   delete ctx;
   return OTG_OK;
 }
@@ -500,7 +503,7 @@ extern "C" otg_status otg_levelize(otg_context* ctx, int32_t n, int32_t num_edge
                                 hipMemcpyDeviceToDevice, s));
   int32_t ne = 0;
   otg_status st = otg_levelize_launch_raw(ctx, n, first_size, fanin_start, fanin, out_copy, frontiers,
-                                          frontiers_ends, level_of_pin, scratch, &ne, s);
+                                          frontiers_ends, level_of_pin, scratch, &ne, s, true);
   OTG_HIP(ctx, hipFreeAsync(out_copy, s));
   OTG_HIP(ctx, hipFreeAsync(scratch, s));
   OTG_HIP(ctx, hipStreamSynchronize(s));

@@ -17,6 +17,8 @@ struct otg_context {
   std::string last_error;
   otg_rct_tuning tuning{};  // defaults for new plans (autotuner may update)
   double last_breakdown[8] = {};  // stages of the last host-pointer P1 call (ms)
+  int32_t* lv_flag = nullptr;      // pinned host copies of the levelization end flag [2]
+  hipEvent_t lv_ev[2] = {};        // their copy events
   // Auxiliary streams that the launches of one batched solve are spread over
   // (forked from and joined back into the caller's stream), created on first
   // use: up to three for streaming launches, one for packed groups that run

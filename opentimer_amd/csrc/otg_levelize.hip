@@ -305,7 +305,7 @@ otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_si
                                    const int32_t* fanin_start, const int32_t* fanin,
                                    int32_t* out_copy, int32_t* frontiers, int32_t* ends,
                                    int32_t* level, int32_t* scratch, int32_t* num_ends_host,
-                                   hipStream_t s) {
+                                   hipStream_t s, bool pipelined) {
   // scratch: [n+2] per-level append counters followed by num_ends
   LevelizeParams P{n, first_size, fanin_start, fanin, out_copy, frontiers, ends, level,
                    scratch, scratch + n + 2, nullptr, nullptr};
@@ -320,15 +320,57 @@ otg_status otg_levelize_launch_raw(otg_context* ctx, int32_t n, int32_t first_si
   // about one thread per pin on the wide levels, never more than the pins
   const int grid = std::max(1, std::min(4 * std::max(1, ctx->num_cus),
                                         (int)(((int64_t)n + kBlock - 1) / kBlock)));
+  // The end flag is checked once per batch of level launches, one batch
+  // behind: batch k+1 is queued before the host waits for batch k's flag
+  // (copied into pinned memory), so the GPU never idles at a check; the
+  // batch queued past the end runs as empty launches.
+  // Without `pipelined` (the entry point that allocates nothing: the pinned
+  // flags are allocated on first use) each batch is checked synchronously.
+  if (pipelined && !ctx->lv_flag) {
+    OTG_HIP(ctx, hipHostMalloc((void**)&ctx->lv_flag, 2 * sizeof(int32_t), hipHostMallocDefault));
+    for (int k = 0; k < 2; ++k)
+      OTG_HIP(ctx, hipEventCreateWithFlags(&ctx->lv_ev[k], hipEventDisableTiming));
+  }
   int32_t ne = INT32_MAX;
-  for (int lv0 = 0; lv0 <= n; lv0 += kBatch) {
-    for (int lv = lv0; lv < lv0 + kBatch && lv <= n; ++lv)
-      hipLaunchKernelGGL((levelize_level_kernel<LV_KAHN, kBlock>), dim3(grid), dim3(kBlock), 0, s,
-                         P, lv);
-    OTG_HIP(ctx, hipGetLastError());
-    OTG_HIP(ctx, hipMemcpyAsync(&ne, P.num_ends, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (!pipelined) {
+    for (int lv0 = 0; lv0 <= n && ne == INT32_MAX; lv0 += kBatch) {
+      for (int lv = lv0; lv < lv0 + kBatch && lv <= n; ++lv)
+        hipLaunchKernelGGL((levelize_level_kernel<LV_KAHN, kBlock>), dim3(grid), dim3(kBlock), 0,
+                           s, P, lv);
+      OTG_HIP(ctx, hipGetLastError());
+      OTG_HIP(ctx, hipMemcpyAsync(&ne, P.num_ends, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      OTG_HIP(ctx, hipStreamSynchronize(s));
+    }
+    if (ne == INT32_MAX) {
+      OTG_SET_ERR(ctx, "levelization did not terminate within n levels");
+      return OTG_ERR_CYCLE;
+    }
+    *num_ends_host = ne;
+    return OTG_OK;
+  }
+  int pending = -1;  // slot of the batch whose flag is in flight
+  for (int lv0 = 0, slot = 0;; lv0 += kBatch, slot ^= 1) {
+    if (lv0 <= n) {
+      for (int lv = lv0; lv < lv0 + kBatch && lv <= n; ++lv)
+        hipLaunchKernelGGL((levelize_level_kernel<LV_KAHN, kBlock>), dim3(grid), dim3(kBlock), 0,
+                           s, P, lv);
+      OTG_HIP(ctx, hipGetLastError());
+      OTG_HIP(ctx, hipMemcpyAsync(ctx->lv_flag + slot, P.num_ends, sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, s));
+      OTG_HIP(ctx, hipEventRecord(ctx->lv_ev[slot], s));
+    }
+    if (pending >= 0) {
+      OTG_HIP(ctx, hipEventSynchronize(ctx->lv_ev[pending]));
+      ne = ctx->lv_flag[pending];
+      if (ne != INT32_MAX) break;
+    }
+    if (lv0 > n) break;
+    pending = slot;
+  }
+  if (ne == INT32_MAX) {
+    // the last batch's flag (n+1 levels launched)
     OTG_HIP(ctx, hipStreamSynchronize(s));
-    if (ne != INT32_MAX) break;
+    OTG_HIP(ctx, hipMemcpy(&ne, P.num_ends, sizeof(int32_t), hipMemcpyDeviceToHost));
   }
   if (ne == INT32_MAX) {
     OTG_SET_ERR(ctx, "levelization did not terminate within n levels");
@@ -416,7 +458,7 @@ extern "C" otg_status otg_levelize_noalloc(otg_context* ctx, int32_t n, int32_t 
                                 hipMemcpyDeviceToDevice, s));
   int32_t ne = 0;
   st = otg_levelize_launch_raw(ctx, n, first_size, fanin_start, fanin, out_copy, frontiers,
-                               frontiers_ends, level_of_pin, counters, &ne, s);
+                               frontiers_ends, level_of_pin, counters, &ne, s, false);
   if (st != OTG_OK) return st;
   *num_ends = ne;
   return OTG_OK;
