@@ -271,20 +271,55 @@ __device__ __forceinline__ void fprop_one(const Src& S, const LibView& L, float*
 
 // Copies the library into LDS (the caller checked that it fits) and returns
 // the view of it; every thread of the workgroup takes part.
+// The library into LDS: the small arrays (table headers, timing entries)
+// as one index space of 4-byte words, the table pool as 16-byte vectors when
+// its address allows.  Every thread issues all its loads before its first
+// store (a store waits for its load: a loop of load-store pairs would pay one
+// memory latency per iteration), so a workgroup pays about one latency.
+constexpr int kStageSmall = 4;  // small-array words per thread and round
+constexpr int kStagePool = 8;   // pool vectors per thread and round
+
 __device__ __forceinline__ LibView stage_library(const otg_prop_lib& lib, int pool_words,
                                                  int32_t* lds) {
   const int nl = lib.num_luts, nt = lib.num_timings;
   const int o_tlut = 4 * nl, o_sense = o_tlut + 8 * nt, o_edge = o_sense + nt,
-            o_has = o_edge + nt, o_pool = o_has + 2 * nt;
-  for (int k = threadIdx.x; k < 4 * nl; k += blockDim.x) lds[k] = lib.lut_hdr[k];
-  for (int k = threadIdx.x; k < 8 * nt; k += blockDim.x) lds[o_tlut + k] = lib.timing_lut[k];
-  for (int k = threadIdx.x; k < nt; k += blockDim.x) {
-    lds[o_sense + k] = lib.timing_sense[k];
-    lds[o_edge + k] = lib.timing_edge[k];
-  }
-  for (int k = threadIdx.x; k < 2 * nt; k += blockDim.x) lds[o_has + k] = lib.timing_has[k];
+            o_has = o_edge + nt, o_pool = o_has + 2 * nt;  // o_pool: a multiple of 4
   const int32_t* pool = reinterpret_cast<const int32_t*>(lib.lut_val);
-  for (int k = threadIdx.x; k < pool_words; k += blockDim.x) lds[o_pool + k] = pool[k];
+  const bool vec = (reinterpret_cast<uintptr_t>(pool) & 15u) == 0;
+  const int nvec = vec ? pool_words / 4 : 0;
+  const int B = blockDim.x, tid = threadIdx.x;
+  for (int sb = tid, pb = tid; sb < o_pool || pb < nvec; sb += kStageSmall * B, pb += kStagePool * B) {
+    int32_t rs[kStageSmall];
+    int4 rp[kStagePool];
+#pragma unroll
+    for (int u = 0; u < kStageSmall; ++u) {
+      const int k = sb + u * B;
+      rs[u] = 0;
+      if (k < o_tlut) rs[u] = lib.lut_hdr[k];
+      else if (k < o_sense) rs[u] = lib.timing_lut[k - o_tlut];
+      else if (k < o_edge) rs[u] = lib.timing_sense[k - o_sense];
+      else if (k < o_has) rs[u] = lib.timing_edge[k - o_edge];
+      else if (k < o_pool) rs[u] = lib.timing_has[k - o_has];
+    }
+#pragma unroll
+    for (int u = 0; u < kStagePool; ++u) {
+      const int k = pb + u * B;
+      rp[u] = k < nvec ? reinterpret_cast<const int4*>(pool)[k] : make_int4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kStageSmall; ++u) {
+      const int k = sb + u * B;
+      if (k < o_pool) lds[k] = rs[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kStagePool; ++u) {
+      const int k = pb + u * B;
+      if (k < nvec) reinterpret_cast<int4*>(lds + o_pool)[k] = rp[u];
+    }
+  }
+  // pool words past the vectors (all of them when the pool is not 16-byte
+  // aligned)
+  for (int k = 4 * nvec + tid; k < pool_words; k += B) lds[o_pool + k] = pool[k];
   __syncthreads();
   return LibView{lds, reinterpret_cast<const float*>(lds + o_pool), lds + o_tlut,
                  lds + o_sense, lds + o_edge, lds + o_has};
@@ -304,7 +339,7 @@ __global__ __launch_bounds__(1024) void fprop_level_kernel(FpropParams P, int lv
   const int64_t total = 4LL * (e - b);
   LibView L = device_library(P.lib);
   if constexpr (kLds) {
-    __shared__ int32_t lds[kLdsWords];
+    __shared__ __attribute__((aligned(16))) int32_t lds[kLdsWords];
     if ((int64_t)blockIdx.x * blockDim.x >= total) return;
     L = stage_library(P.lib, P.pool_words, lds);
   }
@@ -390,7 +425,7 @@ __global__ __launch_bounds__(1024) void fprop_wave_level_kernel(PlanParams P, in
   const int c0 = P.pv.wstart[lv], nch = P.pv.wstart[lv + 1] - c0;
   LibView L = device_library(P.lib);
   if constexpr (kLds) {
-    __shared__ int32_t lds[kLdsWords];
+    __shared__ __attribute__((aligned(16))) int32_t lds[kLdsWords];
     if ((int)blockIdx.x >= nch) return;
     L = stage_library(P.lib, P.pool_words, lds);
   }
@@ -603,6 +638,10 @@ __device__ __forceinline__ void rat_pick(float& v, int& i, float v2, int i2, int
 }
 
 constexpr int kBpropSmall = 8;  // fanout arcs a pin's own thread walks; more go to a wavefront
+constexpr int kBpropStep = 4;   // arcs whose loads a pin's own thread issues together
+constexpr int kBpropWide = 16;  // arcs per arc lane whose loads a wavefront issues together
+                                // (256 arcs per round: a fanout of 512 takes two rounds of
+                                // three dependent loads; 12.8 -> 11.2 ms against 4 per lane)
 
 // Levels first to last, one launch per level: workgroups of 256 threads take
 // the level's (pin, corner) items in chunks of 256; a pin with more than
@@ -628,21 +667,24 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
         if (c == 0) queue[atomicAdd(&qn, 1)] = u;
       } else {
         float r = G.po_rat[4 * u + c];
-        // 4 arcs per step: their ids, then sinks and delays, then the sinks'
-        // required times are loaded for all 4 before the relaxation
-        for (int k = k0; k < k1; k += 4) {
-          int j[4];
-          float2 d[4], rv[4];
+        // kBpropStep arcs per step: their ids, then sinks and delays, then
+        // the sinks' required times are loaded for all of them before the
+        // relaxation in fanout order
+        for (int k = k0; k < k1; k += kBpropStep) {
+          int j[kBpropStep];
+          float2 d[kBpropStep], rv[kBpropStep];
 #pragma unroll
-          for (int z = 0; z < 4; ++z) j[z] = G.fanout_arc[k + z < k1 ? k + z : k0];
+          for (int z = 0; z < kBpropStep; ++z) j[z] = G.fanout_arc[k + z < k1 ? k + z : k0];
 #pragma unroll
-          for (int z = 0; z < 4; ++z) {
+          for (int z = 0; z < kBpropStep; ++z) {
+            // (past the pin's last arc these load its first arc again: loads
+            // without a branch around them)
             const int v = G.arc_to[j[z]];
             d[z] = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j[z] + 4 * el + 2 * frf);
             rv[z] = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
           }
 #pragma unroll
-          for (int z = 0; z < 4; ++z) {
+          for (int z = 0; z < kBpropStep; ++z) {
             if (k + z >= k1) break;
             if (!__builtin_isnan(d[z].x) && !__builtin_isnan(rv[z].x))
               r = relax_rat(r, rv[z].x - d[z].x, el);
@@ -667,19 +709,20 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
       const int k0 = G.fanout_start[u], k1 = G.fanout_start[u + 1];
       float r = a0 == 0 ? G.po_rat[4 * u + c] : __builtin_nanf("");
       int ri = -1;
-      for (int kb = k0 + a0; kb < k1; kb += 64) {
-        int j[4];
-        float2 d[4], rv[4];
+      for (int kb = k0 + a0; kb < k1; kb += 16 * kBpropWide) {
+        int j[kBpropWide];
+        float2 d[kBpropWide], rv[kBpropWide];
 #pragma unroll
-        for (int z = 0; z < 4; ++z) j[z] = G.fanout_arc[kb + 16 * z < k1 ? kb + 16 * z : k0];
+        for (int z = 0; z < kBpropWide; ++z)
+          j[z] = G.fanout_arc[kb + 16 * z < k1 ? kb + 16 * z : k0];
 #pragma unroll
-        for (int z = 0; z < 4; ++z) {
+        for (int z = 0; z < kBpropWide; ++z) {
           const int v = G.arc_to[j[z]];
           d[z] = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j[z] + 4 * el + 2 * frf);
           rv[z] = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
         }
 #pragma unroll
-        for (int z = 0; z < 4; ++z) {
+        for (int z = 0; z < kBpropWide; ++z) {
           const int k = kb + 16 * z;
           if (k >= k1) break;
           if (!__builtin_isnan(d[z].x) && !__builtin_isnan(rv[z].x))
