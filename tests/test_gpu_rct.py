@@ -436,32 +436,45 @@ def test_reference_f64_matches_fp64_oracle(ctx):
         assert rel_err(ref[k], r64[k], fl[k]) < tol, k
 
 
+# launch arrangements of one solve: streams for the streaming launches (1, 3,
+# 4), packed groups first or from a queue beside them (group_alongside 1 or
+# 4 workgroups per CU), the largest ring on the caller's stream
+ARRANGEMENTS = [
+    (0, 0, 0, 0, 1),
+    (0, 0, 0, 0, 3),
+    (0, 0, 0, 0, 4),
+    (0, 0, 0, 0, 3, 0, 0, 0, 0, 1),
+    (0, 0, 0, 0, 3, 0, 0, 0, 0, 4),
+    (0, 0, 0, 0, 4, 0, 0, 1, 0, 2),
+]
+
+
 @pytest.mark.parametrize("method", ["batched", "compensated"])
-def test_streams_do_not_change_results(ctx, method):
-    # the streaming launches spread over 1 or 3 streams give identical bits
-    # (every net is solved by one workgroup, whatever runs beside it), for
-    # full and subset solves
+def test_launch_arrangements_do_not_change_results(ctx, method):
+    # every arrangement gives identical bits (every net is solved by one
+    # workgroup with the same code, whatever runs beside it), for full and
+    # subset solves
     cfg = W.RcConfig("mixstreams", 2500, [W.SizeClass(0.7, 2, 40), W.SizeClass(0.2, 100, 3000, N.SHAPE_CHAIN, 0.1),
                                           W.SizeClass(0.05, 200, 3000, N.SHAPE_CHAIN, 0.0),
                                           W.SizeClass(0.05, 600, 2500, N.SHAPE_BALANCED)], seed=43)
     f, s = _forest(cfg, ctx)
-    out = {}
-    for streams in (1, 3):
-        tun = (0, 0, 0, 0, streams)
-        s.solve(method, tuning=tun)
-        out[streams] = {k: v.clone() for k, v in s.out.items()}
-    st = s.plan_stats(method, (0, 0, 0, 0, 3))
-    assert st["stream_nets"] > 0
-    for k in OUT:
-        assert torch.equal(out[1][k], out[3][k]), k
+    st = s.plan_stats(method, ARRANGEMENTS[1])
+    assert st["stream_nets"] > 0 and st["packed_groups"] > 0
     ids = torch.arange(0, cfg.num_nets, 7, dtype=torch.int32, device=s.device)
-    for streams in (1, 3):
+    full, sub = [], []
+    for tun in ARRANGEMENTS:
+        for k in OUT:
+            s.out[k].fill_(float("nan"))
+        s.solve(method, tuning=tun)
+        full.append({k: v.clone() for k, v in s.out.items()})
         for k in OUT:
             s.out[k].zero_()
-        s.solve_subset(ids, method, tuning=(0, 0, 0, 0, streams))
-        out[streams] = {k: v.clone() for k, v in s.out.items()}
-    for k in OUT:
-        assert torch.equal(out[1][k], out[3][k]), k
+        s.solve_subset(ids, method, tuning=tun)
+        sub.append({k: v.clone() for k, v in s.out.items()})
+    for i, tun in enumerate(ARRANGEMENTS[1:], 1):
+        for k in OUT:
+            assert torch.equal(full[0][k], full[i][k]), (tun, k)
+            assert torch.equal(sub[0][k], sub[i][k]), (tun, k)
 
 
 @pytest.mark.parametrize("tun", [
