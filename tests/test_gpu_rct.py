@@ -461,6 +461,7 @@ def test_launch_arrangements_do_not_change_results(ctx, method):
     st = s.plan_stats(method, ARRANGEMENTS[1])
     assert st["stream_nets"] > 0 and st["packed_groups"] > 0
     ids = torch.arange(0, cfg.num_nets, 7, dtype=torch.int32, device=s.device)
+    s.alloc_outputs()
     full, sub = [], []
     for tun in ARRANGEMENTS:
         for k in OUT:
