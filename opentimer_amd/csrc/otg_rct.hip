@@ -2246,7 +2246,10 @@ otg_status otg_rct_launch_batched(otg_context* ctx, otg_rct_plan* plan, const ot
   hipStream_t st[otg_context::kAux];
   int ns = 1;
   st[0] = s;
-  const int want = can_fork ? std::min(std::min(plan->tuning.streams, nl), otg_context::kAux) : 1;
+  // (at least one: with no streamed nets, packed groups beside them still
+  // take their own stream)
+  const int want =
+      can_fork ? std::max(1, std::min(std::min(plan->tuning.streams, nl), otg_context::kAux)) : 1;
   const int naux = (want - 1) + (along ? 1 : 0);
   hipStream_t gs = s;  // stream of packed groups that run beside the streaming launches
   if (naux > 0) {

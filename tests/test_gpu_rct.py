@@ -478,6 +478,26 @@ def test_launch_arrangements_do_not_change_results(ctx, method):
             assert torch.equal(sub[0][k], sub[i][k]), (tun, k)
 
 
+@pytest.mark.parametrize("method", ["batched", "compensated"])
+def test_groups_beside_no_streamed_nets(ctx, method):
+    # packed groups from a queue on their own stream when the forest has no
+    # net larger than a group (no streaming launch to run beside): same bits
+    cfg = W.RcConfig("smallonly", 3000, [W.SizeClass(1.0, 2, 60)], seed=44)
+    f, s = _forest(cfg, ctx)
+    assert s.plan_stats(method)["stream_nets"] == 0
+    s.alloc_outputs()
+    ids = torch.arange(0, cfg.num_nets, 5, dtype=torch.int32, device=s.device)
+    res = []
+    for tun in ((0, 0, 0, 0, 3), (0, 0, 0, 0, 3, 0, 0, 0, 0, 2)):
+        for k in OUT:
+            s.out[k].fill_(float("nan"))
+        s.solve(method, tuning=tun)
+        s.solve_subset(ids, method, tuning=tun)
+        res.append({k: v.clone() for k, v in s.out.items()})
+    for k in OUT:
+        assert torch.equal(res[0][k], res[1][k]), k
+
+
 @pytest.mark.parametrize("tun", [
     (0, 0, 0, 0, 3, 64),             # nets of 65..256 nodes streamed instead of packed
     (0, 0, 0, 0, 3, 0, 0b11101),     # no 320-slot ring
