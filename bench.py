@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-levelize", action="store_true")
     ap.add_argument("--no-fprop", action="store_true")
+    ap.add_argument("--no-timer", action="store_true",
+                    help="skip the device timing update leg (otg_timer on the cfg2 design)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-autotune", action="store_true",
@@ -54,7 +56,7 @@ def parse():
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip the host-pointer drop-in leg (otg_rct_bfs_and_compute at cfg1)")
     ap.add_argument("--only", default="",
-                    help="rct | plain | levelize | fprop | copy (profiling runs) | precision (cfg3 report) "
+                    help="rct | plain | levelize | fprop | timer | copy (profiling runs) | precision (cfg3 report) "
                          "| incremental (cfg4 report) | spef (SPEF loader + GPU solve)")
     ap.add_argument("--spef-nets", type=int, default=400_000,
                     help="nets of the cfg1 distribution written as SPEF for --only spef")
@@ -513,6 +515,114 @@ def run_incremental(ctx, method, pins=5_000_000, nets=1_000_000):
     return res
 
 
+def run_timer(ctx, method, reps=5):
+    """Timer::update_timing on the device (otg_timer) over a whole design: the
+    cfg2 graph (10M pins, 25M arcs, 200 levels) whose pins with fanout drive
+    RC nets with probability 0.4 (about 4M nets of the cfg1 size
+    distribution), the synthetic NLDM library, primary inputs and outputs on
+    the graph's boundary.  The first update also orders the RC trees,
+    levelizes the graph and builds the plans; the later ones (timed with HIP
+    events on the launch stream, median of `reps`) re-solve every net, gather
+    the net-arc values and propagate forward and backward.  Consistency: the
+    RC delays the timer gathers at 2,000 sampled net-arc sinks equal, bit for
+    bit, an independent solve of the same forest queried at the same pins
+    (otg_rct_query)."""
+    import ctypes as C
+
+    import torch
+
+    from opentimer_amd import native as N
+    from opentimer_amd import workloads as W
+    from opentimer_amd.rct import RcSolver
+    t0 = time.perf_counter()
+    lib = W.prop_library()
+    D = W.timer_design(W.config2(), lib, nthreads=16)
+    f = D.forest
+    log(f"timer design: {f.num_nets} nets {f.num_nodes} nodes, {len(D.fanin)} arcs, "
+        f"generated in {time.perf_counter() - t0:.1f}s")
+    dev = torch.device("cuda", 0)
+    tt = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    T = {"nodes_start": tt(f.nodes_start), "edges": tt(f.edges), "roots": tt(f.roots),
+         "edge_res": tt(f.edge_res), "node_cap": tt(f.node_cap), "fanin_start": tt(D.fanin_start),
+         "fanin": tt(D.fanin), "arc_kind": tt(D.arc_kind), "arc_timing": tt(D.arc_timing),
+         "arc_net": tt(D.arc_net), "arc_node": tt(D.arc_node), "pin_net": tt(D.pin_net),
+         "pin_load": tt(D.pin_load), "pi_slew": tt(D.pi_slew), "pi_at": tt(D.pi_at),
+         "po_rat": tt(D.po_rat)}
+    LB = {k: tt(getattr(lib, k)) for k in ("lut_hdr", "lut_val", "timing_sense", "timing_edge",
+                                            "timing_lut", "timing_has")}
+    a = lambda d, k: d[k].data_ptr()  # noqa: E731
+    n, m = len(D.pin_net), len(D.fanin)
+    design = N.TimerDesign(f.num_nets, f.num_nodes, a(T, "nodes_start"), a(T, "edges"),
+                           a(T, "roots"), a(T, "edge_res"), a(T, "node_cap"), n, m,
+                           a(T, "fanin_start"), a(T, "fanin"), a(T, "arc_kind"),
+                           a(T, "arc_timing"), a(T, "arc_net"), a(T, "arc_node"),
+                           a(T, "pin_net"), a(T, "pin_load"), a(T, "pi_slew"), a(T, "pi_at"),
+                           a(T, "po_rat"))
+    plib = N.PropLib(len(lib.lut_hdr) // 4, len(lib.timing_sense), len(lib.lut_val),
+                     a(LB, "lut_hdr"), a(LB, "lut_val"), a(LB, "timing_sense"),
+                     a(LB, "timing_edge"), a(LB, "timing_lut"), a(LB, "timing_has"))
+    meth = {"batched": N.BATCHED, "compensated": N.BATCHED_COMPENSATED}[method]
+    timer = N.timer_p()
+    ctx.check(N.lib.otg_timer_create(ctx.ptr, C.byref(design), C.byref(plib), meth,
+                                     C.byref(timer)))
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    try:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.check(N.lib.otg_timer_update(timer, sh))
+        torch.cuda.synchronize()
+        first_ms = (time.perf_counter() - t1) * 1e3
+        ctx.check(N.lib.otg_timer_update(timer, sh))
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            ctx.check(N.lib.otg_timer_update(timer, sh))
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        upd_ms = float(np.median(ms))
+        r = N.TimerResults()
+        ctx.check(N.lib.otg_timer_results_get(timer, C.byref(r)))
+        levels = int(r.num_levels)
+        # consistency of the gathered RC delays with an independent solve
+        rng = np.random.default_rng(9)
+        rc_arcs = np.flatnonzero(D.arc_net >= 0)
+        pick = np.sort(rng.choice(rc_arcs, min(2000, len(rc_arcs)), replace=False))
+        gathered = torch.empty(4 * m, dtype=torch.float32, device=dev)
+        gathered.copy_(torch.as_tensor(_DevView(r.arc_rc_delay, 4 * m), device="cuda"))
+        got = gathered.view(m, 4)[torch.from_numpy(pick).to(dev)].cpu().numpy()
+    finally:
+        N.lib.otg_timer_destroy(timer)
+    del T
+    s = RcSolver.from_edges(f, ctx)
+    s.solve(method)
+    q = s.query(torch.from_numpy(D.arc_net[pick]).to(dev), torch.from_numpy(D.arc_node[pick]).to(dev))
+    ref = q["delay"].cpu().numpy()
+    same = bool(np.array_equal(got.view(np.int32), ref.view(np.int32)))
+    s.close()
+    return {"workload": f"cfg2 graph ({n} pins, {m} arcs, {levels} levels) with {f.num_nets} RC nets "
+                        f"({f.num_nodes} nodes, cfg1 sizes) on the pins that drive nets; synthetic "
+                        "7x8 NLDM library", "method": method, "pins": n, "arcs": m,
+            "nets": int(f.num_nets), "nodes": int(f.num_nodes), "levels": levels,
+            "first_update_ms": first_ms, "update_ms": upd_ms,
+            "updates_per_s": 1e3 / upd_ms,
+            "what": "otg_timer_update: RC solve of every net, net-arc delay/impulse and cell loads "
+                    "gathered from it, forward propagation, backward propagation (the first "
+                    "update also BFS-orders, levelizes and builds the plans)",
+            "gathered_rc_delay_equals_independent_solve": same}
+
+
+class _DevView:
+    """A raw device array seen by torch through __cuda_array_interface__."""
+
+    def __init__(self, ptr, count):
+        self.__cuda_array_interface__ = {"shape": (count,), "typestr": "<f4",
+                                         "data": (ptr, False), "version": 3}
+
+
 def run_spef(ctx, nets, method):
     """SPEF -> flattened forest (host loader, all host threads) -> GPU BFS
     ordering -> solve.  The SPEF is written from a cfg1-distribution forest
@@ -759,6 +869,9 @@ def main():
     if args.only == "spef":
         print(json.dumps(run_spef(ctx, args.spef_nets, args.method)))
         return
+    if args.only == "timer":
+        print(json.dumps(run_timer(ctx, args.method)))
+        return
     if args.only == "copy":
         # a plain device copy of known size: the yardstick that converts the
         # FETCH_SIZE / WRITE_SIZE counters of the RC kernel into HBM bytes
@@ -936,6 +1049,10 @@ def main():
     fprop = None
     if not args.no_fprop:
         fprop = run_fprop(ctx, stream, sh, cpu=not args.no_cpu)
+    timer_leg = None
+    if not args.no_timer:
+        timer_leg = run_timer(ctx, args.method)
+        log(f"timer: {timer_leg}")
 
     achieved = alg_bytes / (ms / 1e3) / 1e9
     # HBM traffic per step from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
@@ -1002,6 +1119,7 @@ def main():
         "plan_build_ms": {m: plans[m]["build_ms"] for m in plans},
         "levelize": lev,
         "fprop": fprop,
+        "timer": timer_leg,
         "parity": parity,
     }
     print(json.dumps(out))

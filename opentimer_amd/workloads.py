@@ -360,3 +360,72 @@ def prop_graph(csr: "FaninCsr", lib: PropLibrary, seed: int = 12, net_fraction: 
     return PropGraphData(np.ascontiguousarray(csr.start, np.int32),
                          np.ascontiguousarray(csr.fanin, np.int32), kind, timing, rcd, rci,
                          load, pis, pia)
+
+
+# ---------------------------------------------------------------------------
+# A whole design for the device timer (include/otg_timer.h): a timing graph
+# whose net arcs lead into RC trees.
+# ---------------------------------------------------------------------------
+
+@dataclass
+class TimerDesignData:
+    forest: RcForest          # RC trees, original numbering, root = node 0's driver
+    fanin_start: np.ndarray   # [n+1]
+    fanin: np.ndarray         # [m] source pin of arc j
+    arc_kind: np.ndarray      # [m] OTG_ARC_NET_RC (0) or OTG_ARC_CELL (2)
+    arc_timing: np.ndarray    # [m]
+    arc_net: np.ndarray       # [m] RC net of a net arc, else -1
+    arc_node: np.ndarray      # [m] sink node of a net arc in its net, else -1
+    pin_net: np.ndarray       # [n] RC net the pin drives, else -1
+    pin_load: np.ndarray      # [n, 4]
+    pi_slew: np.ndarray       # [n, 4]
+    pi_at: np.ndarray         # [n, 4]
+    po_rat: np.ndarray        # [n, 4]
+
+
+def timer_design(dag_cfg: DagConfig, lib: PropLibrary, seed: int = 31, net_fraction: float = 0.4,
+                 nthreads: int = 0) -> TimerDesignData:
+    """A design over `dag_cfg`'s graph: a pin with fanout drives an RC net with
+    probability `net_fraction` (all its fanout arcs are then net arcs of that
+    net; every other arc is a cell arc of a random timing).  The nets come
+    from the cfg1 size distribution, the largest nets given to the pins with
+    the largest fanout; the k-th fanout arc of a driver leads to node 1 + (k
+    mod (size - 1)) of its tree (distinct sinks while the tree has more nodes
+    than the driver has fanout arcs).  Pins without fanin are primary inputs
+    (slew 5..20 ps, arrival 0..10 ps), pins without fanout primary outputs
+    (required time 3000..6000 ps); cell-arc loads of pins driving no RC net
+    are 1..200 fF."""
+    rng = np.random.default_rng(seed)
+    csr = fanin_csr(generate_dag(dag_cfg, nthreads=nthreads))
+    n, m = csr.n, len(csr.fanin)
+    od = csr.out_degree
+    drivers = np.flatnonzero((od > 0) & (rng.random(n) < net_fraction)).astype(np.int32)
+    forest = generate_rc(config1(num_nets=len(drivers), seed=seed), nthreads=nthreads)
+    sizes = np.diff(forest.nodes_start)
+    pin_net = np.full(n, -1, np.int32)
+    pin_net[drivers[np.argsort(-od[drivers], kind="stable")]] = \
+        np.argsort(-sizes, kind="stable").astype(np.int32)
+    src = csr.fanin
+    arc_net = pin_net[src]
+    order = np.argsort(src, kind="stable")
+    s_sorted = src[order]
+    rank = np.empty(m, np.int64)
+    rank[order] = np.arange(m) - np.searchsorted(s_sorted, s_sorted, side="left")
+    size_of = sizes[np.maximum(arc_net, 0)].astype(np.int64)
+    rc = arc_net >= 0
+    arc_node = np.where(rc, 1 + rank % np.maximum(size_of - 1, 1), -1).astype(np.int32)
+    arc_kind = np.where(rc, 0, 2).astype(np.int32)
+    arc_timing = rng.integers(0, max(lib.num_timings, 1), size=m, dtype=np.int32)
+    pin_load = rng.uniform(1.0, 200.0, size=(n, 4)).astype(np.float32)
+    deg = np.diff(csr.start)
+    pis = np.full((n, 4), np.nan, np.float32)
+    pia = np.full((n, 4), np.nan, np.float32)
+    pi = deg == 0
+    pis[pi] = rng.uniform(5.0, 20.0, size=(int(pi.sum()), 4)).astype(np.float32)
+    pia[pi] = rng.uniform(0.0, 10.0, size=(int(pi.sum()), 4)).astype(np.float32)
+    po = np.full((n, 4), np.nan, np.float32)
+    sink = od == 0
+    po[sink] = rng.uniform(3000.0, 6000.0, size=(int(sink.sum()), 4)).astype(np.float32)
+    return TimerDesignData(forest, np.ascontiguousarray(csr.start, np.int32),
+                           np.ascontiguousarray(src, np.int32), arc_kind, arc_timing, arc_net,
+                           arc_node, pin_net, pin_load, pis, pia, po)

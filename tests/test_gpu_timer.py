@@ -207,3 +207,72 @@ def test_timer_update_after_the_caps_change(ctx):
         N.lib.otg_timer_destroy(t1)
         N.lib.otg_timer_destroy(t2)
         D["forest"].close()
+
+
+def test_timer_synthetic_design(ctx):
+    """A generated design (workloads.timer_design: a layered graph whose
+    driving pins own RC trees of the cfg1 size distribution): the RC delays
+    and impulses the timer gathers at every net-arc sink equal an independent
+    solve of the same forest queried at those pins (otg_rct_query), and the
+    timer's slews, arrivals, arc delays, required times and slacks equal the
+    oracle's propagation over the timer's own levels and gathered values, bit
+    for bit."""
+    from opentimer_amd import workloads as W
+    lib = W.prop_library()
+    D = W.timer_design(W.DagConfig("timer-syn", 30_000, 75_000, 30, seed=5), lib, seed=6)
+    f = D.forest
+    n, m = len(D.pin_net), len(D.fanin)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    T = {k: t(v) for k, v in (("nodes_start", f.nodes_start), ("edges", f.edges), ("roots", f.roots),
+                              ("edge_res", f.edge_res), ("node_cap", f.node_cap),
+                              ("fanin_start", D.fanin_start), ("fanin", D.fanin),
+                              ("arc_kind", D.arc_kind), ("arc_timing", D.arc_timing),
+                              ("arc_net", D.arc_net), ("arc_node", D.arc_node),
+                              ("pin_net", D.pin_net), ("pin_load", D.pin_load),
+                              ("pi_slew", D.pi_slew), ("pi_at", D.pi_at), ("po_rat", D.po_rat))}
+    lk = ("lut_hdr", "lut_val", "timing_sense", "timing_edge", "timing_lut", "timing_has")
+    LB = {k: t(getattr(lib, k)) for k in lk}
+    a = lambda d, k: d[k].data_ptr()  # noqa: E731
+    design = N.TimerDesign(f.num_nets, f.num_nodes, a(T, "nodes_start"), a(T, "edges"),
+                           a(T, "roots"), a(T, "edge_res"), a(T, "node_cap"), n, m,
+                           a(T, "fanin_start"), a(T, "fanin"), a(T, "arc_kind"),
+                           a(T, "arc_timing"), a(T, "arc_net"), a(T, "arc_node"),
+                           a(T, "pin_net"), a(T, "pin_load"), a(T, "pi_slew"), a(T, "pi_at"),
+                           a(T, "po_rat"))
+    plib = N.PropLib(len(lib.lut_hdr) // 4, len(lib.timing_sense), len(lib.lut_val),
+                     *[a(LB, k) for k in ("lut_hdr", "lut_val", "timing_sense", "timing_edge",
+                                          "timing_lut", "timing_has")])
+    timer = N.timer_p()
+    ctx.check(N.lib.otg_timer_create(ctx.ptr, C.byref(design), C.byref(plib),
+                                     N.BATCHED_COMPENSATED, C.byref(timer)))
+    try:
+        ctx.check(N.lib.otg_timer_update(timer, None))
+        torch.cuda.synchronize()
+        R = _results(timer, n, m)
+    finally:
+        N.lib.otg_timer_destroy(timer)
+    rc = D.arc_net >= 0
+    assert rc.any() and (~rc).any()
+    s = RcSolver.from_edges(f, ctx)
+    s.solve("compensated")
+    q = s.query(t(D.arc_net[rc]), t(D.arc_node[rc]))
+    _same(R["arc_rc_delay"].reshape(-1, 4)[rc], q["delay"].cpu().numpy())
+    # a driving pin carries its net's load (every pin of a net reports it)
+    drv = np.flatnonzero(D.pin_net >= 0)
+    ql = s.query(t(D.pin_net[drv]), t(np.zeros(len(drv), np.int32)))["load"]
+    _same(R["pin_load"].reshape(-1, 4)[drv], ql.cpu().numpy())
+    s.close()
+    g = {"fanin_start": D.fanin_start, "fanin": D.fanin, "arc_kind": D.arc_kind,
+         "arc_timing": D.arc_timing, "arc_rc_delay": R["arc_rc_delay"].reshape(-1, 4),
+         "arc_rc_impulse": R["arc_rc_impulse"].reshape(-1, 4),
+         "pin_load": R["pin_load"].reshape(-1, 4), "pi_slew": D.pi_slew, "pi_at": D.pi_at}
+    fr, ends = R["frontiers"], R["ends"]
+    assert ends[-1] == n
+    os_, oa, od = O.fprop(g, {k: getattr(lib, k) for k in lk}, fr, ends)
+    _same(R["slew"].reshape(-1, 4), os_)
+    _same(R["at"].reshape(-1, 4), oa)
+    _same(R["arc_delay"].reshape(-1, 8), od.reshape(-1, 8))
+    orat, oslack = O.bprop(D.fanin_start, D.fanin, od, D.po_rat, oa, fr, ends)
+    _same(R["rat"].reshape(-1, 4), orat)
+    _same(R["slack"].reshape(-1, 4), oslack)
