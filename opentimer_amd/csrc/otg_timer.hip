@@ -33,7 +33,9 @@ struct otg_timer {
   // RC forest in BFS order
   int32_t* node2bfs = nullptr;  // [V]
   int32_t* pid = nullptr;       // [V]
-  int32_t* edges_dir = nullptr; // [2E] oriented: (parent, child), original ids
+  int32_t* edges_dir = nullptr; // [2E] oriented: (parent, child), original ids (build only)
+  int32_t* src_pos = nullptr;   // [V] node (global original index) at each BFS position
+  int32_t* res_edge = nullptr;  // [V] edge whose child sits at each BFS position; -1 at a root
   float* pres = nullptr;        // [V]
   float* cap_bfs = nullptr;     // [4V]
   float* rc_load = nullptr;     // [4V]
@@ -135,26 +137,67 @@ __global__ void timer_no_fanout_kernel(int32_t n, const int32_t* __restrict__ ou
     flag[p] = out_degree[p] == 0;
 }
 
-// R and C of every net into the kept BFS order (prepare_res_cap,
-// reference rct.cu:314-359): node u of net k to position node2bfs[u], the
-// resistance of each oriented edge (parent, child) to the child's position.
-__global__ void timer_reorder_kernel(int32_t N, const int32_t* __restrict__ ns,
-                                     const int32_t* __restrict__ node2bfs,
-                                     const int32_t* __restrict__ edges_dir,
-                                     const float* __restrict__ edge_res,
-                                     const float4* __restrict__ cap, float4* __restrict__ cap_bfs,
-                                     float* __restrict__ pres) {
+// Where every BFS position takes its R and C from (prepare_res_cap,
+// reference rct.cu:314-359): node u of net k sits at position node2bfs[u],
+// and the resistance of each oriented edge (parent, child) belongs to the
+// child's position.  One wave per net; run once, at the first update.
+__global__ void timer_index_kernel(int32_t N, const int32_t* __restrict__ ns,
+                                   const int32_t* __restrict__ node2bfs,
+                                   const int32_t* __restrict__ edges_dir,
+                                   int32_t* __restrict__ src_pos, int32_t* __restrict__ res_edge) {
   const int warp = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int nw = (gridDim.x * blockDim.x) >> 6;
   for (int k = warp; k < N; k += nw) {
     const int b = ns[k], n = ns[k + 1] - b;
-    const int64_t eb = (int64_t)b - k;
-    for (int u = lane; u < n; u += 64) cap_bfs[b + node2bfs[b + u]] = cap[b + u];
-    if (lane == 0 && n > 0) pres[b] = 0.0f;
-    for (int i = lane; i < n - 1; i += 64) {
-      const int child = edges_dir[2 * (eb + i) + 1];
-      pres[b + node2bfs[b + child]] = edge_res[eb + i];
+    const int eb = b - k;
+    for (int u = lane; u < n; u += 64) src_pos[b + node2bfs[b + u]] = b + u;
+    for (int i = lane; i < n - 1; i += 64)
+      res_edge[b + node2bfs[b + edges_dir[2 * (eb + i) + 1]]] = eb + i;
+  }
+}
+
+// R and C of every net into the kept BFS order, one BFS position per lane
+// through the position's source indices: a wave takes 256 consecutive
+// positions, lane l the positions l, l + 64, l + 128, l + 192, so that every
+// index load, capacitance store and resistance store of the wave covers one
+// contiguous range (the capacitance and resistance loads gather within the
+// few nets those positions belong to).  All loads are issued before the
+// first store.
+__global__ void timer_reorder_kernel(int64_t V, const int32_t* __restrict__ src_pos,
+                                     const int32_t* __restrict__ res_edge,
+                                     const float* __restrict__ edge_res,
+                                     const float4* __restrict__ cap, float4* __restrict__ cap_bfs,
+                                     float* __restrict__ pres) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t b = wave * 256; b < V; b += waves * 256) {
+    if (b + 256 <= V) {
+      int sp[4], ep[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        sp[r] = src_pos[b + lane + 64 * r];
+        ep[r] = res_edge[b + lane + 64 * r];
+      }
+      float4 c[4];
+      float rr[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        c[r] = cap[sp[r]];
+        rr[r] = ep[r] >= 0 ? edge_res[ep[r]] : 0.0f;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cap_bfs[b + lane + 64 * r] = c[r];
+        pres[b + lane + 64 * r] = rr[r];
+      }
+    } else {
+      for (int64_t i = b + lane; i < V; i += 64) {
+        cap_bfs[i] = cap[src_pos[i]];
+        const int e = res_edge[i];
+        pres[i] = e >= 0 ? edge_res[e] : 0.0f;
+      }
     }
   }
 }
@@ -269,6 +312,15 @@ otg_status timer_build(otg_timer* T, hipStream_t s) {
                            d.node_cap};
     otg_status st = otg_rct_bfs(ctx, &in, T->node2bfs, T->pid, T->edges_dir, T->pres, T->cap_bfs, s);
     if (st != OTG_OK) return st;
+    // source indices of every BFS position for the later updates (in-range
+    // defaults first: the identity and "no edge")
+    const int64_t V = d.total_num_nodes;
+    hipLaunchKernelGGL(timer_iota_kernel, dim3(blocks_for(V)), dim3(kBlock), 0, s, T->src_pos, V);
+    hipLaunchKernelGGL(timer_fill_i32_kernel, dim3(blocks_for(V)), dim3(kBlock), 0, s, T->res_edge,
+                       V, -1);
+    hipLaunchKernelGGL(timer_index_kernel, dim3(blocks_for(64LL * N)), dim3(kBlock), 0, s, N,
+                       d.nodes_start, T->node2bfs, T->edges_dir, T->src_pos, T->res_edge);
+    OTG_HIP(ctx, hipGetLastError());
     if (T->method != OTG_RCT_PLAIN) {
       const otg_rct_tree topo{N, d.total_num_nodes, d.nodes_start, T->pid, T->pres, T->cap_bfs,
                               nullptr};
@@ -351,7 +403,8 @@ otg_status timer_build(otg_timer* T, hipStream_t s) {
 void timer_free(otg_timer* T) {
   if (T->rc_plan) otg_rct_plan_destroy(T->rc_plan);
   if (T->fplan) otg_fprop_plan_destroy(T->fplan);
-  for (void* p : {(void*)T->node2bfs, (void*)T->pid, (void*)T->edges_dir, (void*)T->pres,
+  for (void* p : {(void*)T->node2bfs, (void*)T->pid, (void*)T->edges_dir, (void*)T->src_pos,
+                  (void*)T->res_edge, (void*)T->pres,
                   (void*)T->cap_bfs, (void*)T->rc_load, (void*)T->rc_delay, (void*)T->rc_impulse,
                   (void*)T->arc_rc_delay, (void*)T->arc_rc_impulse, (void*)T->cell_load,
                   (void*)T->lev_start, (void*)T->lev_fanin, (void*)T->out_degree, (void*)T->first,
@@ -405,6 +458,8 @@ extern "C" otg_status otg_timer_create(otg_context* ctx, const otg_timer_design*
   A(&T->node2bfs, V);
   A(&T->pid, V);
   A(&T->edges_dir, 2 * (V - N));
+  A(&T->src_pos, V);
+  A(&T->res_edge, V);
   A(&T->pres, V);
   A(&T->cap_bfs, 4 * V);
   A(&T->rc_load, 4 * V);
@@ -463,8 +518,9 @@ extern "C" otg_status otg_timer_update(otg_timer* T, otg_stream stream) {
     T->built = true;
   } else if (N > 0) {
     // R and C afresh, in the kept BFS order
-    hipLaunchKernelGGL(timer_reorder_kernel, dim3(blocks_for(64LL * N)), dim3(kBlock), 0, s, N,
-                       d.nodes_start, T->node2bfs, T->edges_dir, d.edge_res,
+    const int64_t V = d.total_num_nodes;
+    hipLaunchKernelGGL(timer_reorder_kernel, dim3(blocks_for((V + 3) / 4)), dim3(kBlock), 0, s, V,
+                       T->src_pos, T->res_edge, d.edge_res,
                        reinterpret_cast<const float4*>(d.node_cap),
                        reinterpret_cast<float4*>(T->cap_bfs), T->pres);
     OTG_HIP(ctx, hipGetLastError());

@@ -250,6 +250,14 @@ def test_timer_synthetic_design(ctx):
         ctx.check(N.lib.otg_timer_update(timer, None))
         torch.cuda.synchronize()
         R = _results(timer, n, m)
+        # a second update re-orders R and C through the kept BFS order: the
+        # same bits (the first update takes them from the ordering itself)
+        ctx.check(N.lib.otg_timer_update(timer, None))
+        torch.cuda.synchronize()
+        R2 = _results(timer, n, m)
+        for k in ("slew", "at", "rat", "slack", "arc_delay", "arc_rc_delay", "arc_rc_impulse",
+                  "pin_load"):
+            _same(R2[k], R[k])
     finally:
         N.lib.otg_timer_destroy(timer)
     rc = D.arc_net >= 0
