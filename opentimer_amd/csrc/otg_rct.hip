@@ -156,6 +156,39 @@ __device__ __forceinline__ void unpack(float4 v, float (&a)[4]) {
 }
 __device__ __forceinline__ float4 pack(const float (&a)[4]) { return make_float4(a[0], a[1], a[2], a[3]); }
 
+// Compensated mode in LDS: four FP64 values (one per corner) in the entries
+// k of two float4 arrays, corners 0-1 in A and 2-3 in B, so that a 16-byte
+// LDS access moves two doubles to or from the register pairs the FP64
+// instructions use (a value's words split over two arrays cost a register
+// move per word).  The _hl forms take and give the (hi, lo) words.
+__device__ __forceinline__ void lds_get_d4(const float4* A, const float4* B, int k,
+                                           double (&v)[4]) {
+  const double2 p = reinterpret_cast<const double2*>(A)[k];
+  const double2 q = reinterpret_cast<const double2*>(B)[k];
+  v[0] = p.x;
+  v[1] = p.y;
+  v[2] = q.x;
+  v[3] = q.y;
+}
+__device__ __forceinline__ void lds_put_d4(float4* A, float4* B, int k, const double (&v)[4]) {
+  reinterpret_cast<double2*>(A)[k] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2*>(B)[k] = make_double2(v[2], v[3]);
+}
+__device__ __forceinline__ void lds_get_hl(const float4* A, const float4* B, int k, float (&h)[4],
+                                           float (&l)[4]) {
+  double v[4];
+  lds_get_d4(A, B, k, v);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) split_f64(v[c], h[c], l[c]);
+}
+__device__ __forceinline__ void lds_put_hl(float4* A, float4* B, int k, const float (&h)[4],
+                                           const float (&l)[4]) {
+  double v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = as_f64(h[c], l[c]);
+  lds_put_d4(A, B, k, v);
+}
+
 // Global-memory accesses through address-space-1 pointers: the compiler then
 // always emits global_load/global_store.  (A generic pointer it cannot prove
 // global becomes a FLAT access, which also counts against the LDS counter,
@@ -829,8 +862,10 @@ constexpr int kClaim = 256;                   // claim table entries (parent slo
 // cleared: the driver's top-down parent.
 template <int W, bool COMP>
 struct StreamLds {
-  float4 a[W + 1];  // bottom-up: load accumulator (hi) | top-down: delay (hi)
-  float4 b[W + 1];  // bottom-up: L accumulator (hi)    | top-down: impulse (hi)
+  // bottom-up: load and L accumulators | top-down: delay and impulse; FP32,
+  // or in the compensated mode FP64 with corners 0-1 in a/b, 2-3 in al/bl
+  float4 a[W + 1];
+  float4 b[W + 1];
   float4 al[COMP ? W + 1 : 1];
   float4 bl[COMP ? W + 1 : 1];
   int32_t claim[kClaim];  // one writer per parent slot per round
@@ -932,25 +967,28 @@ struct SRing {
     if constexpr ((W & (W - 1)) == 0) return (i - base) & (W - 1);
     else return (int)((unsigned)(i - base) % (unsigned)W);
   }
+  // compensated mode: a slot holds the FP64 values, corners 0-1 in a/b and
+  // 2-3 in al/bl (lds_get_d4)
   __device__ __forceinline__ void get(int k, float (&x)[4], float (&y)[4], float (&xl)[4],
                                       float (&yl)[4]) const {
-    unpack(S.a[k], x);
-    unpack(S.b[k], y);
     if constexpr (COMP) {
-      unpack(S.al[k], xl);
-      unpack(S.bl[k], yl);
+      lds_get_hl(S.a, S.al, k, x, xl);
+      lds_get_hl(S.b, S.bl, k, y, yl);
     } else {
+      unpack(S.a[k], x);
+      unpack(S.b[k], y);
 #pragma unroll
       for (int c = 0; c < 4; ++c) xl[c] = yl[c] = 0.f;
     }
   }
   __device__ __forceinline__ void put(int k, const float (&x)[4], const float (&y)[4],
                                       const float (&xl)[4], const float (&yl)[4]) const {
-    S.a[k] = pack(x);
-    S.b[k] = pack(y);
     if constexpr (COMP) {
-      S.al[k] = pack(xl);
-      S.bl[k] = pack(yl);
+      lds_put_hl(S.a, S.al, k, x, xl);
+      lds_put_hl(S.b, S.bl, k, y, yl);
+    } else {
+      S.a[k] = pack(x);
+      S.b[k] = pack(y);
     }
   }
   __device__ __forceinline__ void zero(int k) const {
@@ -1614,14 +1652,16 @@ template <int G, bool COMP>
 struct GroupLds {
   static constexpr int K = G / kWave;        // positions per lane
   static constexpr int NS = G / (2 * kWave);  // driver slews per lane (<= G/2 nets)
-  // (hi) / (lo): the halves of an FP64 value in the compensated mode
-  float4 cap[G];   // bottom-up: cap by position   | top-down: delay (hi) by position
-  float4 w[G];     // bottom-up: w (hi) by schedule | top-down: impulse (hi) by position
-  float4 ld[G];    // load (hi) by schedule index, rounded to FP32 after the sweep
+  // In the compensated mode an FP64 quantity takes two arrays, corners 0-1
+  // in the first and 2-3 in the second (lds_get_d4):
+  // load (ld, ldl), w (w, wl), delay (cap, wl), impulse (w, ldl).
+  float4 cap[G];   // bottom-up: cap by position   | top-down: delay by position
+  float4 w[G];     // bottom-up: w by schedule      | top-down: impulse by position
+  float4 ld[G];    // load by schedule index, rounded to FP32 after the sweep
   float4 L[G];     // L rounded to FP32 by schedule index
-  float4 ldl[COMP ? G : 1];  // bottom-up: load (lo) by schedule index |
-                             // top-down: impulse (lo) by position
-  float4 wl[COMP ? G : 1];   // bottom-up: w (lo) | top-down: delay (lo) by position
+  float4 ldl[COMP ? G : 1];  // bottom-up: load by schedule index |
+                             // top-down: impulse by position
+  float4 wl[COMP ? G : 1];   // bottom-up: w | top-down: delay by position
   float r[G];
   int32_t pid[G];
   uint32_t pw[G];
@@ -1714,59 +1754,54 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
         sw[c] = (T)0;
       }
       for (int k0 = 0; k0 < cn; k0 += kPull) {
-        float4 cl[kPull], cw[kPull], cll[kPull], cwl[kPull];
 #pragma unroll
         for (int u = 0; u < kPull; ++u) {
           const int jc = cf + k0 + u;
-          const bool ok = k0 + u < cn;
-          cl[u] = ok ? S.ld[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
-          cw[u] = ok ? S.w[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
-          if constexpr (COMP) {
-            cll[u] = ok ? S.ldl[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
-            cwl[u] = ok ? S.wl[jc] : make_float4(0.f, 0.f, 0.f, 0.f);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < kPull; ++u) {
-          float a1[4], a2[4], a3[4], a4[4];
-          unpack(cl[u], a1);
-          unpack(cw[u], a2);
-          if constexpr (COMP) {
-            unpack(cll[u], a3);
-            unpack(cwl[u], a4);
-          }
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
+          if (k0 + u < cn) {
             if constexpr (COMP) {
-              ld[c] += as_f64(a1[c], a3[c]);
-              sw[c] += as_f64(a2[c], a4[c]);
+              double a1[4], a2[4];
+              lds_get_d4(S.ld, S.ldl, jc, a1);
+              lds_get_d4(S.w, S.wl, jc, a2);
+#pragma unroll
+              for (int c = 0; c < 4; ++c) {
+                ld[c] += a1[c];
+                sw[c] += a2[c];
+              }
             } else {
-              ld[c] += a1[c];
-              sw[c] += a2[c];
+              float a1[4], a2[4];
+              unpack(S.ld[jc], a1);
+              unpack(S.w[jc], a2);
+#pragma unroll
+              for (int c = 0; c < 4; ++c) {
+                ld[c] += a1[c];
+                sw[c] += a2[c];
+              }
             }
           }
         }
       }
-      float o_ld[4], o_w[4], o_ldl[4], o_wl[4], o_L[4];
+      float o_L[4];
+      if constexpr (COMP) {
+        double o_w[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if constexpr (COMP) {
-          split_f64(ld[c], o_ld[c], o_ldl[c]);
-          split_f64(sw[c] + (double)r * ld[c] * ld[c], o_w[c], o_wl[c]);
+        for (int c = 0; c < 4; ++c) {
+          o_w[c] = sw[c] + (double)r * ld[c] * ld[c];
           o_L[c] = (float)sw[c];
-        } else {
+        }
+        lds_put_d4(S.ld, S.ldl, j, ld);
+        lds_put_d4(S.w, S.wl, j, o_w);
+      } else {
+        float o_ld[4], o_w[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
           o_ld[c] = ld[c];
           o_w[c] = sw[c] + r * ld[c] * ld[c];
           o_L[c] = sw[c];
         }
+        S.ld[j] = pack(o_ld);
+        S.w[j] = pack(o_w);
       }
-      S.ld[j] = pack(o_ld);
-      S.w[j] = pack(o_w);
       S.L[j] = pack(o_L);
-      if constexpr (COMP) {
-        S.ldl[j] = pack(o_ldl);
-        S.wl[j] = pack(o_wl);
-      }
     }
     wave_sync();
     e = b;
@@ -1776,11 +1811,11 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
   // and the outputs (which reuse ldl by position)
   if constexpr (COMP) {
     for (int j = lane; j < n; j += kWave) {
-      float hi[4], lo[4], o[4];
-      unpack(S.ld[j], hi);
-      unpack(S.ldl[j], lo);
+      double v[4];
+      float o[4];
+      lds_get_d4(S.ld, S.ldl, j, v);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) o[c] = (float)as_f64(hi[c], lo[c]);
+      for (int c = 0; c < 4; ++c) o[c] = (float)v[c];
       S.ld[j] = pack(o);
     }
     wave_sync();
@@ -1806,12 +1841,12 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
       unpack(S.L[j], Lv);
       float pd[4], pm[4], pdl[4], pml[4];
       if (!root) {
-        unpack(S.cap[p], pd);
-        unpack(S.w[p], pm);
         if constexpr (COMP) {
-          unpack(S.wl[p], pdl);
-          unpack(S.ldl[p], pml);
+          lds_get_hl(S.cap, S.wl, p, pd, pdl);
+          lds_get_hl(S.w, S.ldl, p, pm, pml);
         } else {
+          unpack(S.cap[p], pd);
+          unpack(S.w[p], pm);
 #pragma unroll
           for (int c = 0; c < 4; ++c) pdl[c] = pml[c] = 0.f;
         }
@@ -1821,11 +1856,12 @@ __device__ __forceinline__ void group_sweeps(GroupLds<G, COMP>& S, const Item& i
       }
       float d[4], dl[4], mm[4], ml[4];
       td_accum<COMP>(td_incr<COMP>(rr, ldv, Lv), pd, pdl, pm, pml, d, dl, mm, ml);
-      S.cap[q] = pack(d);
-      S.w[q] = pack(mm);
       if constexpr (COMP) {
-        S.wl[q] = pack(dl);
-        S.ldl[q] = pack(ml);
+        lds_put_hl(S.cap, S.wl, q, d, dl);
+        lds_put_hl(S.w, S.ldl, q, mm, ml);
+      } else {
+        S.cap[q] = pack(d);
+        S.w[q] = pack(mm);
       }
       S.net[q] = net;
     }
@@ -1853,12 +1889,12 @@ __device__ __forceinline__ void group_outputs(GroupLds<G, COMP>& S, const RctPar
       float ldv[4], Lv[4], d[4], dl[4], mm[4], ml[4];
       unpack(S.ld[j], ldv);
       unpack(S.L[j], Lv);
-      unpack(S.cap[q], d);
-      unpack(S.w[q], mm);
       if constexpr (COMP) {
-        unpack(S.wl[q], dl);
-        unpack(S.ldl[q], ml);
+        lds_get_hl(S.cap, S.wl, q, d, dl);
+        lds_get_hl(S.w, S.ldl, q, mm, ml);
       } else {
+        unpack(S.cap[q], d);
+        unpack(S.w[q], mm);
 #pragma unroll
         for (int c = 0; c < 4; ++c) dl[c] = ml[c] = 0.f;
       }
