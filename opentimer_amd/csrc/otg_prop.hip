@@ -647,11 +647,16 @@ constexpr int kBpropWide = 16;  // arcs per arc lane whose loads a wavefront iss
 // the level's (pin, corner) items in chunks of 256; a pin with more than
 // kBpropSmall fanout arcs is queued in LDS by its corner-0 thread and taken
 // by a whole wavefront (16 arc lanes x 4 corners) once the chunk is done.
+// The level's pins are dealt to the chunks transposed (chunk w takes pins
+// w, w + B, w + 2B, ... of the level, B chunks of 64 pins), because the
+// levelization appends pins of high fanout close together: in consecutive
+// runs one workgroup held 30-37 of them and set the level's time.
 __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv) {
   __shared__ int32_t queue[64];
   __shared__ int32_t qn;
   const int b = P.ends[lv], e = P.ends[lv + 1];
-  const int64_t total = 4LL * (e - b);
+  const int64_t nblk = ((int64_t)(e - b) + 63) / 64;
+  const int64_t total = 4 * 64 * nblk;
   const otg_bprop_graph& G = P.g;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total;
@@ -659,8 +664,10 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
     const int64_t idx = base + threadIdx.x;
-    if (idx < total) {
-      const int u = P.frontiers[b + (int)(idx >> 2)], c = (int)(idx & 3);
+    const int64_t slot = idx >> 2;
+    const int64_t pos = (slot & 63) * nblk + (slot >> 6);
+    if (pos < e - b) {
+      const int u = P.frontiers[b + (int)pos], c = (int)(idx & 3);
       const int el = c >> 1, frf = c & 1;
       const int k0 = G.fanout_start[u], k1 = G.fanout_start[u + 1];
       if (k1 - k0 > kBpropSmall) {
