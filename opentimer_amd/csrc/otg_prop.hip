@@ -51,15 +51,20 @@ constexpr int kLdsWords = 36 * 1024;  // 144 KiB of the 160 KiB per CU
 constexpr int kChunk = 4;             // fanin arcs whose loads are issued together
 constexpr int kUnrolledIdx = 8;       // index lengths searched without a loop
 
-// The interpolation lambda of Lut::operator() (ot/liberty/lut.cpp:176-192).
+// The interpolation lambda of Lut::operator() (ot/liberty/lut.cpp:176-192):
+// every case's value is formed with the same operations and the first case
+// that holds is selected, without branches (lanes of a wavefront look up
+// different tables, and a branch per case diverges).
 __device__ __forceinline__ float lut_interpolate(float x, float x1, float x2, float y1, float y2) {
-  if (x >= FLT_MAX || x <= -FLT_MAX) return x;
   const float slope = (y2 - y1) / (x2 - x1);
-  if (x < x1) return y1 - (x1 - x) * slope;
-  else if (x > x2) return y2 + (x - x2) * slope;
-  else if (x == x1) return y1;
-  else if (x == x2) return y2;
-  else return y1 + (x - x1) * slope;
+  const float below = y1 - (x1 - x) * slope;
+  const float above = y2 + (x - x2) * slope;
+  const float inside = y1 + (x - x1) * slope;
+  float r = x == x2 ? y2 : inside;
+  r = x == x1 ? y1 : r;
+  r = x > x2 ? above : r;
+  r = x < x1 ? below : r;
+  return (x >= FLT_MAX || x <= -FLT_MAX) ? x : r;
 }
 
 // std::lower_bound over ascending indices (lut.cpp:196-197) as the count of
@@ -131,9 +136,10 @@ __device__ __forceinline__ void table_pair(const LibView& L, int l, float s0, fl
     x2[k] = k < n2 ? i2[k] : 0.0f;
   }
   const bool lf = var1 == 1;
+  // (both input transitions are looked up, and a transition that is not
+  // defined gets NaN afterwards: lanes skipping one would diverge)
 #pragma unroll
   for (int f = 0; f < 2; ++f) {
-    if (!(f ? ok1 : ok0)) continue;
     const float sv = f ? s1 : s0;
     const float val1 = lf ? load : sv, val2 = lf ? sv : load;
     int c1 = 0, c2 = 0;
@@ -156,8 +162,8 @@ __device__ __forceinline__ void table_pair(const LibView& L, int l, float s0, fl
           lut_interpolate(val1, i1[a0], i1[a1], tab[a0 * n2 + b1], tab[a1 * n2 + b1]);
       r = lut_interpolate(val2, i2[b0], i2[b1], v0, v1);
     }
-    if (f) r1 = r;
-    else r0 = r;
+    if (f) r1 = ok1 ? r : __builtin_nanf("");
+    else r0 = ok0 ? r : __builtin_nanf("");
   }
 }
 
@@ -466,12 +472,18 @@ __global__ __launch_bounds__(1024) void fprop_wave_level_kernel(PlanParams P, in
           if (!__builtin_isnan(d[f]) && !__builtin_isnan(au[f])) ac[f] = d[f] + au[f];
         }
       }
+      // each owner relaxes over its own slots of this round, in order,
+      // fetching their candidates from the lanes that computed them; the
+      // round takes as many steps as its owner with the most slots
+      const int lo = max(ks, sb), hi = min(ke, sb + 16);
+      int steps = owner && hi > lo ? hi - lo : 0;
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int from = 4 * i + c;
+      for (int m = 4; m < 64; m <<= 1) steps = max(steps, __shfl_xor(steps, m));
+      for (int i = 0; i < steps; ++i) {
+        const int from = 4 * min(lo + i - sb, 15) + c;
         const float s0 = __shfl(so[0], from), s1 = __shfl(so[1], from);
         const float a0 = __shfl(ac[0], from), a1 = __shfl(ac[1], from);
-        if (owner && sb + i >= ks && sb + i < ke) {
+        if (lo + i < hi) {
           s = relax(s, s0, el);
           a = relax(a, a0, el);
           s = relax(s, s1, el);
