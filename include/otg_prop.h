@@ -143,12 +143,11 @@ otg_status otg_fprop_plan_destroy(otg_fprop_plan* plan);
 /* Pins in the plan's levels and their fanin arcs. */
 otg_status otg_fprop_plan_sizes(const otg_fprop_plan* plan, int32_t* positions, int32_t* slots);
 
-/* otg_fprop over a plan: the same results, bit for bit.  The graph's
- * per-arc and per-pin inputs are gathered into plan order, the levels are
- * propagated over contiguous data, and slew and arrival are scattered back
- * to pin order (arc delays are written in arc order directly).  graph must
- * have the plan's CSR (its fanin_start / fanin are not read again).
- * Asynchronous on `stream`; no allocation. */
+/* otg_fprop over a plan: the same results, bit for bit.  Each level's
+ * wavefront chunks read the graph's per-arc and per-pin inputs through the
+ * plan's slot and position indices and write slew, arrival and arc delays
+ * in pin and arc order.  graph must have the plan's CSR (its fanin_start /
+ * fanin are not read again).  Asynchronous on `stream`; no allocation. */
 otg_status otg_fprop_run(otg_context* ctx, otg_fprop_plan* plan, const otg_prop_graph* graph,
                          const otg_prop_lib* lib, const otg_prop_result* out, otg_stream stream);
 

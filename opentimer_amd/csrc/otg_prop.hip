@@ -361,21 +361,18 @@ __global__ __launch_bounds__(1024) void fprop_level_kernel(FpropParams P, int lv
 }
 
 // ---------------------------------------------------------------------------
-// Level-ordered plan: pins renumbered to their position in the frontiers
-// (the pins of a level are consecutive), fanin arcs to slots in that order,
-// so that a level reads its pins' and arcs' data as contiguous ranges.
+// Level plan: the frontiers' pins as positions (the pins of a level are
+// consecutive), their fanin arcs as slots in that order, and the positions of
+// every level cut into wavefront chunks; the level kernel takes a chunk's
+// positions and slots and reads the caller's pin and arc arrays through them.
 // ---------------------------------------------------------------------------
 
 struct PlanView {
   const int32_t* frontiers;  // [n_lv] position -> pin
   const int32_t* ends;       // [levels+1]
   const int32_t* aofs;       // [n_lv+1] first slot of a position
-  const int32_t* slot_src;   // [m_lv] source position, or -(pin+1) for a source outside the plan
+  const int32_t* slot_src;   // [m_lv] the arc's source pin
   const int32_t* slot_arc;   // [m_lv] the arc's index in the caller's CSR
-  int32_t* slot_kt;          // [2 m_lv] kind, timing
-  float* slot_rc;            // [8 m_lv] RC delay[4], impulse[4]
-  float* pos_in;             // [12 n_lv] load[4], PI slew[4], PI arrival[4]
-  float* pos_sa;             // [8 n_lv] slew[4], arrival[4]
   const int32_t* slot_pos;   // [m_lv] position whose fanin arc the slot is
   const int2* wchunk;        // wavefront chunks: positions [x, y)
   const int32_t* wstart;     // [levels+1] first chunk of every level
@@ -383,6 +380,7 @@ struct PlanView {
 
 struct PlanParams {
   PlanView pv;
+  otg_prop_graph g;
   otg_prop_lib lib;
   otg_prop_result out;
   int32_t pool_words;
@@ -392,30 +390,19 @@ struct Planned {
   const PlanParams& P;
   __device__ int first(int q) const { return P.pv.aofs[q]; }
   __device__ int last(int q) const { return P.pv.aofs[q + 1]; }
-  __device__ void arc(int k, int c, int& kind, int& t, int& src, float& rd, float& ri) const {
-    kind = P.pv.slot_kt[2LL * k];
-    t = P.pv.slot_kt[2LL * k + 1];
-    src = P.pv.slot_src[k];
-    rd = P.pv.slot_rc[8LL * k + c];
-    ri = P.pv.slot_rc[8LL * k + 4 + c];
+  // slot k: arc j of the caller's CSR, its source pin u
+  __device__ void arc(int64_t j, int c, int& kind, int& t, float& rd, float& ri) const {
+    kind = P.g.arc_kind[j];
+    t = P.g.arc_timing[j];
+    rd = P.g.arc_rc_delay[4 * j + c];
+    ri = P.g.arc_rc_impulse[4 * j + c];
   }
-  __device__ void source(int src, int el, float su[2], float au[2]) const {
-    if (src >= 0) {
-      const float2* x = reinterpret_cast<const float2*>(P.pv.pos_sa + 8LL * src + el * 2);
-      const float2 sv = x[0], av = x[2];
-      su[0] = sv.x;
-      su[1] = sv.y;
-      au[0] = av.x;
-      au[1] = av.y;
-    } else {
-      const int u = -src - 1;
-      su[0] = P.out.slew[4 * u + el * 2];
-      su[1] = P.out.slew[4 * u + el * 2 + 1];
-      au[0] = P.out.at[4 * u + el * 2];
-      au[1] = P.out.at[4 * u + el * 2 + 1];
-    }
+  __device__ void source(int u, int el, float su[2], float au[2]) const {
+    su[0] = P.out.slew[4 * u + el * 2];
+    su[1] = P.out.slew[4 * u + el * 2 + 1];
+    au[0] = P.out.at[4 * u + el * 2];
+    au[1] = P.out.at[4 * u + el * 2 + 1];
   }
-  __device__ int64_t arc_id(int k) const { return P.pv.slot_arc[k]; }
 };
 
 // One level over the plan by wavefront chunks: a chunk is consecutive
@@ -425,7 +412,10 @@ struct Planned {
 // transitions) -- one pass of table work for the chunk's arcs -- and lane
 // 4*p + c, the owner of position p, relaxes its slew and arrival over its
 // slots in arc order, fetching the candidates by wavefront shuffles: the
-// same operations in the same order as fprop_one.
+// same operations in the same order as fprop_one.  Inputs are read from the
+// caller's arrays through the plan's indices (the sink's load once, by its
+// owner, and shuffled to the slots), and slew and arrival are written to the
+// caller's arrays, where later levels read their sources.
 template <bool kLds>
 __global__ __launch_bounds__(1024) void fprop_wave_level_kernel(PlanParams P, int lv) {
   const int c0 = P.pv.wstart[lv], nch = P.pv.wstart[lv + 1] - c0;
@@ -444,12 +434,13 @@ __global__ __launch_bounds__(1024) void fprop_wave_level_kernel(PlanParams P, in
     const int k0 = P.pv.aofs[q0], ns = P.pv.aofs[q1] - k0;
     const int pq = q0 + (lane >> 2);
     const bool owner = pq < q1;
-    float s = __builtin_nanf(""), a = __builtin_nanf("");
-    int ks = 0, ke = 0;
+    float s = __builtin_nanf(""), a = __builtin_nanf(""), lown = __builtin_nanf("");
+    int ks = 0, ke = 0, v = 0;
     if (owner) {
-      const float* in = P.pv.pos_in + 12LL * pq;
-      s = in[4 + c];
-      a = in[8 + c];
+      v = P.pv.frontiers[pq];
+      s = P.g.pi_slew[4 * v + c];
+      a = P.g.pi_at[4 * v + c];
+      lown = P.g.pin_load[4 * v + c];
       ks = P.pv.aofs[pq] - k0;
       ke = P.pv.aofs[pq + 1] - k0;
     }
@@ -457,15 +448,17 @@ __global__ __launch_bounds__(1024) void fprop_wave_level_kernel(PlanParams P, in
       float so[2] = {__builtin_nanf(""), __builtin_nanf("")};
       float ac[2] = {__builtin_nanf(""), __builtin_nanf("")};
       const int sl = sb + (lane >> 2);
+      // the sink's load comes from the lane of its owner
+      const int kp = sl < ns ? P.pv.slot_pos[k0 + sl] - q0 : 0;
+      const float lc = __shfl(lown, 4 * kp + c);
       if (sl < ns) {
         const int k = k0 + sl;
-        int kind, t, src;
+        const int64_t j = P.pv.slot_arc[k];
+        int kind, t;
         float rd, ri, su[2], au[2], d[2];
-        S.arc(k, c, kind, t, src, rd, ri);
-        S.source(src, el, su, au);
-        const float lc = P.pv.pos_in[12LL * P.pv.slot_pos[k] + c];
+        S.arc(j, c, kind, t, rd, ri);
+        S.source(P.pv.slot_src[k], el, su, au);
         arc_candidates(L, kind, t, rd, ri, su, lc, el, trf, so, d);
-        const int64_t j = S.arc_id(k);
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
           if (P.out.arc_delay) P.out.arc_delay[8 * j + 4 * el + 2 * f + trf] = d[f];
@@ -492,8 +485,8 @@ __global__ __launch_bounds__(1024) void fprop_wave_level_kernel(PlanParams P, in
       }
     }
     if (owner) {
-      P.pv.pos_sa[8LL * pq + c] = s;
-      P.pv.pos_sa[8LL * pq + 4 + c] = a;
+      P.out.slew[4 * v + c] = s;
+      P.out.at[4 * v + c] = a;
     }
   }
 }
@@ -509,8 +502,8 @@ __global__ void plan_fill_kernel(int32_t* a, int64_t n, int32_t v) {
 // A frontier pin out of range, or a fanin range outside the arcs, reports the
 // position (bad[0], smallest); such a position gets no arcs.
 __global__ void plan_pos_kernel(const int32_t* frontiers, int32_t n_lv, int32_t num_pins,
-                                int32_t num_arcs, const int32_t* fanin_start, int32_t* pin_pos,
-                                int32_t* deg, int32_t* bad) {
+                                int32_t num_arcs, const int32_t* fanin_start, int32_t* deg,
+                                int32_t* bad) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
        q += (int64_t)gridDim.x * blockDim.x) {
     const int v = frontiers[q];
@@ -521,7 +514,6 @@ __global__ void plan_pos_kernel(const int32_t* frontiers, int32_t n_lv, int32_t 
       const int b = fanin_start[v], e = fanin_start[v + 1];
       if (b < 0 || e < b || e > num_arcs) atomicMin(bad, (int)q);
       else d = e - b;
-      pin_pos[v] = (int)q;
     }
     deg[q] = d;
   }
@@ -531,7 +523,7 @@ __global__ void plan_pos_kernel(const int32_t* frontiers, int32_t n_lv, int32_t 
 // as pin 0 (the plan is then discarded).
 __global__ void plan_slot_kernel(const int32_t* frontiers, int32_t n_lv, int32_t num_pins,
                                  const int32_t* fanin_start, const int32_t* fanin,
-                                 const int32_t* pin_pos, const int32_t* aofs, int32_t* slot_src,
+                                 const int32_t* aofs, int32_t* slot_src,
                                  int32_t* slot_arc, int32_t* slot_pos, int32_t* bad) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
        q += (int64_t)gridDim.x * blockDim.x) {
@@ -546,67 +538,10 @@ __global__ void plan_slot_kernel(const int32_t* frontiers, int32_t n_lv, int32_t
         atomicMin(bad + 1, j);
         u = 0;
       }
-      const int pu = pin_pos[u];
-      slot_src[k0 + r] = pu >= 0 ? pu : -u - 1;
+      slot_src[k0 + r] = u;
       slot_arc[k0 + r] = j;
       slot_pos[k0 + r] = (int)q;
     }
-  }
-}
-
-// run: the caller's per-arc and per-pin inputs into plan order ...
-// 4 floats of a pin or arc: one 16-byte access when the caller's array is
-// 16-byte aligned (kVec, checked on the host), else four
-template <bool kVec>
-__device__ __forceinline__ float4 load4(const float* p) {
-  if constexpr (kVec) return *reinterpret_cast<const float4*>(p);
-  else return make_float4(p[0], p[1], p[2], p[3]);
-}
-template <bool kVec>
-__device__ __forceinline__ void store4(float* p, float4 x) {
-  if constexpr (kVec) {
-    *reinterpret_cast<float4*>(p) = x;
-  } else {
-    p[0] = x.x;
-    p[1] = x.y;
-    p[2] = x.z;
-    p[3] = x.w;
-  }
-}
-
-template <bool kVec>
-__global__ void plan_gather_arcs_kernel(PlanView pv, otg_prop_graph g, int32_t m_lv) {
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m_lv;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    const int j = pv.slot_arc[k];
-    reinterpret_cast<int2*>(pv.slot_kt)[k] = make_int2(g.arc_kind[j], g.arc_timing[j]);
-    float4* o = reinterpret_cast<float4*>(pv.slot_rc + 8 * k);
-    o[0] = load4<kVec>(g.arc_rc_delay + 4LL * j);
-    o[1] = load4<kVec>(g.arc_rc_impulse + 4LL * j);
-  }
-}
-
-template <bool kVec>
-__global__ void plan_gather_pins_kernel(PlanView pv, otg_prop_graph g, int32_t n_lv) {
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    const int v = pv.frontiers[q];
-    float4* o = reinterpret_cast<float4*>(pv.pos_in + 12 * q);
-    o[0] = load4<kVec>(g.pin_load + 4LL * v);
-    o[1] = load4<kVec>(g.pi_slew + 4LL * v);
-    o[2] = load4<kVec>(g.pi_at + 4LL * v);
-  }
-}
-
-// ... and the results back to pin order
-template <bool kVec>
-__global__ void plan_scatter_kernel(PlanView pv, otg_prop_result out, int32_t n_lv) {
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_lv;
-       q += (int64_t)gridDim.x * blockDim.x) {
-    const int v = pv.frontiers[q];
-    const float4* x = reinterpret_cast<const float4*>(pv.pos_sa + 8 * q);
-    store4<kVec>(out.slew + 4LL * v, x[0]);
-    store4<kVec>(out.at + 4LL * v, x[1]);
   }
 }
 
@@ -780,10 +715,6 @@ struct otg_fprop_plan {
   int32_t* aofs = nullptr;
   int32_t* slot_src = nullptr;
   int32_t* slot_arc = nullptr;
-  int32_t* slot_kt = nullptr;
-  float* slot_rc = nullptr;
-  float* pos_in = nullptr;
-  float* pos_sa = nullptr;
   int32_t* slot_pos = nullptr;
   int2* wchunk = nullptr;
   int32_t* wstart = nullptr;
@@ -794,8 +725,7 @@ namespace {
 
 void plan_free(otg_fprop_plan* p) {
   for (void* x : {(void*)p->frontiers, (void*)p->ends, (void*)p->aofs, (void*)p->slot_src,
-                  (void*)p->slot_arc, (void*)p->slot_kt, (void*)p->slot_rc, (void*)p->pos_in,
-                  (void*)p->pos_sa, (void*)p->slot_pos, (void*)p->wchunk, (void*)p->wstart})
+                  (void*)p->slot_arc, (void*)p->slot_pos, (void*)p->wchunk, (void*)p->wstart})
     if (x) (void)hipFree(x);
   delete p;
 }
@@ -834,13 +764,11 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
   p->num_pins = num_pins;
   p->num_arcs = num_arcs;
   p->num_levels = num_levels;
-  int32_t* pin_pos = nullptr;
   int32_t* deg = nullptr;
   int32_t* bad = nullptr;
   void* tmp = nullptr;
   auto fail = [&](hipError_t e) {
     (void)hipStreamSynchronize(s);
-    if (pin_pos) (void)hipFree(pin_pos);
     if (deg) (void)hipFree(deg);
     if (bad) (void)hipFree(bad);
     if (tmp) (void)hipFree(tmp);
@@ -873,7 +801,6 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
   if ((e = hipMalloc(&p->frontiers, I * std::max(n_lv, 1))) ||
       (e = hipMalloc(&p->ends, I * (num_levels + 1))) ||
       (e = hipMalloc(&p->aofs, I * (n_lv + 1))) ||
-      (e = hipMalloc(&pin_pos, I * std::max(num_pins, 1))) ||
       (e = hipMalloc(&deg, I * std::max(n_lv, 1))) || (e = hipMalloc(&bad, I * 2)))
     return fail(e);
   hipLaunchKernelGGL(plan_fill_kernel, dim3(1), dim3(64), 0, s, bad, (int64_t)2, INT32_MAX);
@@ -883,12 +810,10 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
   if (num_levels > 0 &&
       (e = hipMemcpyAsync(p->ends, frontier_ends, I * (num_levels + 1), hipMemcpyDeviceToDevice, s)))
     return fail(e);
-  hipLaunchKernelGGL(plan_fill_kernel, dim3(grid_for(ctx, num_pins, 256)), dim3(256), 0, s,
-                     pin_pos, (int64_t)num_pins, -1);
   hipLaunchKernelGGL(plan_fill_kernel, dim3(1), dim3(64), 0, s, p->aofs, (int64_t)1, 0);
   if (n_lv > 0) {
     hipLaunchKernelGGL(plan_pos_kernel, dim3(grid_for(ctx, n_lv, 256)), dim3(256), 0, s,
-                       p->frontiers, n_lv, num_pins, num_arcs, fanin_start, pin_pos, deg, bad);
+                       p->frontiers, n_lv, num_pins, num_arcs, fanin_start, deg, bad);
     size_t tb = 0;
     if ((e = hipcub::DeviceScan::InclusiveSum(nullptr, tb, deg, p->aofs + 1, n_lv, s)) ||
         (e = hipMalloc(&tmp, std::max<size_t>(tb, 1))) ||
@@ -902,15 +827,11 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
   p->m_lv = m_lv;
   if ((e = hipMalloc(&p->slot_src, I * std::max(m_lv, 1))) ||
       (e = hipMalloc(&p->slot_arc, I * std::max(m_lv, 1))) ||
-      (e = hipMalloc(&p->slot_kt, 2 * I * std::max(m_lv, 1))) ||
-      (e = hipMalloc(&p->slot_rc, 8 * sizeof(float) * std::max(m_lv, 1))) ||
-      (e = hipMalloc(&p->pos_in, 12 * sizeof(float) * std::max(n_lv, 1))) ||
-      (e = hipMalloc(&p->pos_sa, 8 * sizeof(float) * std::max(n_lv, 1))) ||
       (e = hipMalloc(&p->slot_pos, I * std::max(m_lv, 1))))
     return fail(e);
   if (n_lv > 0)
     hipLaunchKernelGGL(plan_slot_kernel, dim3(grid_for(ctx, n_lv, 256)), dim3(256), 0, s,
-                       p->frontiers, n_lv, num_pins, fanin_start, fanin, pin_pos, p->aofs,
+                       p->frontiers, n_lv, num_pins, fanin_start, fanin, p->aofs,
                        p->slot_src, p->slot_arc, p->slot_pos, bad);
   // wavefront chunks, level by level: consecutive positions, at most 16 of
   // them and at most 16 arc slots (a position with more slots alone)
@@ -922,7 +843,6 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
         (e = hipStreamSynchronize(s)))
       return fail(e);
     if (hbad[0] != INT32_MAX || hbad[1] != INT32_MAX) {
-      (void)hipFree(pin_pos);
       (void)hipFree(deg);
       (void)hipFree(bad);
       if (tmp) (void)hipFree(tmp);
@@ -959,7 +879,6 @@ extern "C" otg_status otg_fprop_plan_create(otg_context* ctx, int32_t num_pins, 
       return fail(e);
   }
   if ((e = hipStreamSynchronize(s)) || (e = hipGetLastError())) return fail(e);
-  (void)hipFree(pin_pos);
   (void)hipFree(deg);
   (void)hipFree(bad);
   if (tmp) (void)hipFree(tmp);
@@ -999,21 +918,9 @@ extern "C" otg_status otg_fprop_run(otg_context* ctx, otg_fprop_plan* plan,
     return OTG_ERR_INVALID_ARGUMENT;
   OTG_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = otg_pick_stream(ctx, stream);
-  PlanView pv{plan->frontiers, plan->ends,     plan->aofs,     plan->slot_src,
-              plan->slot_arc,  plan->slot_kt,  plan->slot_rc,  plan->pos_in,
-              plan->pos_sa,    plan->slot_pos, plan->wchunk,   plan->wstart};
-  auto a16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
-  const bool vec_arcs = a16(graph->arc_rc_delay) && a16(graph->arc_rc_impulse);
-  const bool vec_pins = a16(graph->pin_load) && a16(graph->pi_slew) && a16(graph->pi_at);
-  const bool vec_out = a16(out->slew) && a16(out->at);
-  const dim3 ga(grid_for(ctx, plan->m_lv, 256)), gp(grid_for(ctx, plan->n_lv, 256));
-  if (plan->m_lv > 0) {
-    if (vec_arcs) hipLaunchKernelGGL(plan_gather_arcs_kernel<true>, ga, dim3(256), 0, s, pv, *graph, plan->m_lv);
-    else hipLaunchKernelGGL(plan_gather_arcs_kernel<false>, ga, dim3(256), 0, s, pv, *graph, plan->m_lv);
-  }
-  if (vec_pins) hipLaunchKernelGGL(plan_gather_pins_kernel<true>, gp, dim3(256), 0, s, pv, *graph, plan->n_lv);
-  else hipLaunchKernelGGL(plan_gather_pins_kernel<false>, gp, dim3(256), 0, s, pv, *graph, plan->n_lv);
-  PlanParams P{pv, *lib, *out, lib->num_lut_vals};
+  PlanView pv{plan->frontiers, plan->ends,     plan->aofs,   plan->slot_src,
+              plan->slot_arc,  plan->slot_pos, plan->wchunk, plan->wstart};
+  PlanParams P{pv, *graph, *lib, *out, lib->num_lut_vals};
   if (lib_in_lds(lib)) {
     for (int lv = plan->num_levels - 1; lv >= 0; --lv)
       hipLaunchKernelGGL(fprop_wave_level_kernel<true>, dim3(std::max(1, ctx->num_cus)),
@@ -1023,8 +930,6 @@ extern "C" otg_status otg_fprop_run(otg_context* ctx, otg_fprop_plan* plan,
       hipLaunchKernelGGL(fprop_wave_level_kernel<false>, dim3(std::max(1, ctx->num_cus * 2)),
                          dim3(1024), 0, s, P, lv);
   }
-  if (vec_out) hipLaunchKernelGGL(plan_scatter_kernel<true>, gp, dim3(256), 0, s, pv, *out, plan->n_lv);
-  else hipLaunchKernelGGL(plan_scatter_kernel<false>, gp, dim3(256), 0, s, pv, *out, plan->n_lv);
   OTG_HIP(ctx, hipGetLastError());
   return OTG_OK;
 }
