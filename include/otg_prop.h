@@ -177,6 +177,9 @@ typedef struct otg_bprop_graph {
   const float* arc_delay;      /* [8m]  otg_prop_result.arc_delay           */
   const float* po_rat;         /* [4n]  primary-output required time or NaN */
   const float* at;             /* [4n]  arrival (for slack) or NULL         */
+  const int32_t* fanout_to;    /* [m]   sink pin of fanout entry k, i.e.
+                                *       arc_to[fanout_arc[k]], or NULL: saves
+                                *       one dependent load per arc          */
 } otg_bprop_graph;
 
 otg_status otg_bprop(otg_context* ctx, const otg_bprop_graph* graph, const int32_t* frontiers,

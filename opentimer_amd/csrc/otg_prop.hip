@@ -598,8 +598,13 @@ constexpr int kBpropWide = 16;  // arcs per arc lane whose loads a wavefront iss
 // w, w + B, w + 2B, ... of the level, B chunks of 64 pins), because the
 // levelization appends pins of high fanout close together: in consecutive
 // runs one workgroup held 30-37 of them and set the level's time.
+// TO: the graph lists every fanout entry's sink (fanout_to), read beside the
+// arc id instead of through it (a choice per launch, not per load: a load
+// under a branch leaves the compiler unable to count what is in flight, and
+// it then waits for everything before each dependent load).
+template <bool TO>
 __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv) {
-  __shared__ int32_t queue[64];
+  __shared__ int3 queue[64];  // (pin, first fanout entry, end)
   __shared__ int32_t qn;
   const int b = P.ends[lv], e = P.ends[lv + 1];
   const int64_t nblk = ((int64_t)(e - b) + 63) / 64;
@@ -618,7 +623,7 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
       const int el = c >> 1, frf = c & 1;
       const int k0 = G.fanout_start[u], k1 = G.fanout_start[u + 1];
       if (k1 - k0 > kBpropSmall) {
-        if (c == 0) queue[atomicAdd(&qn, 1)] = u;
+        if (c == 0) queue[atomicAdd(&qn, 1)] = make_int3(u, k0, k1);
       } else {
         float r = G.po_rat[4 * u + c];
         // kBpropStep arcs per step: their ids, then sinks and delays, then
@@ -627,13 +632,20 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
         for (int k = k0; k < k1; k += kBpropStep) {
           int j[kBpropStep];
           float2 d[kBpropStep], rv[kBpropStep];
+          int vz[kBpropStep];
 #pragma unroll
-          for (int z = 0; z < kBpropStep; ++z) j[z] = G.fanout_arc[k + z < k1 ? k + z : k0];
+          for (int z = 0; z < kBpropStep; ++z) {
+            const int kk = k + z < k1 ? k + z : k0;
+            j[z] = G.fanout_arc[kk];
+            if constexpr (TO) vz[z] = G.fanout_to[kk];
+          }
 #pragma unroll
           for (int z = 0; z < kBpropStep; ++z) {
             // (past the pin's last arc these load its first arc again: loads
             // without a branch around them)
-            const int v = G.arc_to[j[z]];
+            int v;
+            if constexpr (TO) v = vz[z];
+            else v = G.arc_to[j[z]];
             d[z] = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j[z] + 4 * el + 2 * frf);
             rv[z] = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
           }
@@ -658,20 +670,26 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
     __syncthreads();
     const int nq = qn;
     for (int q = wave; q < nq; q += blockDim.x >> 6) {
-      const int u = queue[q], c = lane & 3, a0 = lane >> 2;
+      const int3 qe = queue[q];
+      const int u = qe.x, k0 = qe.y, k1 = qe.z, c = lane & 3, a0 = lane >> 2;
       const int el = c >> 1, frf = c & 1;
-      const int k0 = G.fanout_start[u], k1 = G.fanout_start[u + 1];
       float r = a0 == 0 ? G.po_rat[4 * u + c] : __builtin_nanf("");
       int ri = -1;
       for (int kb = k0 + a0; kb < k1; kb += 16 * kBpropWide) {
         int j[kBpropWide];
         float2 d[kBpropWide], rv[kBpropWide];
-#pragma unroll
-        for (int z = 0; z < kBpropWide; ++z)
-          j[z] = G.fanout_arc[kb + 16 * z < k1 ? kb + 16 * z : k0];
+        int vz[kBpropWide];
 #pragma unroll
         for (int z = 0; z < kBpropWide; ++z) {
-          const int v = G.arc_to[j[z]];
+          const int kk = kb + 16 * z < k1 ? kb + 16 * z : k0;
+          j[z] = G.fanout_arc[kk];
+          if constexpr (TO) vz[z] = G.fanout_to[kk];
+        }
+#pragma unroll
+        for (int z = 0; z < kBpropWide; ++z) {
+          int v;
+          if constexpr (TO) v = vz[z];
+          else v = G.arc_to[j[z]];
           d[z] = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j[z] + 4 * el + 2 * frf);
           rv[z] = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
         }
@@ -982,8 +1000,10 @@ extern "C" otg_status otg_bprop(otg_context* ctx, const otg_bprop_graph* graph,
   hipStream_t s = otg_pick_stream(ctx, stream);
   BpropParams P{*graph, frontiers, frontier_ends, rat, slack};
   const int grid = std::max(1, ctx->num_cus * 8);
-  for (int lv = 0; lv < num_levels; ++lv)
-    hipLaunchKernelGGL(bprop_level_kernel, dim3(grid), dim3(256), 0, s, P, lv);
+  for (int lv = 0; lv < num_levels; ++lv) {
+    if (graph->fanout_to) hipLaunchKernelGGL(bprop_level_kernel<true>, dim3(grid), dim3(256), 0, s, P, lv);
+    else hipLaunchKernelGGL(bprop_level_kernel<false>, dim3(grid), dim3(256), 0, s, P, lv);
+  }
   OTG_HIP(ctx, hipGetLastError());
   return OTG_OK;
 }

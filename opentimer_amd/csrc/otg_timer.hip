@@ -58,6 +58,7 @@ struct otg_timer {
   // fanout CSR for the backward pass
   int32_t* fanout_start = nullptr;  // [n+1]
   int32_t* fanout_arc = nullptr;    // [m]
+  int32_t* fanout_to = nullptr;     // [m] sink of every fanout entry
   int32_t* arc_to = nullptr;        // [m]
   // results
   float* slew = nullptr;       // [4n]
@@ -279,6 +280,13 @@ otg_status exclusive_scan(otg_context* ctx, const int32_t* in, int32_t* out, int
   return OTG_OK;
 }
 
+__global__ void timer_fanout_to_kernel(int32_t m, const int32_t* __restrict__ fanout_arc,
+                                       const int32_t* __restrict__ arc_to,
+                                       int32_t* __restrict__ fanout_to) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+    fanout_to[k] = arc_to[fanout_arc[k]];
+}
+
 __global__ void timer_iota_kernel(int32_t* a, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -375,6 +383,8 @@ otg_status timer_build(otg_timer* T, hipStream_t s) {
     OTG_HIP(ctx, W.get(&tmp, (int64_t)tb));
     OTG_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key2, seq, T->fanout_arc, m, 0,
                                                     end_bit, s));
+    hipLaunchKernelGGL(timer_fanout_to_kernel, dim3(blocks_for(m)), dim3(kBlock), 0, s, m,
+                       T->fanout_arc, T->arc_to, T->fanout_to);
   }
   // first frontier: the pins without fanout, ascending
   hipLaunchKernelGGL(timer_no_fanout_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n,
@@ -409,7 +419,8 @@ void timer_free(otg_timer* T) {
                   (void*)T->arc_rc_delay, (void*)T->arc_rc_impulse, (void*)T->cell_load,
                   (void*)T->lev_start, (void*)T->lev_fanin, (void*)T->out_degree, (void*)T->first,
                   (void*)T->frontiers, (void*)T->frontier_ends, (void*)T->fanout_start,
-                  (void*)T->fanout_arc, (void*)T->arc_to, (void*)T->slew, (void*)T->at,
+                  (void*)T->fanout_arc, (void*)T->fanout_to, (void*)T->arc_to, (void*)T->slew,
+                  (void*)T->at,
                   (void*)T->rat, (void*)T->slack, (void*)T->arc_delay})
     if (p) (void)hipFree(p);
   delete T;
@@ -477,6 +488,7 @@ extern "C" otg_status otg_timer_create(otg_context* ctx, const otg_timer_design*
   A(&T->fanout_start, n + 1);
   A(&T->fanout_arc, m);
   A(&T->arc_to, m);
+  A(&T->fanout_to, m);
   A(&T->slew, 4 * n);
   A(&T->at, 4 * n);
   A(&T->rat, 4 * n);
@@ -550,8 +562,8 @@ extern "C" otg_status otg_timer_update(otg_timer* T, otg_stream stream) {
   const otg_prop_result out{T->slew, T->at, T->arc_delay};
   otg_status st = otg_fprop_run(ctx, T->fplan, &g, &T->lib, &out, stream);
   if (st != OTG_OK) return st;
-  const otg_bprop_graph bg{n, m, T->fanout_start, T->fanout_arc, T->arc_to, T->arc_delay,
-                           d.po_rat, T->at};
+  const otg_bprop_graph bg{n,          m,         T->fanout_start, T->fanout_arc, T->arc_to,
+                           T->arc_delay, d.po_rat, T->at,           T->fanout_to};
   return otg_bprop(ctx, &bg, T->frontiers, T->frontier_ends, T->num_levels, T->rat, T->slack,
                    stream);
 }

@@ -119,7 +119,7 @@ class PropResult(C.Structure):
 class BpropGraph(C.Structure):
     _fields_ = [("num_pins", C.c_int32), ("num_arcs", C.c_int32), ("fanout_start", vp),
                 ("fanout_arc", vp), ("arc_to", vp), ("arc_delay", vp), ("po_rat", vp),
-                ("at", vp)]
+                ("at", vp), ("fanout_to", vp)]
 
 
 class TimerDesign(C.Structure):

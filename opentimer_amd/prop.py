@@ -114,13 +114,16 @@ class Propagator:
             arc_to = np.zeros(max(self.m, 1), np.int32)
             N.check(N.lib.otg_build_fanout_csr(self.n, N.addr(fs), N.addr(fi), N.addr(fo_start),
                                                N.addr(fo_arc), N.addr(arc_to)))
-            self._fanout = tuple(torch.from_numpy(a).to(self.device)
-                                 for a in (fo_start, fo_arc, arc_to))
+            fo_to = arc_to[fo_arc[: self.m]] if self.m else np.zeros(1, np.int32)
+            self._fanout = tuple(torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(self.device)
+                                 for a in (fo_start, fo_arc, arc_to, fo_to))
         pr = torch.from_numpy(np.ascontiguousarray(po_rat, np.float32).reshape(-1, 4)).to(self.device)
         rat = torch.full((max(self.n, 1), 4), float("nan"), dtype=torch.float32, device=self.device)
         slack = torch.full_like(rat, float("nan"))
-        g = N.BpropGraph(self.n, self.m, *(t.data_ptr() for t in self._fanout),
-                         self.arc_delay.data_ptr(), pr.data_ptr(), self.at.data_ptr())
+        fo_start, fo_arc, arc_to, fo_to = self._fanout
+        g = N.BpropGraph(self.n, self.m, fo_start.data_ptr(), fo_arc.data_ptr(),
+                         arc_to.data_ptr(), self.arc_delay.data_ptr(), pr.data_ptr(),
+                         self.at.data_ptr(), fo_to.data_ptr())
         self._bp = (g, pr, rat, slack)
         self.rerun_required(frontiers, ends, num_levels, stream)
         return rat[: self.n], slack[: self.n]
