@@ -584,7 +584,8 @@ __device__ __forceinline__ void rat_pick(float& v, int& i, float v2, int i2, int
   }
 }
 
-constexpr int kBpropSmall = 8;  // fanout arcs a pin's own thread walks; more go to a wavefront
+constexpr int kBpropSmall = 8;  // fanout arcs a pin's own thread walks; more are queued
+constexpr int kBpropMid = 64;   // fanout arcs a quarter wavefront takes in one round
 constexpr int kBpropStep = 4;   // arcs whose loads a pin's own thread issues together
 constexpr int kBpropWide = 16;  // arcs per arc lane whose loads a wavefront issues together
                                 // (256 arcs per round: a fanout of 512 takes two rounds of
@@ -592,8 +593,10 @@ constexpr int kBpropWide = 16;  // arcs per arc lane whose loads a wavefront iss
 
 // Levels first to last, one launch per level: workgroups of 256 threads take
 // the level's (pin, corner) items in chunks of 256; a pin with more than
-// kBpropSmall fanout arcs is queued in LDS by its corner-0 thread and taken
-// by a whole wavefront (16 arc lanes x 4 corners) once the chunk is done.
+// kBpropSmall fanout arcs is queued in LDS by its corner-0 thread and taken,
+// once the chunk is done, by a whole wavefront (16 arc lanes x 4 corners)
+// if it has more than kBpropMid arcs, else by a quarter wavefront (4 arc
+// lanes x 4 corners, four such pins per wavefront).
 // The level's pins are dealt to the chunks transposed (chunk w takes pins
 // w, w + B, w + 2B, ... of the level, B chunks of 64 pins), because the
 // levelization appends pins of high fanout close together: in consecutive
@@ -602,18 +605,85 @@ constexpr int kBpropWide = 16;  // arcs per arc lane whose loads a wavefront iss
 // arc id instead of through it (a choice per launch, not per load: a load
 // under a branch leaves the compiler unable to count what is in flight, and
 // it then waits for everything before each dependent load).
+// Queued pins of one workgroup, each taken by AL arc lanes x 4 corners (16 /
+// AL pins per wavefront; kBpropWide arcs' loads per arc lane and round) with
+// an order-preserving reduction over its arc lanes: bit for bit the
+// sequential relaxation, +-0 ties included.  Wavefront w takes the groups
+// first, first + stride, ...
+template <bool TO, int AL>
+__device__ __forceinline__ void bprop_queued(const BpropParams& P, const int3* q, int nq,
+                                             int first, int stride, int lane) {
+  constexpr int PPW = 16 / AL;
+  const otg_bprop_graph& G = P.g;
+  const int p = lane / (4 * AL), a0 = (lane >> 2) % AL, c = lane & 3;
+  const int el = c >> 1, frf = c & 1;
+  for (int g = first; g * PPW < nq; g += stride) {
+    const int qi = g * PPW + p;
+    const bool has = qi < nq;
+    const int3 qe = has ? q[qi] : make_int3(0, 0, 0);
+    const int u = qe.x, k0 = qe.y, k1 = qe.z;
+    float r = (a0 == 0 && has) ? G.po_rat[4 * u + c] : __builtin_nanf("");
+    int ri = -1;
+    for (int kb = k0 + a0; kb < k1; kb += AL * kBpropWide) {
+      int j[kBpropWide];
+      float2 d[kBpropWide], rv[kBpropWide];
+      int vz[kBpropWide];
+#pragma unroll
+      for (int z = 0; z < kBpropWide; ++z) {
+        const int kk = kb + AL * z < k1 ? kb + AL * z : k0;
+        j[z] = G.fanout_arc[kk];
+        if constexpr (TO) vz[z] = G.fanout_to[kk];
+      }
+#pragma unroll
+      for (int z = 0; z < kBpropWide; ++z) {
+        int v;
+        if constexpr (TO) v = vz[z];
+        else v = G.arc_to[j[z]];
+        d[z] = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j[z] + 4 * el + 2 * frf);
+        rv[z] = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
+      }
+#pragma unroll
+      for (int z = 0; z < kBpropWide; ++z) {
+        const int k = kb + AL * z;
+        if (k >= k1) break;
+        if (!__builtin_isnan(d[z].x) && !__builtin_isnan(rv[z].x))
+          rat_pick(r, ri, rv[z].x - d[z].x, 2 * (k - k0), el);
+        if (!__builtin_isnan(d[z].y) && !__builtin_isnan(rv[z].y))
+          rat_pick(r, ri, rv[z].y - d[z].y, 2 * (k - k0) + 1, el);
+      }
+    }
+#pragma unroll
+    for (int m = 4; m < 4 * AL; m <<= 1) {
+      const float r2 = __shfl_xor(r, m);
+      const int i2 = __shfl_xor(ri, m);
+      rat_pick(r, ri, r2, i2, el);
+    }
+    if (a0 == 0 && has) {
+      P.rat[4 * u + c] = r;
+      if (P.slack) {
+        const float a = G.at[4 * u + c];
+        P.slack[4 * u + c] = (__builtin_isnan(a) || __builtin_isnan(r))
+                                 ? __builtin_nanf("")
+                                 : (el == 0 ? a - r : r - a);
+      }
+    }
+  }
+}
+
 template <bool TO>
 __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv) {
-  __shared__ int3 queue[64];  // (pin, first fanout entry, end)
-  __shared__ int32_t qn;
+  __shared__ int3 queue[64];   // (pin, first fanout entry, end): more than kBpropMid arcs
+  __shared__ int3 mqueue[64];  // kBpropSmall < arcs <= kBpropMid
+  __shared__ int32_t qn, mqn;
   const int b = P.ends[lv], e = P.ends[lv + 1];
   const int64_t nblk = ((int64_t)(e - b) + 63) / 64;
   const int64_t total = 4 * 64 * nblk;
   const otg_bprop_graph& G = P.g;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total;
        base += (int64_t)gridDim.x * blockDim.x) {
-    if (threadIdx.x == 0) qn = 0;
+    if (threadIdx.x == 0) qn = mqn = 0;
     __syncthreads();
     const int64_t idx = base + threadIdx.x;
     const int64_t slot = idx >> 2;
@@ -622,8 +692,10 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
       const int u = P.frontiers[b + (int)pos], c = (int)(idx & 3);
       const int el = c >> 1, frf = c & 1;
       const int k0 = G.fanout_start[u], k1 = G.fanout_start[u + 1];
-      if (k1 - k0 > kBpropSmall) {
+      if (k1 - k0 > kBpropMid) {
         if (c == 0) queue[atomicAdd(&qn, 1)] = make_int3(u, k0, k1);
+      } else if (k1 - k0 > kBpropSmall) {
+        if (c == 0) mqueue[atomicAdd(&mqn, 1)] = make_int3(u, k0, k1);
       } else {
         float r = G.po_rat[4 * u + c];
         // kBpropStep arcs per step: their ids, then sinks and delays, then
@@ -668,57 +740,11 @@ __global__ __launch_bounds__(256) void bprop_level_kernel(BpropParams P, int lv)
       }
     }
     __syncthreads();
-    const int nq = qn;
-    for (int q = wave; q < nq; q += blockDim.x >> 6) {
-      const int3 qe = queue[q];
-      const int u = qe.x, k0 = qe.y, k1 = qe.z, c = lane & 3, a0 = lane >> 2;
-      const int el = c >> 1, frf = c & 1;
-      float r = a0 == 0 ? G.po_rat[4 * u + c] : __builtin_nanf("");
-      int ri = -1;
-      for (int kb = k0 + a0; kb < k1; kb += 16 * kBpropWide) {
-        int j[kBpropWide];
-        float2 d[kBpropWide], rv[kBpropWide];
-        int vz[kBpropWide];
-#pragma unroll
-        for (int z = 0; z < kBpropWide; ++z) {
-          const int kk = kb + 16 * z < k1 ? kb + 16 * z : k0;
-          j[z] = G.fanout_arc[kk];
-          if constexpr (TO) vz[z] = G.fanout_to[kk];
-        }
-#pragma unroll
-        for (int z = 0; z < kBpropWide; ++z) {
-          int v;
-          if constexpr (TO) v = vz[z];
-          else v = G.arc_to[j[z]];
-          d[z] = *reinterpret_cast<const float2*>(G.arc_delay + 8LL * j[z] + 4 * el + 2 * frf);
-          rv[z] = *reinterpret_cast<const float2*>(P.rat + 4LL * v + 2 * el);
-        }
-#pragma unroll
-        for (int z = 0; z < kBpropWide; ++z) {
-          const int k = kb + 16 * z;
-          if (k >= k1) break;
-          if (!__builtin_isnan(d[z].x) && !__builtin_isnan(rv[z].x))
-            rat_pick(r, ri, rv[z].x - d[z].x, 2 * (k - k0), el);
-          if (!__builtin_isnan(d[z].y) && !__builtin_isnan(rv[z].y))
-            rat_pick(r, ri, rv[z].y - d[z].y, 2 * (k - k0) + 1, el);
-        }
-      }
-#pragma unroll
-      for (int m = 4; m < 64; m <<= 1) {
-        const float r2 = __shfl_xor(r, m);
-        const int i2 = __shfl_xor(ri, m);
-        rat_pick(r, ri, r2, i2, el);
-      }
-      if (a0 == 0) {
-        P.rat[4 * u + c] = r;
-        if (P.slack) {
-          const float a = G.at[4 * u + c];
-          P.slack[4 * u + c] = (__builtin_isnan(a) || __builtin_isnan(r))
-                                   ? __builtin_nanf("")
-                                   : (el == 0 ? a - r : r - a);
-        }
-      }
-    }
+    // the pins of highest fanout one per wavefront, then the others four per
+    // wavefront, starting after the wavefronts that took one of the former
+    const int nq = qn, nm = mqn;
+    bprop_queued<TO, 16>(P, queue, nq, wave, nw, lane);
+    bprop_queued<TO, 4>(P, mqueue, nm, (wave - nq % nw + nw) % nw, nw, lane);
     __syncthreads();
   }
 }
