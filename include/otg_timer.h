@@ -82,7 +82,8 @@ otg_status otg_timer_destroy(otg_timer* timer);
  * the plans (the topology must not change afterwards; it synchronises
  * `stream` and allocates its working arrays); later updates reuse them,
  * read R, C, pin loads and boundary values afresh, and are asynchronous on
- * `stream`.  A fanin arc naming a pin out of range gives
+ * `stream`.  A fanin_start that does not run from 0 to num_arcs without
+ * decreasing, or a fanin arc naming a pin out of range, gives
  * OTG_ERR_INVALID_ARGUMENT, a net that is not a tree OTG_ERR_NOT_A_TREE
  * (first update). */
 otg_status otg_timer_update(otg_timer* timer, otg_stream stream);

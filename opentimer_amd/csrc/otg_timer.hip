@@ -93,13 +93,21 @@ __global__ void timer_fill_i32_kernel(int32_t* a, int64_t n, int32_t v) {
 // count (lev_count[p], exclusive-scanned into the CSR offsets), each arc's
 // sink, each source pin's count of propagating fanout arcs (out_degree) and
 // every arc's key for the fanout sort (its source; n for a loop breaker).
-__global__ void timer_arc_count_kernel(int32_t n, const int32_t* __restrict__ fanin_start,
+__global__ void timer_arc_count_kernel(int32_t n, int32_t m,
+                                       const int32_t* __restrict__ fanin_start,
                                        const int32_t* __restrict__ fanin,
                                        const int32_t* __restrict__ kind, int32_t* __restrict__ lev_count,
                                        int32_t* __restrict__ out_degree, int32_t* __restrict__ arc_to,
                                        int32_t* __restrict__ sort_key, int32_t* __restrict__ bad) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     const int b = fanin_start[p], e = fanin_start[p + 1];
+    // a CSR that does not run from 0 to m without decreasing reports the pin
+    // (bad[1]) and its range is not walked
+    if (b < 0 || e < b || e > m || (p == 0 && b != 0) || (p == n - 1 && e != m)) {
+      atomicMin(bad + 1, p);
+      lev_count[p] = 0;
+      continue;
+    }
     int c = 0;
     for (int j = b; j < e; ++j) {
       const int u = fanin[j];
@@ -341,7 +349,7 @@ otg_status timer_build(otg_timer* T, hipStream_t s) {
   int32_t *cnt = nullptr, *bad = nullptr, *key = nullptr, *key2 = nullptr, *seq = nullptr,
           *flag = nullptr, *nsel = nullptr;
   OTG_HIP(ctx, W.get(&cnt, n + 1));
-  OTG_HIP(ctx, W.get(&bad, 1));
+  OTG_HIP(ctx, W.get(&bad, 2));
   OTG_HIP(ctx, W.get(&key, m));
   OTG_HIP(ctx, W.get(&key2, m));
   OTG_HIP(ctx, W.get(&seq, m));
@@ -350,16 +358,21 @@ otg_status timer_build(otg_timer* T, hipStream_t s) {
   // arcs' sinks, propagating fanin counts, fanout counts
   hipLaunchKernelGGL(timer_fill_i32_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, T->out_degree,
                      (int64_t)n, 0);
-  hipLaunchKernelGGL(timer_fill_i32_kernel, dim3(1), dim3(kBlock), 0, s, bad, (int64_t)1, INT32_MAX);
+  hipLaunchKernelGGL(timer_fill_i32_kernel, dim3(1), dim3(kBlock), 0, s, bad, (int64_t)2, INT32_MAX);
   hipLaunchKernelGGL(timer_fill_i32_kernel, dim3(1), dim3(kBlock), 0, s, cnt + n, (int64_t)1, 0);
-  hipLaunchKernelGGL(timer_arc_count_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n,
+  hipLaunchKernelGGL(timer_arc_count_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, m,
                      d.fanin_start, d.fanin, d.arc_kind, cnt, T->out_degree, T->arc_to, key, bad);
   OTG_HIP(ctx, hipGetLastError());
-  int32_t hbad = INT32_MAX;
-  OTG_HIP(ctx, hipMemcpyAsync(&hbad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  int32_t hbad[2] = {INT32_MAX, INT32_MAX};
+  OTG_HIP(ctx, hipMemcpyAsync(hbad, bad, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   OTG_HIP(ctx, hipStreamSynchronize(s));
-  if (hbad != INT32_MAX) {
-    OTG_SET_ERR(ctx, "pin %d: a fanin arc names a pin out of range", hbad);
+  if (hbad[1] != INT32_MAX) {
+    OTG_SET_ERR(ctx, "pin %d: fanin_start must run from 0 to num_arcs without decreasing",
+                hbad[1]);
+    return OTG_ERR_INVALID_ARGUMENT;
+  }
+  if (hbad[0] != INT32_MAX) {
+    OTG_SET_ERR(ctx, "pin %d: a fanin arc names a pin out of range", hbad[0]);
     return OTG_ERR_INVALID_ARGUMENT;
   }
   // levelization CSR (exclusive scan of the counts, then the arcs in order)

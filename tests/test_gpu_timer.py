@@ -209,6 +209,188 @@ def test_timer_update_after_the_caps_change(ctx):
         D["forest"].close()
 
 
+class _SynDesign:
+    """Device copies of a workloads.TimerDesignData and the C structs over
+    them (the arrays may be replaced before .build())."""
+
+    def __init__(self, D, lib):
+        self.D, self.lib = D, lib
+        self.h = {"nodes_start": D.forest.nodes_start, "edges": D.forest.edges,
+                  "roots": D.forest.roots, "edge_res": D.forest.edge_res,
+                  "node_cap": D.forest.node_cap, "fanin_start": D.fanin_start, "fanin": D.fanin,
+                  "arc_kind": D.arc_kind, "arc_timing": D.arc_timing, "arc_net": D.arc_net,
+                  "arc_node": D.arc_node, "pin_net": D.pin_net, "pin_load": D.pin_load,
+                  "pi_slew": D.pi_slew, "pi_at": D.pi_at, "po_rat": D.po_rat}
+        self.num_nets = D.forest.num_nets
+
+    def build(self):
+        dev = torch.device("cuda:0")
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        self.T = {k: t(v) for k, v in self.h.items()}
+        lk = ("lut_hdr", "lut_val", "timing_sense", "timing_edge", "timing_lut", "timing_has")
+        self.LB = {k: t(getattr(self.lib, k)) for k in lk}
+        a = lambda d, k: d[k].data_ptr()  # noqa: E731
+        T = self.T
+        self.n, self.m = len(self.h["pin_net"]), len(self.h["fanin"])
+        self.design = N.TimerDesign(
+            self.num_nets, int(self.h["nodes_start"][-1]) if self.num_nets else 0,
+            a(T, "nodes_start"), a(T, "edges"), a(T, "roots"), a(T, "edge_res"),
+            a(T, "node_cap"), self.n, self.m, a(T, "fanin_start"), a(T, "fanin"),
+            a(T, "arc_kind"), a(T, "arc_timing"), a(T, "arc_net"), a(T, "arc_node"),
+            a(T, "pin_net"), a(T, "pin_load"), a(T, "pi_slew"), a(T, "pi_at"), a(T, "po_rat"))
+        self.plib = N.PropLib(len(self.lib.lut_hdr) // 4, len(self.lib.timing_sense),
+                              len(self.lib.lut_val), *[a(self.LB, k) for k in lk])
+        return self
+
+    def oracle_check(self, R):
+        """The timer's slews, arrivals, arc delays, required times and
+        slacks against the oracle's propagation over the timer's own levels
+        and gathered net values, bit for bit."""
+        h, lib = self.h, self.lib
+        lk = ("lut_hdr", "lut_val", "timing_sense", "timing_edge", "timing_lut", "timing_has")
+        g = {"fanin_start": h["fanin_start"], "fanin": h["fanin"], "arc_kind": h["arc_kind"],
+             "arc_timing": h["arc_timing"], "arc_rc_delay": R["arc_rc_delay"].reshape(-1, 4),
+             "arc_rc_impulse": R["arc_rc_impulse"].reshape(-1, 4),
+             "pin_load": R["pin_load"].reshape(-1, 4), "pi_slew": h["pi_slew"], "pi_at": h["pi_at"]}
+        fr, ends = R["frontiers"], R["ends"]
+        assert ends[-1] == self.n
+        os_, oa, od = O.fprop(g, {k: getattr(lib, k) for k in lk}, fr, ends)
+        _same(R["slew"].reshape(-1, 4), os_)
+        _same(R["at"].reshape(-1, 4), oa)
+        _same(R["arc_delay"].reshape(-1, 8), od.reshape(-1, 8))
+        orat, oslack = O.bprop(h["fanin_start"], h["fanin"], od, h["po_rat"], oa, fr, ends)
+        _same(R["rat"].reshape(-1, 4), orat)
+        _same(R["slack"].reshape(-1, 4), oslack)
+
+
+def _last_error(ctx):
+    return (N.lib.otg_last_error(ctx.ptr) or b"").decode(errors="replace")
+
+
+def _run_timer(ctx, S, updates=1):
+    timer = N.timer_p()
+    ctx.check(N.lib.otg_timer_create(ctx.ptr, C.byref(S.design), C.byref(S.plib),
+                                     N.BATCHED_COMPENSATED, C.byref(timer)))
+    try:
+        for _ in range(updates):
+            ctx.check(N.lib.otg_timer_update(timer, None))
+        torch.cuda.synchronize()
+        return _results(timer, S.n, S.m)
+    finally:
+        N.lib.otg_timer_destroy(timer)
+
+
+def test_timer_net_arcs_out_of_range_and_designs_without_nets(ctx):
+    """Net arcs naming a net or node outside the forest get NaN net values
+    (otg_timer.h: out-of-range ids give NaN), which then propagate as the
+    oracle propagates them; a design whose pins drive no RC net (no forest at
+    all) takes every cell load from pin_load; an empty design updates."""
+    from opentimer_amd import workloads as W
+    lib = W.prop_library()
+    D = W.timer_design(W.DagConfig("timer-oor", 6_000, 15_000, 12, seed=15), lib, seed=16)
+    S = _SynDesign(D, lib)
+    rc = np.flatnonzero(D.arc_net >= 0)
+    rng = np.random.default_rng(17)
+    pick = rng.choice(rc, 40, replace=False)
+    arc_net, arc_node = D.arc_net.copy(), D.arc_node.copy()
+    sizes = np.diff(D.forest.nodes_start)
+    arc_net[pick[:10]] = D.forest.num_nets + 3            # net past the forest
+    arc_net[pick[10:20]] = -7                             # negative net on an RC arc
+    arc_node[pick[20:30]] = sizes[arc_net[pick[20:30]]]   # node one past its net
+    arc_node[pick[30:40]] = -1                            # negative node
+    S.h["arc_net"], S.h["arc_node"] = arc_net, arc_node
+    R = _run_timer(ctx, S.build())
+    d = R["arc_rc_delay"].reshape(-1, 4)
+    assert np.isnan(d[pick]).all()
+    good = np.setdiff1d(rc, pick)
+    assert not np.isnan(d[good]).any()
+    S.oracle_check(R)
+    # no RC nets: every arc a cell arc, no pin drives a net
+    S2 = _SynDesign(D, lib)
+    S2.num_nets = 0
+    S2.h["arc_kind"] = np.where(D.arc_kind == 0, 2, D.arc_kind).astype(np.int32)
+    S2.h["arc_net"] = np.full_like(D.arc_net, -1)
+    S2.h["arc_node"] = np.full_like(D.arc_node, -1)
+    S2.h["pin_net"] = np.full_like(D.pin_net, -1)
+    S2.h["nodes_start"] = np.zeros(1, np.int32)
+    R2 = _run_timer(ctx, S2.build(), updates=2)
+    _same(R2["pin_load"].reshape(-1, 4), D.pin_load)
+    S2.oracle_check(R2)
+    # an empty design
+    S3 = _SynDesign(D, lib)
+    S3.num_nets = 0
+    for k in ("fanin", "arc_kind", "arc_timing", "arc_net", "arc_node"):
+        S3.h[k] = np.zeros(1, np.int32)
+    S3.h["nodes_start"] = np.zeros(1, np.int32)
+    S3.build()
+    S3.design.num_pins = S3.design.num_arcs = 0
+    timer = N.timer_p()
+    ctx.check(N.lib.otg_timer_create(ctx.ptr, C.byref(S3.design), C.byref(S3.plib),
+                                     N.BATCHED_COMPENSATED, C.byref(timer)))
+    ctx.check(N.lib.otg_timer_update(timer, None))
+    N.lib.otg_timer_destroy(timer)
+
+
+def test_timer_rejects_malformed_designs(ctx):
+    """Argument and design errors come back as status codes with a message,
+    before any kernel reads out of range: NULL arguments, negative sizes, a
+    fanin CSR that decreases or does not end at num_arcs, a fanin arc naming
+    a pin out of range (OTG_ERR_INVALID_ARGUMENT), an RC net that is not a
+    tree (OTG_ERR_NOT_A_TREE)."""
+    INVALID, NOT_A_TREE = 1, 2
+    from opentimer_amd import workloads as W
+    lib = W.prop_library()
+    D = W.timer_design(W.DagConfig("timer-bad", 4_000, 10_000, 10, seed=25), lib, seed=26)
+
+    def status(S, update=True):
+        timer = N.timer_p()
+        st = N.lib.otg_timer_create(ctx.ptr, C.byref(S.design), C.byref(S.plib),
+                                    N.BATCHED_COMPENSATED, C.byref(timer))
+        if st != 0 or not update:
+            return st, _last_error(ctx) if st else ""
+        try:
+            st = N.lib.otg_timer_update(timer, None)
+            torch.cuda.synchronize()
+            return st, _last_error(ctx) if st else ""
+        finally:
+            N.lib.otg_timer_destroy(timer)
+
+    S = _SynDesign(D, lib).build()
+    timer = N.timer_p()
+    assert N.lib.otg_timer_create(ctx.ptr, None, C.byref(S.plib), N.BATCHED_COMPENSATED,
+                                  C.byref(timer)) == INVALID
+    S.design.num_pins = -1
+    assert status(S)[0] == INVALID
+    # fanin CSR decreasing, and not ending at num_arcs
+    for fix in ("decrease", "end"):
+        S = _SynDesign(D, lib)
+        fs = D.fanin_start.copy()
+        if fix == "decrease":
+            fs[100] = fs[102] + 1
+        else:
+            fs[-1] = fs[-1] - 1
+        S.h["fanin_start"] = fs
+        st, msg = status(S.build())
+        assert st == INVALID and "fanin_start" in msg, (fix, st, msg)
+    # a fanin arc from a pin out of range
+    S = _SynDesign(D, lib)
+    fi = D.fanin.copy()
+    fi[5] = len(D.pin_net) + 9
+    S.h["fanin"] = fi
+    st, msg = status(S.build())
+    assert st == INVALID and "out of range" in msg, msg
+    # an RC net that is not a tree (a duplicated edge)
+    S = _SynDesign(D, lib)
+    f = D.forest
+    k = int(np.flatnonzero(np.diff(f.nodes_start) >= 4)[0])
+    e0 = int(f.nodes_start[k] - k)
+    bad = f.edges.copy()
+    bad[2 * e0], bad[2 * e0 + 1] = bad[2 * e0 + 2], bad[2 * e0 + 3]
+    S.h["edges"] = bad
+    st, msg = status(S.build())
+    assert st == NOT_A_TREE and f"net {k}" in msg, (st, msg)
+
+
 def test_timer_synthetic_design(ctx):
     """A generated design (workloads.timer_design: a layered graph whose
     driving pins own RC trees of the cfg1 size distribution): the RC delays
