@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "otg_internal.hpp"
@@ -99,6 +100,14 @@ extern "C" otg_status otg_destroy(otg_context* ctx) {
   for (hipEvent_t e : ctx->lv_ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->lv_flag) (void)hipHostFree(ctx->lv_flag);
+  for (int t = 0; t < otg_context::kStage; ++t) {
+    if (ctx->stage_stream[t]) {
+      (void)hipStreamSynchronize(ctx->stage_stream[t]);
+      (void)hipStreamDestroy(ctx->stage_stream[t]);
+    }
+    for (void* b : ctx->stage_buf[t])
+      if (b) (void)hipHostFree(b);
+  }
   delete ctx;
   return OTG_OK;
 }
@@ -352,6 +361,73 @@ extern "C" otg_status otg_rct_last_breakdown(const otg_context* ctx, double ms[8
   return OTG_OK;
 }
 
+// A copy between a caller's host array and device memory, staged through
+// pinned memory by otg_context::kStage host threads: each takes a contiguous
+// share and moves it in kStageBytes chunks, the DMA of one chunk overlapping
+// its own copy of the chunk before (a copy to or from pageable memory is
+// otherwise staged by the runtime one chunk at a time, ~17 GB/s device to
+// host measured on MI355X).  Synchronous: the device side of `dst` / `src`
+// must be ready (the caller synchronises stream `s` first).  Copies below
+// four chunks are queued on `s` unchanged.
+static otg_status staged_copy(otg_context* ctx, void* dst, const void* src, size_t bytes,
+                              hipMemcpyKind kind, hipStream_t s) {
+  constexpr int T = otg_context::kStage;
+  constexpr size_t C = otg_context::kStageBytes;
+  if (bytes < 4 * C) {
+    OTG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind, s));
+    return OTG_OK;
+  }
+  for (int t = 0; t < T; ++t) {
+    if (!ctx->stage_stream[t])
+      OTG_HIP(ctx, hipStreamCreateWithFlags(&ctx->stage_stream[t], hipStreamNonBlocking));
+    for (void*& b : ctx->stage_buf[t])
+      if (!b) OTG_HIP(ctx, hipHostMalloc(&b, C, hipHostMallocDefault));
+  }
+  const bool d2h = kind == hipMemcpyDeviceToHost;
+  const size_t share = ((bytes + T - 1) / T + 4095) & ~size_t(4095);
+  hipError_t err[T];
+  std::vector<std::thread> pool;
+  pool.reserve(T);
+  for (int t = 0; t < T; ++t) {
+    err[t] = hipSuccess;
+    const size_t a = std::min(bytes, t * share), b = std::min(bytes, a + share);
+    pool.emplace_back([=, &err]() {
+      hipError_t e = hipSetDevice(ctx->device);
+      hipStream_t st = ctx->stage_stream[t];
+      char* const* buf = reinterpret_cast<char* const*>(ctx->stage_buf[t]);
+      char* out = static_cast<char*>(dst);
+      const char* in = static_cast<const char*>(src);
+      size_t prev_off = 0, prev_len = 0;
+      int cur = 0;
+      for (size_t off = a; off < b && e == hipSuccess; off += C, cur ^= 1) {
+        const size_t len = std::min(C, b - off);
+        if (d2h) {
+          // chunk k lands in one buffer while chunk k-1 leaves the other
+          e = hipMemcpyAsync(buf[cur], in + off, len, hipMemcpyDeviceToHost, st);
+          if (prev_len) std::memcpy(out + prev_off, buf[cur ^ 1], prev_len);
+          if (e == hipSuccess) e = hipStreamSynchronize(st);
+          prev_off = off;
+          prev_len = len;
+        } else {
+          // chunk k is filled while chunk k-1 is on its way (waited for
+          // before its buffer is reused)
+          std::memcpy(buf[cur], in + off, len);
+          e = hipStreamSynchronize(st);
+          if (e == hipSuccess) e = hipMemcpyAsync(out + off, buf[cur], len, hipMemcpyHostToDevice, st);
+        }
+      }
+      if (e == hipSuccess) {
+        if (d2h && prev_len) std::memcpy(out + prev_off, buf[cur ^ 1], prev_len);
+        if (!d2h) e = hipStreamSynchronize(st);
+      }
+      err[t] = e;
+    });
+  }
+  for (std::thread& th : pool) th.join();
+  for (int t = 0; t < T; ++t) OTG_HIP(ctx, err[t]);
+  return OTG_OK;
+}
+
 extern "C" otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_array* d,
                                               otg_rct_method method) {
   if (!ctx || !d || d->num_nets < 0 || d->total_num_nodes < 0) return OTG_ERR_INVALID_ARGUMENT;
@@ -391,12 +467,16 @@ extern "C" otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_arr
     if (d->slew) OTG_HIP(ctx, B.alloc(&slew, 4 * V));
     if (d->slew_in) OTG_HIP(ctx, B.alloc(&si, 4 * N));
     OTG_HIP(ctx, clk.mark(0));
-    OTG_HIP(ctx, hipMemcpyAsync(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, s));
-    OTG_HIP(ctx, hipMemcpyAsync(ed, d->rct_edges, sizeof(int32_t) * 2 * E, hipMemcpyHostToDevice, s));
-    OTG_HIP(ctx, hipMemcpyAsync(rt, d->rct_roots, sizeof(int32_t) * N, hipMemcpyHostToDevice, s));
-    OTG_HIP(ctx, hipMemcpyAsync(res, d->rct_edges_res, sizeof(float) * E, hipMemcpyHostToDevice, s));
-    OTG_HIP(ctx, hipMemcpyAsync(ncap, d->rct_nodes_cap, sizeof(float) * 4 * V, hipMemcpyHostToDevice, s));
-    if (si) OTG_HIP(ctx, hipMemcpyAsync(si, d->slew_in, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s));
+    auto h2d = [&](void* dst, const void* src, size_t bytes) {
+      return staged_copy(ctx, dst, src, bytes, hipMemcpyHostToDevice, s);
+    };
+    if ((st = h2d(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1))) != OTG_OK ||
+        (st = h2d(ed, d->rct_edges, sizeof(int32_t) * 2 * E)) != OTG_OK ||
+        (st = h2d(rt, d->rct_roots, sizeof(int32_t) * N)) != OTG_OK ||
+        (st = h2d(res, d->rct_edges_res, sizeof(float) * E)) != OTG_OK ||
+        (st = h2d(ncap, d->rct_nodes_cap, sizeof(float) * 4 * V)) != OTG_OK ||
+        (si && (st = h2d(si, d->slew_in, sizeof(float) * 4 * N)) != OTG_OK))
+      return st;
     OTG_HIP(ctx, clk.mark(1));
     otg_rct_edges in{d->num_nets, d->total_num_nodes, ns, ed, rt, res, ncap};
     st = otg_rct_bfs(ctx, &in, n2b, pid, ed, pres, cap, s);
@@ -406,15 +486,20 @@ extern "C" otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_arr
     otg_rct_result out{load, delay, ldelay, beta, imp, slew};
     st = solve_timed(ctx, &tree, &out, method, s, clk);
     if (st != OTG_OK) return st;
-    OTG_HIP(ctx, hipMemcpyAsync(d->rct_edges, ed, sizeof(int32_t) * 2 * E, hipMemcpyDeviceToHost, s));
-    OTG_HIP(ctx, hipMemcpyAsync(d->rct_node2bfs_order, n2b, sizeof(int32_t) * V, hipMemcpyDeviceToHost, s));
-    OTG_HIP(ctx, hipMemcpyAsync(d->rct_pid, pid, sizeof(int32_t) * V, hipMemcpyDeviceToHost, s));
-    OTG_HIP(ctx, hipMemcpyAsync(d->load, load, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    OTG_HIP(ctx, hipMemcpyAsync(d->delay, delay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    OTG_HIP(ctx, hipMemcpyAsync(d->ldelay, ldelay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    OTG_HIP(ctx, hipMemcpyAsync(d->impulse, imp, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    if (d->beta) OTG_HIP(ctx, hipMemcpyAsync(d->beta, beta, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    if (d->slew) OTG_HIP(ctx, hipMemcpyAsync(d->slew, slew, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipStreamSynchronize(s));
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+      return staged_copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost, s);
+    };
+    if ((st = d2h(d->rct_edges, ed, sizeof(int32_t) * 2 * E)) != OTG_OK ||
+        (st = d2h(d->rct_node2bfs_order, n2b, sizeof(int32_t) * V)) != OTG_OK ||
+        (st = d2h(d->rct_pid, pid, sizeof(int32_t) * V)) != OTG_OK ||
+        (st = d2h(d->load, load, sizeof(float) * 4 * V)) != OTG_OK ||
+        (st = d2h(d->delay, delay, sizeof(float) * 4 * V)) != OTG_OK ||
+        (st = d2h(d->ldelay, ldelay, sizeof(float) * 4 * V)) != OTG_OK ||
+        (st = d2h(d->impulse, imp, sizeof(float) * 4 * V)) != OTG_OK ||
+        (d->beta && (st = d2h(d->beta, beta, sizeof(float) * 4 * V)) != OTG_OK) ||
+        (d->slew && (st = d2h(d->slew, slew, sizeof(float) * 4 * V)) != OTG_OK))
+      return st;
     OTG_HIP(ctx, clk.mark(5));
   }  // device buffers released here
   OTG_HIP(ctx, clk.mark(6));
@@ -450,22 +535,31 @@ extern "C" otg_status otg_rct_compute(otg_context* ctx, const otg_rct_host* d,
     if (d->slew) OTG_HIP(ctx, B.alloc(&slew, 4 * V));
     if (d->slew_in) OTG_HIP(ctx, B.alloc(&si, 4 * N));
     OTG_HIP(ctx, clk.mark(0));
-    OTG_HIP(ctx, hipMemcpyAsync(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, s));
-    OTG_HIP(ctx, hipMemcpyAsync(pid, d->rct_pid, sizeof(int32_t) * V, hipMemcpyHostToDevice, s));
-    OTG_HIP(ctx, hipMemcpyAsync(pres, d->pres, sizeof(float) * V, hipMemcpyHostToDevice, s));
-    OTG_HIP(ctx, hipMemcpyAsync(cap, d->cap, sizeof(float) * 4 * V, hipMemcpyHostToDevice, s));
-    if (si) OTG_HIP(ctx, hipMemcpyAsync(si, d->slew_in, sizeof(float) * 4 * N, hipMemcpyHostToDevice, s));
+    auto h2d = [&](void* dst, const void* src, size_t bytes) {
+      return staged_copy(ctx, dst, src, bytes, hipMemcpyHostToDevice, s);
+    };
+    if ((st = h2d(ns, d->rct_nodes_start, sizeof(int32_t) * (N + 1))) != OTG_OK ||
+        (st = h2d(pid, d->rct_pid, sizeof(int32_t) * V)) != OTG_OK ||
+        (st = h2d(pres, d->pres, sizeof(float) * V)) != OTG_OK ||
+        (st = h2d(cap, d->cap, sizeof(float) * 4 * V)) != OTG_OK ||
+        (si && (st = h2d(si, d->slew_in, sizeof(float) * 4 * N)) != OTG_OK))
+      return st;
     OTG_HIP(ctx, clk.mark(1));
     otg_rct_tree tree{d->num_nets, d->total_num_nodes, ns, pid, pres, cap, si};
     otg_rct_result out{load, delay, ldelay, beta, imp, slew};
     st = solve_timed(ctx, &tree, &out, method, s, clk);
     if (st != OTG_OK) return st;
-    OTG_HIP(ctx, hipMemcpyAsync(d->load, load, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    OTG_HIP(ctx, hipMemcpyAsync(d->delay, delay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    OTG_HIP(ctx, hipMemcpyAsync(d->ldelay, ldelay, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    OTG_HIP(ctx, hipMemcpyAsync(d->impulse, imp, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    if (d->beta) OTG_HIP(ctx, hipMemcpyAsync(d->beta, beta, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
-    if (d->slew) OTG_HIP(ctx, hipMemcpyAsync(d->slew, slew, sizeof(float) * 4 * V, hipMemcpyDeviceToHost, s));
+    OTG_HIP(ctx, hipStreamSynchronize(s));
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+      return staged_copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost, s);
+    };
+    if ((st = d2h(d->load, load, sizeof(float) * 4 * V)) != OTG_OK ||
+        (st = d2h(d->delay, delay, sizeof(float) * 4 * V)) != OTG_OK ||
+        (st = d2h(d->ldelay, ldelay, sizeof(float) * 4 * V)) != OTG_OK ||
+        (st = d2h(d->impulse, imp, sizeof(float) * 4 * V)) != OTG_OK ||
+        (d->beta && (st = d2h(d->beta, beta, sizeof(float) * 4 * V)) != OTG_OK) ||
+        (d->slew && (st = d2h(d->slew, slew, sizeof(float) * 4 * V)) != OTG_OK))
+      return st;
     OTG_HIP(ctx, clk.mark(5));
   }
   OTG_HIP(ctx, clk.mark(6));

@@ -27,6 +27,13 @@ struct otg_context {
   hipStream_t aux[kAux] = {};
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[kAux] = {};
+  // Host-pointer entry points: large copies between the caller's (pageable)
+  // arrays and the device are split over kStage host threads, each with its
+  // own stream and two pinned buffers of kStageBytes (created on first use).
+  static constexpr int kStage = 8;
+  static constexpr size_t kStageBytes = size_t(8) << 20;
+  hipStream_t stage_stream[kStage] = {};
+  void* stage_buf[kStage][2] = {};
 };
 
 // One unit of work of the batched P1 launch: a contiguous range of whole

@@ -304,6 +304,51 @@ def test_bfs_and_compute_host_api(ctx):
         assert rel_err(outs[k], r64[k], fl[k]) < 1e-5, k
 
 
+def test_host_api_large_arrays(ctx):
+    """Host arrays of 60-130 MB (7.9M nodes) go through the library's staged
+    pinned copies, several chunks per copy thread: both host entry points
+    must return exactly what the device-pointer path returns for the same
+    forest (same kernels, so bit-identical)."""
+    import ctypes as C
+    f = W.generate_rc(W.config1(num_nets=150000, seed=5))
+    V, Nn = f.num_nodes, f.num_nets
+    assert 16 * V > (64 << 20)
+    s = RcSolver.from_edges(f, ctx)
+    ref = {k: v.cpu().numpy() for k, v in s.solve("compensated").items()}
+    a = N.RctEdgeArray()
+    bufs = dict(edges=f.edges.copy(), n2b=np.full(V, -1, np.int32), pid=np.full(V, -1, np.int32))
+    outs = {k: np.full(4 * V, np.nan, np.float32) for k in OUT}
+    a.num_nets, a.total_num_nodes, a.total_num_edges = Nn, V, V - Nn
+    a.rct_nodes_start = N.addr(f.nodes_start)
+    a.rct_edges = N.addr(bufs["edges"])
+    a.rct_roots = N.addr(f.roots)
+    a.rct_edges_res = N.addr(f.edge_res)
+    a.rct_nodes_cap = N.addr(f.node_cap)
+    a.rct_node2bfs_order = N.addr(bufs["n2b"])
+    a.rct_pid = N.addr(bufs["pid"])
+    for k in OUT:
+        setattr(a, k, N.addr(outs[k]))
+    a.slew_in = N.addr(f.slew_in)
+    ctx.check(N.lib.otg_rct_bfs_and_compute(ctx.ptr, C.byref(a), N.BATCHED_COMPENSATED))
+    assert np.array_equal(bufs["n2b"], s.node2bfs.cpu().numpy())
+    assert np.array_equal(bufs["pid"], s.pid.cpu().numpy())
+    assert np.array_equal(bufs["edges"], s.edges_directed.cpu().numpy())
+    for k in OUT:
+        assert np.array_equal(outs[k].view(np.uint32), ref[k].reshape(-1).view(np.uint32)), k
+    pres = s.pres.cpu().numpy()
+    cap = s.cap.cpu().numpy()
+    outs2 = {k: np.full(4 * V, np.nan, np.float32) for k in OUT}
+    h = N.RctHost(num_nets=Nn, total_num_nodes=V, rct_nodes_start=N.addr(f.nodes_start),
+                  rct_pid=N.addr(bufs["pid"]), pres=N.addr(pres), cap=N.addr(cap),
+                  load=N.addr(outs2["load"]), delay=N.addr(outs2["delay"]),
+                  ldelay=N.addr(outs2["ldelay"]), impulse=N.addr(outs2["impulse"]),
+                  beta=N.addr(outs2["beta"]), slew=N.addr(outs2["slew"]),
+                  slew_in=N.addr(f.slew_in))
+    ctx.check(N.lib.otg_rct_compute(ctx.ptr, C.byref(h), N.BATCHED_COMPENSATED))
+    for k in OUT:
+        assert np.array_equal(outs2[k].view(np.uint32), ref[k].reshape(-1).view(np.uint32)), k
+
+
 @pytest.mark.parametrize("method", ["plain", "batched", "compensated"])
 def test_rct_compute_host_api(ctx, method):
     """otg_rct_compute: host-pointer drop-in for rct_compute_cuda(RctCUDA)."""
