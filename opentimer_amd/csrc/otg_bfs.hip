@@ -1014,6 +1014,7 @@ __global__ __launch_bounds__(T) void rct_bfs_tour_kernel(BfsParams P, const int3
 // Size classes of the on-chip kernel, and the class of the nets larger than
 // kMaxLds (global scratch).
 constexpr int kClasses = 5;
+constexpr int kBfsAux = otg_context::kAux;  // auxiliary streams used (small nets on the first)
 constexpr int kClassCap[kClasses] = {1024, 2048, 4096, 8192, 0};
 
 __device__ __forceinline__ int bfs_class(int n) {
@@ -1115,10 +1116,39 @@ otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t
   P.bad = d_bad;
   const int N = in->num_nets;
   const int ncu = std::max(1, ctx->num_cus);
-  // small nets: packed into wavefronts, independent of the large-net work
+  // The size classes are independent (disjoint nets and output ranges): the
+  // small nets run on an auxiliary stream forked from `s` while the large
+  // nets are counted and listed, and the large classes each take a stream of
+  // their own, so that one class's last nets do not leave the CUs idle until
+  // the next class starts.  All of it is joined back into `s`.
+  for (int k = 0; k < kBfsAux; ++k) {
+    if (!ctx->aux[k]) {
+      OTG_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
+      OTG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join[k], hipEventDisableTiming));
+    }
+  }
+  if (!ctx->ev_fork) OTG_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+  OTG_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
+  for (int k = 0; k < kBfsAux; ++k) OTG_HIP(ctx, hipStreamWaitEvent(ctx->aux[k], ctx->ev_fork, 0));
+  auto join = [&]() {
+    hipError_t r = hipSuccess;
+    for (int k = 0; k < kBfsAux; ++k) {
+      const hipError_t e1 = hipEventRecord(ctx->ev_join[k], ctx->aux[k]);
+      const hipError_t e2 = hipStreamWaitEvent(s, ctx->ev_join[k], 0);
+      if (r == hipSuccess) r = e1 != hipSuccess ? e1 : e2;
+    }
+    return r;
+  };
+  // small nets: packed into wavefronts
   const int sgrid = std::max(1, std::min((N + kSmallRange - 1) / kSmallRange, ncu * 16));
-  hipLaunchKernelGGL(rct_bfs_small_kernel, dim3(sgrid), dim3(64), 0, s, P);
-  OTG_HIP(ctx, hipGetLastError());
+  hipLaunchKernelGGL(rct_bfs_small_kernel, dim3(sgrid), dim3(64), 0, ctx->aux[0], P);
+  {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      (void)join();
+      OTG_HIP(ctx, e);
+    }
+  }
   // large nets: count by class, list, then one launch per class
   int32_t* ctl = nullptr;
   OTG_HIP(ctx, hipMallocAsync((void**)&ctl, sizeof(int32_t) * 32, s));
@@ -1129,6 +1159,7 @@ otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t
   if (e == hipSuccess) e = hipMemcpyAsync(h, ctl, sizeof(h), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
+    (void)join();
     (void)hipFreeAsync(ctl, s);
     OTG_HIP(ctx, e);
   }
@@ -1149,6 +1180,12 @@ otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t
     L.list = list;
     if (le == hipSuccess)
       hipLaunchKernelGGL(rct_bfs_list_kernel, dim3((N + 255) / 256), dim3(256), 0, s, P, ctl, L);
+    // the lists are ready: the classes below the largest go to the other
+    // auxiliary streams (largest first, round robin)
+    if (le == hipSuccess) le = hipEventRecord(ctx->ev_fork, s);
+    for (int k = 1; k < kBfsAux && le == hipSuccess; ++k)
+      le = hipStreamWaitEvent(ctx->aux[k], ctx->ev_fork, 0);
+    int launched = 0;
     // largest class first
     for (int c = kClasses - 2; c >= 0 && le == hipSuccess; --c) {
       if (h[c] <= 0) continue;
@@ -1171,7 +1208,9 @@ otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t
       int32_t cnt = h[c];
       int32_t* head = ctl + 10 + c;
       void* args[] = {&P, &lc, &cnt, &head};
-      le = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s);
+      hipStream_t cs = launched == 0 ? s : ctx->aux[1 + (launched - 1) % (kBfsAux - 1)];
+      le = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, cs);
+      ++launched;
     }
     const int c4 = kClasses - 1;
     if (h[c4] > 0 && le == hipSuccess) {
@@ -1186,6 +1225,10 @@ otg_status otg_rct_bfs_launch(otg_context* ctx, const otg_rct_edges* in, int32_t
     (void)nodes4;
   }
   if (le == hipSuccess) le = hipGetLastError();
+  {
+    const hipError_t je = join();
+    if (le == hipSuccess) le = je;
+  }
   if (gs) (void)hipFreeAsync(gs, s);
   if (list) (void)hipFreeAsync(list, s);
   (void)hipFreeAsync(ctl, s);
