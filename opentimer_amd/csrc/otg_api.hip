@@ -368,20 +368,30 @@ extern "C" otg_status otg_rct_last_breakdown(const otg_context* ctx, double ms[8
 // otherwise staged by the runtime one chunk at a time, ~17 GB/s device to
 // host measured on MI355X).  Synchronous: the device side of `dst` / `src`
 // must be ready (the caller synchronises stream `s` first).  Copies below
-// four chunks are queued on `s` unchanged.
+// four chunks, and every copy if the pinned buffers cannot be allocated, are
+// queued on `s` unchanged.
 static otg_status staged_copy(otg_context* ctx, void* dst, const void* src, size_t bytes,
                               hipMemcpyKind kind, hipStream_t s) {
   constexpr int T = otg_context::kStage;
   constexpr size_t C = otg_context::kStageBytes;
-  if (bytes < 4 * C) {
+  bool staged = bytes >= 4 * C;
+  for (int t = 0; t < T && staged; ++t) {
+    if (!ctx->stage_stream[t] &&
+        hipStreamCreateWithFlags(&ctx->stage_stream[t], hipStreamNonBlocking) != hipSuccess) {
+      ctx->stage_stream[t] = nullptr;
+      staged = false;
+    }
+    for (void*& b : ctx->stage_buf[t])
+      if (staged && !b && hipHostMalloc(&b, C, hipHostMallocDefault) != hipSuccess) {
+        b = nullptr;
+        staged = false;
+      }
+  }
+  if (!staged) {
+    // small copies, or no pinned memory to be had: the runtime's own staging
+    (void)hipGetLastError();
     OTG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind, s));
     return OTG_OK;
-  }
-  for (int t = 0; t < T; ++t) {
-    if (!ctx->stage_stream[t])
-      OTG_HIP(ctx, hipStreamCreateWithFlags(&ctx->stage_stream[t], hipStreamNonBlocking));
-    for (void*& b : ctx->stage_buf[t])
-      if (!b) OTG_HIP(ctx, hipHostMalloc(&b, C, hipHostMallocDefault));
   }
   const bool d2h = kind == hipMemcpyDeviceToHost;
   const size_t share = ((bytes + T - 1) / T + 4095) & ~size_t(4095);
