@@ -22,9 +22,9 @@
  *     stream (NULL = the device's default stream) and use device pointers; the
  *     reference-shaped entry points (otg_rct_bfs_and_compute,
  *     otg_toposort_compute) take host pointers and are synchronous, like the
- *     functions they replace.  otg_rct_bfs_and_compute and otg_rct_compute
- *     move arrays above 32 MB through pinned buffers with 8 host threads
- *     (128 MB of pinned memory held by the context after first use);
+ *     functions they replace.  They move arrays above 32 MB through pinned
+ *     buffers with 8 host threads (128 MB of pinned memory held by the
+ *     context after first use);
  *   - a context is not thread-safe; calls that overlap on one context race.
  */
 #ifndef OTG_H_

@@ -642,18 +642,24 @@ extern "C" otg_status otg_toposort_compute(otg_context* ctx, int32_t n, int32_t 
   OTG_HIP(ctx, B.alloc(&fr_d, (size_t)n));
   OTG_HIP(ctx, B.alloc(&ends_d, (size_t)n + 2));
   if (level_of_pin) OTG_HIP(ctx, B.alloc(&lv_d, (size_t)n));
-  OTG_HIP(ctx, hipMemcpyAsync(st_d, edgelist_start, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
-  if (num_edges)
-    OTG_HIP(ctx, hipMemcpyAsync(el_d, edgelist, sizeof(int32_t) * num_edges, hipMemcpyHostToDevice, s));
-  OTG_HIP(ctx, hipMemcpyAsync(out_d, out, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-  OTG_HIP(ctx, hipMemcpyAsync(fr_d, frontiers, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  otg_status st = OTG_OK;
+  auto copy = [&](void* dst, const void* src, size_t count, hipMemcpyKind kind) {
+    return staged_copy(ctx, dst, src, sizeof(int32_t) * count, kind, s);
+  };
+  constexpr hipMemcpyKind H2D = hipMemcpyHostToDevice, D2H = hipMemcpyDeviceToHost;
+  if ((st = copy(st_d, edgelist_start, (size_t)n + 1, H2D)) != OTG_OK ||
+      (num_edges && (st = copy(el_d, edgelist, (size_t)num_edges, H2D)) != OTG_OK) ||
+      (st = copy(out_d, out, (size_t)n, H2D)) != OTG_OK ||
+      (st = copy(fr_d, frontiers, (size_t)n, H2D)) != OTG_OK)
+    return st;
   int32_t ne = 0;
-  otg_status st = otg_levelize(ctx, n, num_edges, st_d, el_d, out_d, fr_d, first_size, fr_d, ends_d,
-                               lv_d, &ne, s);
+  st = otg_levelize(ctx, n, num_edges, st_d, el_d, out_d, fr_d, first_size, fr_d, ends_d, lv_d, &ne, s);
   if (st != OTG_OK) return st;
-  OTG_HIP(ctx, hipMemcpyAsync(frontiers, fr_d, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
-  OTG_HIP(ctx, hipMemcpyAsync(frontiers_ends, ends_d, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, s));
-  if (lv_d) OTG_HIP(ctx, hipMemcpyAsync(level_of_pin, lv_d, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  OTG_HIP(ctx, hipStreamSynchronize(s));
+  if ((st = copy(frontiers, fr_d, (size_t)n, D2H)) != OTG_OK ||
+      (st = copy(frontiers_ends, ends_d, (size_t)ne, D2H)) != OTG_OK ||
+      (lv_d && (st = copy(level_of_pin, lv_d, (size_t)n, D2H)) != OTG_OK))
+    return st;
   OTG_HIP(ctx, hipStreamSynchronize(s));
   *num_ends = ne;
   return OTG_OK;

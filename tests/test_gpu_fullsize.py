@@ -163,6 +163,14 @@ def test_cfg2_levelize_full(ctx):
     lvl_of_pos = np.repeat(np.arange(len(ends) - 1, dtype=np.int32), np.diff(ends))
     assert np.array_equal(olv[fr], lvl_of_pos)
     assert np.array_equal(np.sort(fr), np.arange(csr.n, dtype=np.int32))
+    # the host-pointer drop-in on the same graph (its 40-100 MB arrays go
+    # through the library's staged pinned copies)
+    from opentimer_amd.levelize import toposort_compute
+    hfr, hends, hlv = toposort_compute(ctx, csr)
+    assert np.array_equal(hends, oends)
+    assert np.array_equal(hlv, olv)
+    assert np.array_equal(olv[hfr], lvl_of_pos)
+    assert np.array_equal(np.sort(hfr), np.arange(csr.n, dtype=np.int32))
 
 
 # ---------------------------------------------------------------------------
