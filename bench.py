@@ -212,6 +212,17 @@ def run_levelize(ctx, stream, sh, reps=5):
     _, oends, olv = O.levelize(csr.n, csr.start, csr.fanin, csr.out_degree, csr.frontier)
     serial_ms = (time.perf_counter() - t0) * 1e3
     lev["levels_bitexact_vs_oracle"] = bool(np.array_equal(lv, olv) and np.array_equal(ends, oends))
+    # the host-pointer drop-in (otg_toposort_compute, what toposort_compute_cuda
+    # is replaced by): copies in, levelization, copies out; second of two calls
+    from opentimer_amd.levelize import toposort_compute
+    for _ in range(2):
+        t0 = time.perf_counter()
+        _, hends, hlv = toposort_compute(ctx, csr)
+        host_ms = (time.perf_counter() - t0) * 1e3
+    lev["dropin"] = {"ms": host_ms, "levels_bitexact_vs_oracle":
+                     bool(np.array_equal(hlv, olv) and np.array_equal(hends, oends)),
+                     "what": "otg_toposort_compute from pageable host arrays (fanin CSR, fanout "
+                             "counts, seeded frontier in; frontiers, ends and per-pin levels out)"}
     # CPU baselines: the sink-first Kahn serially (the run above) and over the
     # host cores (median of 3 runs, levels checked against the serial run)
     cores, aff = host_threads()
