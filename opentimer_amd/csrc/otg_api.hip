@@ -370,8 +370,8 @@ extern "C" otg_status otg_rct_last_breakdown(const otg_context* ctx, double ms[8
 // must be ready (the caller synchronises stream `s` first).  Copies below
 // four chunks, and every copy if the pinned buffers cannot be allocated, are
 // queued on `s` unchanged.
-static otg_status staged_copy(otg_context* ctx, void* dst, const void* src, size_t bytes,
-                              hipMemcpyKind kind, hipStream_t s) {
+static hipError_t staged_copy_raw(otg_context* ctx, void* dst, const void* src, size_t bytes,
+                                  hipMemcpyKind kind, hipStream_t s) {
   constexpr int T = otg_context::kStage;
   constexpr size_t C = otg_context::kStageBytes;
   bool staged = bytes >= 4 * C;
@@ -390,8 +390,7 @@ static otg_status staged_copy(otg_context* ctx, void* dst, const void* src, size
   if (!staged) {
     // small copies, or no pinned memory to be had: the runtime's own staging
     (void)hipGetLastError();
-    OTG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind, s));
-    return OTG_OK;
+    return hipMemcpyAsync(dst, src, bytes, kind, s);
   }
   const bool d2h = kind == hipMemcpyDeviceToHost;
   const size_t share = ((bytes + T - 1) / T + 4095) & ~size_t(4095);
@@ -434,9 +433,35 @@ static otg_status staged_copy(otg_context* ctx, void* dst, const void* src, size
     });
   }
   for (std::thread& th : pool) th.join();
-  for (int t = 0; t < T; ++t) OTG_HIP(ctx, err[t]);
+  for (int t = 0; t < T; ++t)
+    if (err[t] != hipSuccess) return err[t];
+  return hipSuccess;
+}
+
+static otg_status staged_copy(otg_context* ctx, void* dst, const void* src, size_t bytes,
+                              hipMemcpyKind kind, hipStream_t s) {
+  OTG_HIP(ctx, staged_copy_raw(ctx, dst, src, bytes, kind, s));
   return OTG_OK;
 }
+
+// Device-to-host copies run by a host thread beside the caller's further
+// work (staged_copy_raw, which leaves the context's error message alone);
+// joined by wait() or, on every early return, by the destructor.
+struct BackgroundCopies {
+  std::thread th;
+  hipError_t err = hipSuccess;
+  template <class F>
+  void start(F f) {
+    th = std::thread([this, f]() { err = f(); });
+  }
+  hipError_t wait() {
+    if (th.joinable()) th.join();
+    return err;
+  }
+  ~BackgroundCopies() {
+    if (th.joinable()) th.join();
+  }
+};
 
 extern "C" otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_array* d,
                                               otg_rct_method method) {
@@ -492,18 +517,32 @@ extern "C" otg_status otg_rct_bfs_and_compute(otg_context* ctx, otg_rct_edge_arr
     st = otg_rct_bfs(ctx, &in, n2b, pid, ed, pres, cap, s);
     if (st != OTG_OK) return st;
     OTG_HIP(ctx, clk.mark(2));
+    // the ordering's outputs go back to the host while the plan is built and
+    // the forest solved (neither writes them; the thread is joined before
+    // the device buffers are released, on every path)
+    BackgroundCopies bg;
+    bg.start([=]() {
+      hipError_t e = hipSetDevice(ctx->device);  // staging streams are made on the current device
+      if (e == hipSuccess)
+        e = staged_copy_raw(ctx, d->rct_edges, ed, sizeof(int32_t) * 2 * E, hipMemcpyDeviceToHost,
+                            s);
+      if (e == hipSuccess)
+        e = staged_copy_raw(ctx, d->rct_node2bfs_order, n2b, sizeof(int32_t) * V,
+                            hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess)
+        e = staged_copy_raw(ctx, d->rct_pid, pid, sizeof(int32_t) * V, hipMemcpyDeviceToHost, s);
+      return e;
+    });
     otg_rct_tree tree{d->num_nets, d->total_num_nodes, ns, pid, pres, cap, si};
     otg_rct_result out{load, delay, ldelay, beta, imp, slew};
     st = solve_timed(ctx, &tree, &out, method, s, clk);
     if (st != OTG_OK) return st;
     OTG_HIP(ctx, hipStreamSynchronize(s));
+    OTG_HIP(ctx, bg.wait());
     auto d2h = [&](void* dst, const void* src, size_t bytes) {
       return staged_copy(ctx, dst, src, bytes, hipMemcpyDeviceToHost, s);
     };
-    if ((st = d2h(d->rct_edges, ed, sizeof(int32_t) * 2 * E)) != OTG_OK ||
-        (st = d2h(d->rct_node2bfs_order, n2b, sizeof(int32_t) * V)) != OTG_OK ||
-        (st = d2h(d->rct_pid, pid, sizeof(int32_t) * V)) != OTG_OK ||
-        (st = d2h(d->load, load, sizeof(float) * 4 * V)) != OTG_OK ||
+    if ((st = d2h(d->load, load, sizeof(float) * 4 * V)) != OTG_OK ||
         (st = d2h(d->delay, delay, sizeof(float) * 4 * V)) != OTG_OK ||
         (st = d2h(d->ldelay, ldelay, sizeof(float) * 4 * V)) != OTG_OK ||
         (st = d2h(d->impulse, imp, sizeof(float) * 4 * V)) != OTG_OK ||
